@@ -1,0 +1,158 @@
+/*
+ * ytrc.h -- C ABI of libytrc.so, the MI355X (gfx950) brick ray-caster.
+ *
+ * The reference's boundary for this path is the ImageSampler protocol of
+ * yt/utilities/lib/image_samplers.pyx:
+ *   ImageSampler(vp_pos, vp_dir, center, bounds, image, x_vec, y_vec, width,
+ *                volume_method, ...)            image_samplers.pyx:59-130
+ *   sampler(pg: PartitionedGrid, num_threads)   image_samplers.pyx:132-197
+ *   sampler.aimage                              image_samplers.pxd:52
+ * with bricks described by PartitionedGrid / VolumeContainer
+ * (partitioned_grid.pyx:28-63, volume_container.pxd:13-33).
+ *
+ * This ABI splits that protocol into: a device-resident dataset of bricks
+ * (the PartitionedGrid data the sampler is called with), a request (the
+ * sampler constructor arguments plus the brick call order), and a batch of
+ * requests that runs on the GPU and is read back into the caller's image.
+ *
+ * All functions return 0 on success and a negative YTRC_E* code on error;
+ * ytrc_last_error() returns a thread-local message for the last error.
+ * Plain pointers and sizes only.  Host pointers unless stated otherwise.
+ */
+#ifndef YTRC_H
+#define YTRC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YTRC_OK 0
+#define YTRC_EINVAL (-1)    /* bad argument (the reference raises ValueError/RuntimeError) */
+#define YTRC_EDEVICE (-2)   /* HIP runtime error                                          */
+#define YTRC_EBUDGET (-3)   /* request would exceed the context's HBM budget              */
+#define YTRC_ENOTIMPL (-4)  /* e.g. unknown volume method (image_samplers.pyx:137-141)    */
+#define YTRC_ECANCELLED (-5)
+
+/* Sampler kinds: ProjectionSampler (image_samplers.pyx:327-343) and
+ * VolumeRenderSampler (image_samplers.pyx:405-494). */
+#define YTRC_PROJECTION 0
+#define YTRC_VOLUME_RENDER 1
+
+/* Lens: plane-parallel (lenses.pyx:22-55, :179-192). */
+#define YTRC_LENS_PLANE_PARALLEL 0
+
+/* Request flags. */
+#define YTRC_FLAG_RECORD 1u /* per-pixel visit count, visit hash and sample count */
+#define YTRC_FLAG_ERT 2u    /* early ray termination (VR only; changes op order)  */
+
+/* Batch flags. */
+#define YTRC_BATCH_PLAIN 0u      /* one kernel launch per request                    */
+#define YTRC_BATCH_PERSISTENT 1u /* one persistent launch draining a global tile queue */
+#define YTRC_BATCH_GRAPH 2u      /* capture the launch sequence into a hipGraph       */
+
+typedef struct ytrc_ctx ytrc_ctx;
+typedef struct ytrc_batch ytrc_batch;
+
+/* Context: one per device.  budget_bytes caps the device memory this context
+ * may hold (datasets + batches); 0 means no cap. */
+int ytrc_ctx_create(int32_t device, uint64_t budget_bytes, ytrc_ctx **out);
+void ytrc_ctx_destroy(ytrc_ctx *ctx);
+int ytrc_ctx_used_bytes(const ytrc_ctx *ctx, uint64_t *used, uint64_t *budget);
+const char *ytrc_last_error(void);
+
+/* Brick descriptor: PartitionedGrid(parent_grid_id, data, mask, left_edge,
+ * right_edge, dims) (partitioned_grid.pyx:28-63).  vertex_centred = 1 when
+ * each field holds (dims+1)^3 vertex values (AMRKDTree bricks for
+ * VolumeRenderSampler, amr_kdtree.py:360-367); 0 for dims^3 cell values
+ * (off_axis_projection bricks, off_axis_projection.py:506-522). */
+typedef struct {
+    double left_edge[3];
+    double right_edge[3];
+    int32_t dims[3];
+    int32_t vertex_centred;
+} ytrc_brick_desc;
+
+/* Copies n_bricks bricks to the device.  field_ptrs holds n_bricks*n_fields
+ * host pointers, brick-major, each a C-order array of src_dtype
+ * (0 = float64, 1 = float32) with dims or dims+1 points per axis.
+ * mask_ptrs: NULL, or n_bricks pointers to dims-shaped uint8 masks (a NULL
+ * entry means all ones).  Device data is float32, fields interleaved per
+ * point.  Replaces the per-call numpy views of PartitionedGrid. */
+int ytrc_dataset_upload(ytrc_ctx *ctx, int32_t n_bricks, const ytrc_brick_desc *bricks,
+                        int32_t n_fields, int32_t src_dtype, const void *const *field_ptrs,
+                        const uint8_t *const *mask_ptrs, int32_t *ds_id_out);
+int ytrc_dataset_free(ytrc_ctx *ctx, int32_t ds_id);
+int ytrc_dataset_bytes(const ytrc_ctx *ctx, int32_t ds_id, uint64_t *bytes);
+
+/* Transfer function: the FieldInterpolationTable set a VolumeRenderSampler
+ * builds from tf_obj (image_samplers.pyx:425-447,
+ * field_interpolation_tables.pxd:27-61).  At most 6 tables. */
+typedef struct {
+    int32_t n_fits;
+    int32_t nbins[6];
+    double bounds[12]; /* (x_bounds[0], x_bounds[1]) per table */
+    const double *y[6];
+    int32_t field_ids[6];
+    int32_t weight_field_ids[6];
+    int32_t weight_table_ids[6];
+    int32_t field_table_ids[6];
+    int32_t grey_opacity;
+} ytrc_tf_desc;
+
+/* One image: the ImageSampler constructor arguments (image_samplers.pyx:59-130,
+ * built by utils.py:55-139 from PlaneParallelLens._get_sampler_params,
+ * lens.py:83-112) plus the brick call order (the sequence of sampler(pg)
+ * calls, render_source.py:594-597 / off_axis_projection.py:506-524). */
+typedef struct {
+    int32_t kind;
+    int32_t lens;
+    int32_t nv[2];
+    double vp_pos[12];
+    double vp_dir[3];
+    double center[3];
+    double bounds[4];
+    double x_vec[3];
+    double y_vec[3];
+    double width[3];
+    int32_t n_samples;           /* VR sub-samples per cell (render_source.py:246) */
+    int32_t ds_id;
+    int32_t n_order;
+    const int32_t *order;        /* brick indices into the dataset, call order    */
+    const double *zbuffer;       /* nv0*nv1 or NULL (ones, utils.py:136)           */
+    const double *image_in;      /* nv0*nv1*4 initial image or NULL (zeros)        */
+    const ytrc_tf_desc *tf;      /* VR only                                        */
+    uint32_t flags;
+} ytrc_request;
+
+/* Builds a batch: validates requests, computes the per-brick pixel extents
+ * (lenses.pyx:22-55 with the +-1 clip of image_samplers.pyx:159-166) and
+ * stages every descriptor and output buffer in device memory. */
+int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
+                      uint32_t batch_flags, ytrc_batch **out);
+/* Enqueues the batch's kernels on the context stream; returns immediately. */
+int ytrc_batch_launch(ytrc_batch *b);
+/* Waits for the last launch; ms_out (may be NULL) gets the GPU time between
+ * the HIP events recorded around it on the same stream. */
+int ytrc_batch_sync(ytrc_batch *b, float *ms_out);
+/* Copies request i's image to image_out (float64, nv0*nv1*4, the reference's
+ * aimage layout); channels the sampler does not write keep image_in. */
+int ytrc_batch_read_image(ytrc_batch *b, int32_t i, double *image_out);
+/* Record mode only: per-pixel visit count, visit hash, sample count. */
+int ytrc_batch_read_record(ytrc_batch *b, int32_t i, int64_t *visits, uint64_t *hashes,
+                           int64_t *samples);
+/* Tuning: rays per tile edge (8x8 default) and waves per workgroup. */
+int ytrc_batch_set_launch(ytrc_batch *b, int32_t waves_per_group, int32_t groups_per_cu);
+/* Device bytes held by the batch (descriptors, extents, outputs). */
+int ytrc_batch_bytes(const ytrc_batch *b, uint64_t *bytes);
+void ytrc_batch_destroy(ytrc_batch *b);
+
+/* Average duration (ms) of the dominant kernel over the launches since
+ * batch creation, from HIP events on the launch stream. */
+int ytrc_batch_kernel_ms(const ytrc_batch *b, float *avg_ms, int32_t *n_launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* YTRC_H */

@@ -1,0 +1,203 @@
+"""GPU parity: the HIP ray-caster (through the C ABI) against the FP64 oracle.
+
+Bars (written into each test):
+  * cell-visit sequences: per-ray visit count and order-sensitive hash of
+    (brick, i, j, k, bits(enter_t), bits(exit_t)) bit-exact;
+  * projection images: relative L2 <= 1e-5 and max-abs/max <= 1e-5 vs FP64;
+  * volume-render images: relative L2 <= 1e-5 vs FP64.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from yt_amd import _lib, render
+from yt_amd.transfer_functions import ColorTransferFunction
+
+from .conftest import rel_l2
+from .helpers import (
+    c0_params,
+    oracle_proj_bricks,
+    oracle_vr_bricks,
+    small_amr,
+    tf_dict,
+    uniform_lognormal,
+    vr_tf_for,
+)
+
+pytestmark = pytest.mark.gpu
+
+PROJ_TOL = 1e-5
+VR_TOL = 1e-5
+
+
+def _run(ctx, specs, flags=_lib.BATCH_PLAIN):
+    b = _lib.Batch(ctx, specs, flags)
+    try:
+        b.run()
+        out = []
+        for i, s in enumerate(specs):
+            img = b.read_image(i)
+            rec = b.read_record(i) if s.record else None
+            out.append((img, rec))
+        return out
+    finally:
+        b.close()
+
+
+def _check_visits(rec, ovis, ohash, tag):
+    v, h, _ = rec
+    nd = int(np.sum(v != ovis))
+    nh = int(np.sum(h != ohash))
+    print(f"{tag}: total visits gpu={int(v.sum())} oracle={int(ovis.sum())} "
+          f"count-mismatch={nd} hash-mismatch={nh}", flush=True)
+    assert nd == 0 and nh == 0
+
+
+@pytest.mark.parametrize("res", [64, 256])
+def test_c0_projection(ctx, res):
+    """64^3 lognormal, normal (0.3,0.5,0.81), unweighted projection."""
+    ds = uniform_lognormal(64)
+    vol = render.ResidentVolume(ctx, ds, proj_fields="density")
+    try:
+        p = c0_params(res)
+        (img, rec), = _run(ctx, [vol.projection_request(p, record=True)])
+        oimg, ovis, ohash = oracle.cast(0, p, oracle_proj_bricks(ds, ["density"]), record=True)
+        _check_visits(rec, ovis, ohash, f"c0 res={res}")
+        e = rel_l2(img[..., 0], oimg[..., 0])
+        m = np.abs(img[..., 0] - oimg[..., 0]).max() / np.abs(oimg[..., 0]).max()
+        print(f"c0 res={res}: rel-L2={e:.3e} max-abs/max={m:.3e}", flush=True)
+        assert e <= PROJ_TOL and m <= PROJ_TOL
+    finally:
+        vol.free()
+
+
+def test_weighted_projection_multibrick(ctx):
+    """Weighted projection over an 8-grid uniform split (load_uniform_grid nprocs=8)."""
+    ds = uniform_lognormal(64, nprocs=8, with_temperature=True)
+    assert len(ds.grids) == 8
+    vol = render.ResidentVolume(ctx, ds, proj_fields="temperature", weight="density")
+    try:
+        p = c0_params(200, normal=(-0.41, 0.27, 0.6))
+        (img, rec), = _run(ctx, [vol.projection_request(p, record=True)])
+        oimg, ovis, ohash = oracle.cast(
+            0, p, oracle_proj_bricks(ds, ["temperature"], weight="density"), record=True)
+        _check_visits(rec, ovis, ohash, "weighted 8 grids")
+        for c in (0, 1):
+            e = rel_l2(img[..., c], oimg[..., c])
+            print(f"weighted ch{c}: rel-L2={e:.3e}", flush=True)
+            assert e <= PROJ_TOL
+        g = render.finalize_projection(img, p["width"], True)
+        og = render.finalize_projection(oimg, p["width"], True)
+        assert rel_l2(g, og) <= PROJ_TOL
+    finally:
+        vol.free()
+
+
+def test_axis_aligned_and_zbuffer(ctx):
+    """Axis-aligned normal (zero v_dir components) and a per-pixel max_t."""
+    ds = uniform_lognormal(32, seed=11)
+    vol = render.ResidentVolume(ctx, ds, proj_fields="density")
+    try:
+        p = c0_params(96, normal=(0.0, 0.0, 1.0))
+        rng = np.random.default_rng(5)
+        zb = rng.uniform(-0.2, 1.3, size=(96, 96))
+        spec = vol.projection_request(p, record=True, zbuffer=zb)
+        (img, rec), = _run(ctx, [spec])
+        oimg, ovis, ohash = oracle.cast(0, p, oracle_proj_bricks(ds, ["density"]), zbuffer=zb,
+                                        record=True)
+        _check_visits(rec, ovis, ohash, "axis-aligned+zbuffer")
+        assert rel_l2(img[..., 0], oimg[..., 0]) <= PROJ_TOL
+    finally:
+        vol.free()
+
+
+def test_vr_amr(ctx):
+    """VolumeRenderSampler over kd bricks of a 2-level AMR set, 8-Gaussian CTF."""
+    ds = small_amr()
+    tf = vr_tf_for(ds)
+    vol = render.ResidentVolume(ctx, ds, vr_field="density", log_field=True)
+    try:
+        p = c0_params(128, normal=(0.2, -0.7, 0.5))
+        spec = vol.vr_request(p, tf, n_samples=10, record=True)
+        (img, rec), = _run(ctx, [spec])
+        bricks = oracle_vr_bricks(vol)
+        oimg, ovis, ohash = oracle.cast(1, p, bricks, order=spec.order, tf=tf_dict(tf),
+                                        n_samples=10, record=True)
+        _check_visits(rec, ovis, ohash, f"vr amr ({len(bricks)} bricks)")
+        for c in range(3):
+            e = rel_l2(img[..., c], oimg[..., c])
+            print(f"vr ch{c}: rel-L2={e:.3e} max={oimg[..., c].max():.3e}", flush=True)
+            assert e <= VR_TOL
+        assert np.array_equal(img[..., 3], oimg[..., 3])
+    finally:
+        vol.free()
+
+
+def test_vr_grey_opacity(ctx):
+    ds = uniform_lognormal(24, seed=3)
+    lo, hi = np.log10(ds.grids[0]["density"].min()), np.log10(ds.grids[0]["density"].max())
+    tf = ColorTransferFunction((lo, hi), nbins=64, grey_opacity=True)
+    tf.add_gaussian(0.5 * (lo + hi), 0.05 * (hi - lo) ** 2, [0.8, 0.4, 0.2, 3.0])
+    vol = render.ResidentVolume(ctx, ds, vr_field="density")
+    try:
+        p = c0_params(64, normal=(0.6, 0.3, -0.2))
+        spec = vol.vr_request(p, tf, n_samples=7, record=True)
+        (img, rec), = _run(ctx, [spec])
+        oimg, ovis, ohash = oracle.cast(1, p, oracle_vr_bricks(vol), order=spec.order,
+                                        tf=tf_dict(tf), n_samples=7, record=True)
+        _check_visits(rec, ovis, ohash, "vr grey")
+        for c in range(4):
+            assert rel_l2(img[..., c], oimg[..., c]) <= VR_TOL
+    finally:
+        vol.free()
+
+
+def test_batch_many_images(ctx):
+    """Several requests of different sizes and kinds in one batch give the
+    same images as each run alone."""
+    ds = uniform_lognormal(32, seed=21)
+    tf = vr_tf_for(ds, nbins=128)
+    vol = render.ResidentVolume(ctx, ds, proj_fields="density", vr_field="density")
+    try:
+        rng = np.random.default_rng(1004)
+        specs = []
+        for i in range(6):
+            n = rng.normal(size=3)
+            res = int(rng.choice([16, 40, 64, 100]))
+            p = c0_params(res, normal=tuple(n / np.linalg.norm(n)))
+            specs.append(vol.projection_request(p) if i % 2 == 0 else vol.vr_request(p, tf))
+        together = _run(ctx, specs)
+        for i, s in enumerate(specs):
+            (alone, _), = _run(ctx, [s])
+            assert np.array_equal(alone, together[i][0]), i
+    finally:
+        vol.free()
+
+
+def test_drop_in_off_axis_projection():
+    """yt_amd.off_axis_projection (ImageSampler protocol) against the oracle."""
+    import yt_amd
+
+    ds = uniform_lognormal(32, seed=5)
+    img = yt_amd.off_axis_projection(ds, (0.5, 0.5, 0.5), (0.3, 0.5, 0.81), 1.0, 64, "density")
+    p = c0_params(64)
+    oimg = oracle.cast(0, p, oracle_proj_bricks(ds, ["density"]))
+    ref = render.finalize_projection(oimg, p["width"], False)
+    assert img.shape == (64, 64)
+    assert rel_l2(img, ref) <= PROJ_TOL
+
+
+def test_sqrt3_chord(ctx):
+    """Closed form (test_offaxisprojection_pytestonly.py:518-552): projecting
+    ones along (1,1,1) through the unit cube peaks at sqrt(3)."""
+    ds = uniform_lognormal(16, seed=1)
+    ds.grids[0].fields["ones"] = np.ones((16, 16, 16))
+    vol = render.ResidentVolume(ctx, ds, proj_fields="ones")
+    try:
+        p = c0_params(65, normal=(1.0, 1.0, 1.0), width=np.sqrt(3.0))
+        (img, _), = _run(ctx, [vol.projection_request(p)])
+        out = render.finalize_projection(img, p["width"], False)
+        assert abs(out.max() - np.sqrt(3.0)) < 1e-2
+    finally:
+        vol.free()

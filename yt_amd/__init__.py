@@ -1,0 +1,42 @@
+"""yt_amd: an MI355X-native implementation of yt's brick ray-caster.
+
+The hot path -- walk_volume's DDA feeding ProjectionSampler and
+VolumeRenderSampler -- runs as hand-written HIP kernels for gfx950 in
+libytrc.so (C ABI: include/ytrc.h, bound with ctypes in yt_amd._lib).  The
+Python modules mirror the reference's host-side interface for that path:
+off_axis_projection, the ImageSampler classes, PartitionedGrid, AMRKDTree,
+the camera/lens parameters and the transfer functions.
+"""
+
+from . import _lib
+from .amr_kdtree import AMRKDTree
+from .camera import Camera, Orientation, PlaneParallelLens, off_axis_camera
+from .data import load_amr_grids, load_uniform_grid
+from .image_samplers import ImageSampler, ProjectionSampler, VolumeRenderSampler
+from .off_axis_projection import off_axis_projection
+from .partitioned_grid import PartitionedGrid
+from .transfer_functions import (
+    ColorTransferFunction,
+    MultiVariateTransferFunction,
+    ProjectionTransferFunction,
+    TransferFunction,
+)
+
+__all__ = [
+    "AMRKDTree",
+    "Camera",
+    "ColorTransferFunction",
+    "ImageSampler",
+    "MultiVariateTransferFunction",
+    "Orientation",
+    "PartitionedGrid",
+    "PlaneParallelLens",
+    "ProjectionSampler",
+    "ProjectionTransferFunction",
+    "TransferFunction",
+    "VolumeRenderSampler",
+    "load_amr_grids",
+    "load_uniform_grid",
+    "off_axis_camera",
+    "off_axis_projection",
+]

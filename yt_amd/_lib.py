@@ -1,0 +1,423 @@
+"""ctypes binding of libytrc.so (include/ytrc.h).
+
+The product path has no CPU fallback: importing this module succeeds without
+the library, but every entry point raises YtrcLibraryError when libytrc.so
+is absent or cannot be loaded.
+"""
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("YTRC_LIB", os.path.join(HERE, "libytrc.so"))
+
+YTRC_OK = 0
+YTRC_EINVAL = -1
+YTRC_EDEVICE = -2
+YTRC_EBUDGET = -3
+YTRC_ENOTIMPL = -4
+YTRC_ECANCELLED = -5
+
+PROJECTION = 0
+VOLUME_RENDER = 1
+LENS_PLANE_PARALLEL = 0
+
+FLAG_RECORD = 1
+FLAG_ERT = 2
+
+BATCH_PLAIN = 0
+BATCH_PERSISTENT = 1
+BATCH_GRAPH = 2
+
+EXPORTED = [
+    "ytrc_ctx_create",
+    "ytrc_ctx_destroy",
+    "ytrc_ctx_used_bytes",
+    "ytrc_last_error",
+    "ytrc_dataset_upload",
+    "ytrc_dataset_free",
+    "ytrc_dataset_bytes",
+    "ytrc_batch_create",
+    "ytrc_batch_launch",
+    "ytrc_batch_sync",
+    "ytrc_batch_read_image",
+    "ytrc_batch_read_record",
+    "ytrc_batch_set_launch",
+    "ytrc_batch_bytes",
+    "ytrc_batch_destroy",
+    "ytrc_batch_kernel_ms",
+]
+
+
+class YtrcLibraryError(RuntimeError):
+    """libytrc.so is missing or failed to load: there is no CPU fallback."""
+
+
+class YtrcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ytrc error {code}: {msg}")
+        self.code = code
+
+
+class YtrcBudgetError(YtrcError):
+    """The request does not fit the context's HBM budget."""
+
+
+class BrickDesc(C.Structure):
+    _fields_ = [
+        ("left_edge", C.c_double * 3),
+        ("right_edge", C.c_double * 3),
+        ("dims", C.c_int32 * 3),
+        ("vertex_centred", C.c_int32),
+    ]
+
+
+class TFDesc(C.Structure):
+    _fields_ = [
+        ("n_fits", C.c_int32),
+        ("nbins", C.c_int32 * 6),
+        ("bounds", C.c_double * 12),
+        ("y", C.POINTER(C.c_double) * 6),
+        ("field_ids", C.c_int32 * 6),
+        ("weight_field_ids", C.c_int32 * 6),
+        ("weight_table_ids", C.c_int32 * 6),
+        ("field_table_ids", C.c_int32 * 6),
+        ("grey_opacity", C.c_int32),
+    ]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("lens", C.c_int32),
+        ("nv", C.c_int32 * 2),
+        ("vp_pos", C.c_double * 12),
+        ("vp_dir", C.c_double * 3),
+        ("center", C.c_double * 3),
+        ("bounds", C.c_double * 4),
+        ("x_vec", C.c_double * 3),
+        ("y_vec", C.c_double * 3),
+        ("width", C.c_double * 3),
+        ("n_samples", C.c_int32),
+        ("ds_id", C.c_int32),
+        ("n_order", C.c_int32),
+        ("order", C.POINTER(C.c_int32)),
+        ("zbuffer", C.POINTER(C.c_double)),
+        ("image_in", C.POINTER(C.c_double)),
+        ("tf", C.POINTER(TFDesc)),
+        ("flags", C.c_uint32),
+    ]
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def _declare(lib):
+    P = C.c_void_p
+    lib.ytrc_ctx_create.argtypes = [C.c_int32, C.c_uint64, C.POINTER(P)]
+    lib.ytrc_ctx_destroy.argtypes = [P]
+    lib.ytrc_ctx_destroy.restype = None
+    lib.ytrc_ctx_used_bytes.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    lib.ytrc_last_error.restype = C.c_char_p
+    lib.ytrc_last_error.argtypes = []
+    lib.ytrc_dataset_upload.argtypes = [
+        P, C.c_int32, C.POINTER(BrickDesc), C.c_int32, C.c_int32,
+        C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_int32)]
+    lib.ytrc_dataset_free.argtypes = [P, C.c_int32]
+    lib.ytrc_dataset_bytes.argtypes = [P, C.c_int32, C.POINTER(C.c_uint64)]
+    lib.ytrc_batch_create.argtypes = [P, C.c_int32, C.POINTER(Request), C.c_uint32, C.POINTER(P)]
+    lib.ytrc_batch_launch.argtypes = [P]
+    lib.ytrc_batch_sync.argtypes = [P, C.POINTER(C.c_float)]
+    lib.ytrc_batch_read_image.argtypes = [P, C.c_int32, C.POINTER(C.c_double)]
+    lib.ytrc_batch_read_record.argtypes = [
+        P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_uint64), C.POINTER(C.c_int64)]
+    lib.ytrc_batch_set_launch.argtypes = [P, C.c_int32, C.c_int32]
+    lib.ytrc_batch_bytes.argtypes = [P, C.POINTER(C.c_uint64)]
+    lib.ytrc_batch_destroy.argtypes = [P]
+    lib.ytrc_batch_destroy.restype = None
+    lib.ytrc_batch_kernel_ms.argtypes = [P, C.POINTER(C.c_float), C.POINTER(C.c_int32)]
+    for name in EXPORTED:
+        f = getattr(lib, name)
+        if name not in ("ytrc_ctx_destroy", "ytrc_batch_destroy", "ytrc_last_error"):
+            f.restype = C.c_int
+
+
+def lib():
+    """Loads libytrc.so once; raises YtrcLibraryError if it is unavailable."""
+    global _lib
+    with _lib_lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise YtrcLibraryError(
+                    f"{LIB_PATH} not found: build it with yt_amd.build.build() "
+                    "(there is no CPU fallback for the ray-caster)")
+            try:
+                handle = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+            except OSError as e:
+                raise YtrcLibraryError(f"cannot load {LIB_PATH}: {e}") from e
+            _declare(handle)
+            _lib = handle
+        return _lib
+
+
+def _check(rc):
+    if rc != YTRC_OK:
+        msg = lib().ytrc_last_error().decode(errors="replace")
+        if rc == YTRC_EBUDGET:
+            raise YtrcBudgetError(rc, msg)
+        if rc == YTRC_ENOTIMPL:
+            raise NotImplementedError(msg)
+        if rc == YTRC_EINVAL:
+            raise ValueError(msg)
+        raise YtrcError(rc, msg)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Context:
+    """One device context: a stream, resident datasets and an HBM budget."""
+
+    def __init__(self, device=0, budget_bytes=0):
+        self._h = C.c_void_p()
+        _check(lib().ytrc_ctx_create(int(device), int(budget_bytes), C.byref(self._h)))
+        self.device = device
+
+    def close(self):
+        if self._h:
+            lib().ytrc_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def used_bytes(self):
+        u, b = C.c_uint64(), C.c_uint64()
+        _check(lib().ytrc_ctx_used_bytes(self._h, C.byref(u), C.byref(b)))
+        return u.value, b.value
+
+    def upload(self, bricks, fields, masks=None):
+        """bricks: list of (left_edge, right_edge, dims, vertex_centred);
+        fields: list (per brick) of lists of arrays (float64 or float32, all
+        the same dtype); masks: None or list of uint8 arrays / None."""
+        n = len(bricks)
+        if n == 0:
+            raise ValueError("upload: no bricks")
+        nf = len(fields[0])
+        descs = (BrickDesc * n)()
+        keep = []
+        ptrs = (C.c_void_p * (n * nf))()
+        dt = None
+        for b, (le, re, dims, vc) in enumerate(bricks):
+            descs[b].left_edge[:] = [float(x) for x in le]
+            descs[b].right_edge[:] = [float(x) for x in re]
+            descs[b].dims[:] = [int(x) for x in dims]
+            descs[b].vertex_centred = int(bool(vc))
+            if len(fields[b]) != nf:
+                raise ValueError("upload: every brick needs the same number of fields")
+            shape = tuple(int(d) + int(bool(vc)) for d in dims)
+            for f, arr in enumerate(fields[b]):
+                a = np.asarray(arr)
+                if a.dtype not in (np.float64, np.float32):
+                    a = a.astype(np.float64)
+                if dt is None:
+                    dt = a.dtype
+                a = np.ascontiguousarray(a, dtype=dt)
+                if a.shape != shape:
+                    raise ValueError(f"upload: brick {b} field {f} has shape {a.shape}, expected {shape}")
+                keep.append(a)
+                ptrs[b * nf + f] = a.ctypes.data
+        mptr = None
+        if masks is not None:
+            mptr = (C.c_void_p * n)()
+            for b, m in enumerate(masks):
+                if m is None:
+                    mptr[b] = None
+                    continue
+                mm = np.ascontiguousarray(m, dtype=np.uint8)
+                if mm.shape != tuple(int(d) for d in bricks[b][2]):
+                    raise ValueError(f"upload: mask {b} has shape {mm.shape}")
+                keep.append(mm)
+                mptr[b] = mm.ctypes.data
+        ds = C.c_int32()
+        _check(lib().ytrc_dataset_upload(
+            self._h, n, descs, nf, 0 if dt == np.float64 else 1, ptrs,
+            C.cast(mptr, C.POINTER(C.c_void_p)) if mptr is not None else None, C.byref(ds)))
+        return ds.value
+
+    def free(self, ds_id):
+        _check(lib().ytrc_dataset_free(self._h, int(ds_id)))
+
+    def dataset_bytes(self, ds_id):
+        v = C.c_uint64()
+        _check(lib().ytrc_dataset_bytes(self._h, int(ds_id), C.byref(v)))
+        return v.value
+
+
+class TF:
+    """Host copy of a transfer function in the C-ABI layout."""
+
+    def __init__(self, tables, field_ids, weight_field_ids, weight_table_ids,
+                 field_table_ids, grey_opacity):
+        n = len(tables)
+        if n > 6:
+            raise ValueError("at most 6 field tables (image_samplers.pyx:430)")
+        self._ys = [np.ascontiguousarray(y, dtype=np.float64) for (y, _, _) in tables]
+        d = TFDesc()
+        d.n_fits = n
+        for i, (y, b0, b1) in enumerate(tables):
+            d.nbins[i] = len(self._ys[i])
+            d.bounds[2 * i] = float(b0)
+            d.bounds[2 * i + 1] = float(b1)
+            d.y[i] = _dptr(self._ys[i])
+        for i in range(6):
+            d.field_ids[i] = int(field_ids[i])
+            d.weight_field_ids[i] = int(weight_field_ids[i])
+            d.weight_table_ids[i] = int(weight_table_ids[i])
+            d.field_table_ids[i] = int(field_table_ids[i])
+        d.grey_opacity = int(bool(grey_opacity))
+        self.desc = d
+
+    @classmethod
+    def from_tf_obj(cls, tf_obj):
+        """Takes the attributes VolumeRenderSampler reads from tf_obj
+        (image_samplers.pyx:429-446)."""
+        n = tf_obj.n_field_tables
+        tables = [(tf_obj.tables[i].y, tf_obj.tables[i].x_bounds[0],
+                   tf_obj.tables[i].x_bounds[1]) for i in range(n)]
+        return cls(tables, tf_obj.field_ids, tf_obj.weight_field_ids, tf_obj.weight_table_ids,
+                   tf_obj.field_table_ids, getattr(tf_obj, "grey_opacity", 0))
+
+
+class RequestSpec:
+    """One image: sampler constructor parameters + brick order."""
+
+    def __init__(self, kind, params, ds_id, order, n_samples=10, tf=None, zbuffer=None,
+                 image_in=None, record=False, ert=False):
+        self.kind = kind
+        self.params = params
+        self.ds_id = ds_id
+        self.order = np.ascontiguousarray(order, dtype=np.int32)
+        self.n_samples = int(n_samples)
+        self.tf = tf
+        self.nv = (int(params["nv"][0]), int(params["nv"][1]))
+        self.zbuffer = None if zbuffer is None else np.ascontiguousarray(zbuffer, dtype=np.float64)
+        self.image_in = None if image_in is None else np.ascontiguousarray(image_in, dtype=np.float64)
+        self.record = record
+        self.ert = ert
+
+    def to_c(self):
+        r = Request()
+        p = self.params
+        r.kind = self.kind
+        r.lens = LENS_PLANE_PARALLEL
+        r.nv[:] = list(self.nv)
+        r.vp_pos[:] = [float(x) for x in np.ravel(p["vp_pos"])]
+        r.vp_dir[:] = [float(x) for x in np.ravel(p["vp_dir"])]
+        r.center[:] = [float(x) for x in np.ravel(p["center"])]
+        r.bounds[:] = [float(x) for x in p["bounds"]]
+        r.x_vec[:] = [float(x) for x in np.ravel(p["x_vec"])]
+        r.y_vec[:] = [float(x) for x in np.ravel(p["y_vec"])]
+        r.width[:] = [float(x) for x in np.ravel(p["width"])]
+        r.n_samples = self.n_samples
+        r.ds_id = self.ds_id
+        r.n_order = len(self.order)
+        r.order = self.order.ctypes.data_as(C.POINTER(C.c_int32))
+        if self.zbuffer is not None:
+            if self.zbuffer.size != self.nv[0] * self.nv[1]:
+                raise ValueError("zbuffer shape does not match the image")
+            r.zbuffer = _dptr(self.zbuffer)
+        if self.image_in is not None:
+            if self.image_in.size != self.nv[0] * self.nv[1] * 4:
+                raise ValueError("image shape does not match (nv0, nv1, 4)")
+            r.image_in = _dptr(self.image_in)
+        if self.tf is not None:
+            r.tf = C.pointer(self.tf.desc)
+        r.flags = (FLAG_RECORD if self.record else 0) | (FLAG_ERT if self.ert else 0)
+        return r
+
+
+class Batch:
+    """A set of requests staged on the device; launch() runs them."""
+
+    def __init__(self, ctx, specs, flags=BATCH_PLAIN):
+        self.ctx = ctx
+        self.specs = list(specs)
+        n = len(self.specs)
+        arr = (Request * n)()
+        for i, s in enumerate(self.specs):
+            arr[i] = s.to_c()
+        self._h = C.c_void_p()
+        _check(lib().ytrc_batch_create(ctx._h, n, arr, int(flags), C.byref(self._h)))
+
+    def set_launch(self, waves_per_group=4, groups_per_cu=0):
+        _check(lib().ytrc_batch_set_launch(self._h, int(waves_per_group), int(groups_per_cu)))
+
+    def launch(self):
+        _check(lib().ytrc_batch_launch(self._h))
+
+    def sync(self):
+        ms = C.c_float()
+        _check(lib().ytrc_batch_sync(self._h, C.byref(ms)))
+        return ms.value
+
+    def run(self):
+        self.launch()
+        return self.sync()
+
+    def kernel_ms(self):
+        ms, n = C.c_float(), C.c_int32()
+        _check(lib().ytrc_batch_kernel_ms(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def nbytes(self):
+        v = C.c_uint64()
+        _check(lib().ytrc_batch_bytes(self._h, C.byref(v)))
+        return v.value
+
+    def read_image(self, i):
+        nv = self.specs[i].nv
+        out = np.empty((nv[0], nv[1], 4), dtype=np.float64)
+        _check(lib().ytrc_batch_read_image(self._h, int(i), _dptr(out)))
+        return out
+
+    def read_record(self, i):
+        nv = self.specs[i].nv
+        v = np.empty(nv, dtype=np.int64)
+        h = np.empty(nv, dtype=np.uint64)
+        s = np.empty(nv, dtype=np.int64)
+        _check(lib().ytrc_batch_read_record(
+            self._h, int(i), v.ctypes.data_as(C.POINTER(C.c_int64)),
+            h.ctypes.data_as(C.POINTER(C.c_uint64)), s.ctypes.data_as(C.POINTER(C.c_int64))))
+        return v, h, s
+
+    def close(self):
+        if self._h:
+            lib().ytrc_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    """The process-wide context on device 0 used by the drop-in samplers."""
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(int(os.environ.get("YTRC_DEVICE", "0")),
+                               int(os.environ.get("YTRC_HBM_BUDGET", "0")))
+    return _default_ctx

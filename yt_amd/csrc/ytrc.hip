@@ -1,0 +1,1063 @@
+// ytrc.hip -- libytrc.so: MI355X (gfx950) brick ray-caster behind the C ABI
+// declared in include/ytrc.h.
+//
+// Kernels: one 64-lane wavefront casts one 8x8 tile of rays through the
+// bricks of a request in the reference's call order; per brick it applies
+// the reference's pixel-extent culling, the FP64 DDA (grid_traversal.pyx
+// walk_volume, restated bit-exactly in ytrc_device.h) and the sampler:
+//   ProjectionSampler.sample    image_samplers.pyx:330-343
+//   VolumeRenderSampler.sample  image_samplers.pyx:453-487
+//     + offset_interpolate      fixed_interpolator.cpp:34-52
+//     + FIT_eval_transfer       field_interpolation_tables.pxd:66-110
+// Data: float32 bricks, fields interleaved per point, packed in one pool.
+// Accumulation: projection in FP32 with Kahan compensation; VR compositing
+// state in FP64 (full-rate VALU on CDNA4), samples and table lookups FP32.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ytrc.h"
+#include "ytrc_device.h"
+
+using namespace ytrc;
+
+// ---------------------------------------------------------------------------
+// Device samplers
+
+namespace {
+
+template <int NFP>
+struct FieldLoad;
+template <>
+struct FieldLoad<1> {
+    static __device__ __forceinline__ void get(const float *p, float v[1]) { v[0] = p[0]; }
+};
+template <>
+struct FieldLoad<2> {
+    static __device__ __forceinline__ void get(const float *p, float v[2])
+    {
+        float2 t = *reinterpret_cast<const float2 *>(p);
+        v[0] = t.x;
+        v[1] = t.y;
+    }
+};
+template <>
+struct FieldLoad<4> {
+    static __device__ __forceinline__ void get(const float *p, float v[4])
+    {
+        float4 t = *reinterpret_cast<const float4 *>(p);
+        v[0] = t.x;
+        v[1] = t.y;
+        v[2] = t.z;
+        v[3] = t.w;
+    }
+};
+template <>
+struct FieldLoad<8> {
+    static __device__ __forceinline__ void get(const float *p, float v[8])
+    {
+        float4 a = *reinterpret_cast<const float4 *>(p);
+        float4 b = *reinterpret_cast<const float4 *>(p + 4);
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = a.z;
+        v[3] = a.w;
+        v[4] = b.x;
+        v[5] = b.y;
+        v[6] = b.z;
+        v[7] = b.w;
+    }
+};
+
+__device__ __forceinline__ void kahan_add(float &s, float &c, float x)
+{
+    float y = x - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// ProjectionSampler.sample (image_samplers.pyx:330-343): rgba[i] +=
+// data[i][cell] * (exit_t - enter_t) for i < min(4, n_fields).  No mask.
+template <int NFP, bool REC>
+struct ProjSampler {
+    const float *data;
+    int d1, d2;
+    int dims0;
+    float s[NFP];
+    float c[NFP];
+    Rec *rec;
+    __device__ __forceinline__ void operator()(double en, double ex, int c0, int c1, int c2)
+    {
+        if (REC) {
+            rec->visit(en, ex, c0, c1, c2);
+            if (ex > en) rec->samples += 1;
+        }
+        if (!((unsigned)c0 < (unsigned)dims0 && (unsigned)c1 < (unsigned)d1 &&
+              (unsigned)c2 < (unsigned)d2))
+            return;
+        const float dl = (float)(ex - en);
+        const int di = (c0 * d1 + c1) * d2 + c2;
+        float v[NFP];
+        FieldLoad<NFP>::get(data + (size_t)di * NFP, v);
+#pragma unroll
+        for (int f = 0; f < NFP; f++) kahan_add(s[f], c[f], v[f] * dl);
+    }
+};
+
+template <int N>
+__device__ __forceinline__ float pick(const float (&a)[N], int i)
+{
+    float r = a[0];
+#pragma unroll
+    for (int k = 1; k < N; k++) r = (i == k) ? a[k] : r;
+    return r;
+}
+
+// FIT_get_value + FIT_eval_transfer (field_interpolation_tables.pxd:66-110)
+// with the tables in LDS as float4 {y, dy, d0, 0}.  The reference's
+// sequential weight-table products (istorage[i] *= istorage[weight_table_id])
+// and the channel links (trgba[c] = istorage[field_table_ids[c]]) are
+// resolved on the host into chexp[c]: 2 bits per raw table u giving the power
+// of table u's value in channel c, so the device never indexes a register
+// array with a runtime value.
+template <int NFP>
+__device__ __forceinline__ void fit_eval(const DTF &tf, const float4 *__restrict__ lt,
+                                         const float (&dvs)[NFP], float e[4])
+{
+    float raw[6];
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        raw[t] = 0.0f;
+        if (t < tf.n_fits) {
+            const float v = (NFP == 1) ? dvs[0] : pick(dvs, tf.field_id[t]);
+            if (v < tf.b1[t] && v > tf.b0[t] && __builtin_isnormal(v)) {
+                int bin = (int)((v - tf.b0[t]) * tf.idbin[t]);
+                bin = bin < 0 ? 0 : (bin > tf.nbins[t] - 2 ? tf.nbins[t] - 2 : bin);
+                const float4 q = lt[tf.off[t] + bin];
+                float r = __builtin_fmaf(v - q.z, q.y, q.x);
+                if (tf.wfield[t] != -1) r *= (NFP == 1) ? dvs[0] : pick(dvs, tf.wfield[t]);
+                raw[t] = r;
+            }
+        }
+    }
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++) {
+        const int ce = tf.chexp[ch];
+        float r = (ce < 0) ? 0.0f : 1.0f;
+#pragma unroll
+        for (int u = 0; u < 6; u++) {
+            const int ex = (ce >> (2 * u)) & 3;
+            if (ex >= 1) r *= raw[u];
+            if (ex >= 2) r *= raw[u];
+            if (ex >= 3) r *= raw[u];
+        }
+        e[ch] = r;
+    }
+}
+
+// VolumeRenderSampler.sample (image_samplers.pyx:453-487).
+template <int NFP, bool REC>
+struct VRSampler {
+    const float *data;
+    const uint8_t *mask;
+    const DBrick *b;
+    double pos[3];
+    double dir[3];
+    const DTF *tf;
+    const float4 *lt;
+    double inv_ns;
+    int n_samples;
+    double rgba[4];
+    Rec *rec;
+    __device__ __forceinline__ void operator()(double en, double ex, int c0, int c1, int c2)
+    {
+        if (REC) rec->visit(en, ex, c0, c1, c2);
+        const int d0 = b->dims[0], d1 = b->dims[1], d2 = b->dims[2];
+        if (!((unsigned)c0 < (unsigned)d0 && (unsigned)c1 < (unsigned)d1 &&
+              (unsigned)c2 < (unsigned)d2))
+            return;
+        const int cell = (c0 * d1 + c1) * d2 + c2;
+        if (mask != nullptr && mask[cell] != 1) return;
+        if (REC) rec->samples += n_samples;
+        const int s1 = d2 + 1;
+        const int s0 = (d1 + 1) * (d2 + 1);
+        const int voff = c0 * s0 + c1 * s1 + c2;
+        // dt = (exit_t - enter_t) / n_samples; positions from FP64 enter_t.
+        const double dt = (ex - en) * inv_ns;
+        const double tt = en + 0.5 * dt;
+        float dp[3], ds[3];
+        {
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const int ci = (i == 0) ? c0 : ((i == 1) ? c1 : c2);
+                double p = tt * dir[i] + pos[i];
+                p -= (double)ci * b->dds[i] + b->le[i];
+                p *= b->idds[i];
+                dp[i] = (float)p;
+                ds[i] = (float)(dir[i] * b->idds[i] * dt);
+            }
+        }
+        float cv[8][NFP];
+        {
+            const float *p0 = data + (size_t)voff * NFP;
+            FieldLoad<NFP>::get(p0, cv[0]);
+            FieldLoad<NFP>::get(p0 + NFP, cv[1]);
+            FieldLoad<NFP>::get(p0 + (size_t)s1 * NFP, cv[2]);
+            FieldLoad<NFP>::get(p0 + (size_t)(s1 + 1) * NFP, cv[3]);
+            FieldLoad<NFP>::get(p0 + (size_t)s0 * NFP, cv[4]);
+            FieldLoad<NFP>::get(p0 + (size_t)(s0 + 1) * NFP, cv[5]);
+            FieldLoad<NFP>::get(p0 + (size_t)(s0 + s1) * NFP, cv[6]);
+            FieldLoad<NFP>::get(p0 + (size_t)(s0 + s1 + 1) * NFP, cv[7]);
+        }
+        const DTF &T = *tf;
+        for (int s = 0; s < n_samples; s++) {
+            float dvs[NFP];
+            const float dv2 = 1.0f - dp[2];
+            const float dv1 = 1.0f - dp[1];
+            const float dv0 = 1.0f - dp[0];
+#pragma unroll
+            for (int f = 0; f < NFP; f++) {
+                float z0 = __builtin_fmaf(dp[2], cv[1][f], dv2 * cv[0][f]);
+                float z1 = __builtin_fmaf(dp[2], cv[3][f], dv2 * cv[2][f]);
+                float z2 = __builtin_fmaf(dp[2], cv[5][f], dv2 * cv[4][f]);
+                float z3 = __builtin_fmaf(dp[2], cv[7][f], dv2 * cv[6][f]);
+                float y0 = __builtin_fmaf(dp[1], z1, dv1 * z0);
+                float y1 = __builtin_fmaf(dp[1], z3, dv1 * z2);
+                dvs[f] = __builtin_fmaf(dp[0], y1, dv0 * y0);
+            }
+            float e[4];
+            fit_eval<NFP>(T, lt, dvs, e);
+            if (T.grey) {
+                const double ta = y_fmax(1.0 - dt * (double)e[3], 0.0);
+#pragma unroll
+                for (int ch = 0; ch < 4; ch++) rgba[ch] = __builtin_fma(ta, rgba[ch], dt * (double)e[ch]);
+            } else {
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++) {
+                    const double a = dt * (double)e[ch];
+                    const double ta = y_fmax(1.0 - a, 0.0);
+                    rgba[ch] = __builtin_fma(ta, rgba[ch], a);
+                }
+            }
+            dp[0] = dp[0] + ds[0];
+            dp[1] = dp[1] + ds[1];
+            dp[2] = dp[2] + ds[2];
+        }
+    }
+};
+
+// Casts the rays of one 8x8 tile of image `im` through its bricks.
+template <int KIND, int NFP, bool REC>
+__device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
+                                            const float4 *lt)
+{
+    const int tx = tile / im.tiles1;
+    const int ty = tile - tx * im.tiles1;
+    const int ti0 = tx * TILE, tj0 = ty * TILE;
+    const int vi = ti0 + (lane >> 3);
+    const int vj = tj0 + (lane & 7);
+    const bool active = (vi < im.nv0) && (vj < im.nv1);
+    const int64_t npix = (int64_t)im.nv0 * im.nv1;
+    const int64_t pix = active ? (int64_t)vi * im.nv1 + vj : 0;
+
+    double pos[3], dir[3], iv[3];
+    ray_setup(im, vi, vj, pos, dir);
+#pragma unroll
+    for (int i = 0; i < 3; i++) iv[i] = (dir[i] != 0.0) ? 1.0 / dir[i] : 0.0;
+    double max_t = 1.0;
+    if (im.zbuf != nullptr && active) max_t = y_fmin(y_fmax(im.zbuf[pix], 0.0), 1.0);
+
+    Rec rec;
+    rec.visits = 0;
+    rec.samples = 0;
+    rec.hash = 0;
+    rec.brick = 0;
+
+    float ps[NFP], pc[NFP];
+    double rgba[4];
+#pragma unroll
+    for (int f = 0; f < NFP; f++) {
+        ps[f] = 0.0f;
+        pc[f] = 0.0f;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) rgba[c] = 0.0;
+    if (im.init != nullptr && active) {
+        if (KIND == 0) {
+#pragma unroll
+            for (int f = 0; f < NFP; f++)
+                if (f < 4) ps[f] = im.init[f * npix + pix];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; c++) rgba[c] = (double)im.init[c * npix + pix];
+        }
+    }
+
+    for (int k = 0; k < im.n_order; k++) {
+        const int4 e = im.ext[k];
+        if (e.y <= ti0 || e.x >= ti0 + TILE || e.w <= tj0 || e.z >= tj0 + TILE) continue;
+        if (!(active && vi >= e.x && vi < e.y && vj >= e.z && vj < e.w)) continue;
+        const int bid = im.order[k];
+        const DBrick &b = im.bricks[bid];
+        Walk w;
+        if (!walk_init(b, pos, dir, iv, max_t, w)) continue;
+        rec.brick = bid;
+        if (KIND == 0) {
+            ProjSampler<NFP, REC> s;
+            s.data = im.pool + b.data_off;
+            s.d1 = b.dims[1];
+            s.d2 = b.dims[2];
+            s.dims0 = b.dims[0];
+#pragma unroll
+            for (int f = 0; f < NFP; f++) {
+                s.s[f] = ps[f];
+                s.c[f] = pc[f];
+            }
+            s.rec = &rec;
+            walk_loop(b, w, max_t, s);
+#pragma unroll
+            for (int f = 0; f < NFP; f++) {
+                ps[f] = s.s[f];
+                pc[f] = s.c[f];
+            }
+        } else {
+            VRSampler<NFP, REC> s;
+            s.data = im.pool + b.data_off;
+            s.mask = (b.mask_off >= 0) ? im.masks + b.mask_off : nullptr;
+            s.b = &b;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                s.pos[i] = pos[i];
+                s.dir[i] = dir[i];
+            }
+            s.tf = im.tf;
+            s.lt = lt;
+            s.inv_ns = im.inv_ns;
+            s.n_samples = im.n_samples;
+#pragma unroll
+            for (int c = 0; c < 4; c++) s.rgba[c] = rgba[c];
+            s.rec = &rec;
+            walk_loop(b, w, max_t, s);
+#pragma unroll
+            for (int c = 0; c < 4; c++) rgba[c] = s.rgba[c];
+        }
+    }
+    if (!active) return;
+    if (KIND == 0) {
+#pragma unroll
+        for (int f = 0; f < NFP; f++)
+            if (f < im.nch_out) im.out[f * npix + pix] = ps[f] - pc[f];
+    } else {
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            if (c < im.nch_out) im.out[c * npix + pix] = (float)rgba[c];
+    }
+    if (REC) {
+        im.rec_visits[pix] = rec.visits;
+        im.rec_hash[pix] = rec.hash;
+        im.rec_samples[pix] = rec.samples;
+    }
+}
+
+// Plain launch: one request; blockDim.x/64 tiles per workgroup.
+template <int KIND, int NFP, bool REC>
+__global__ __launch_bounds__(256) void k_render(const DImg *__restrict__ imgs, int idx)
+{
+    extern __shared__ float4 s_tf[];
+    const DImg &im = imgs[idx];
+    if (KIND == 1) {
+        const DTF &T = *im.tf;
+        for (int i = threadIdx.x; i < T.total_bins; i += blockDim.x) s_tf[i] = T.table[i];
+        __syncthreads();
+    }
+    const int waves = blockDim.x >> 6;
+    const int tile = blockIdx.x * waves + (threadIdx.x >> 6);
+    if (tile >= im.tiles0 * im.tiles1) return;
+    render_tile<KIND, NFP, REC>(im, tile, threadIdx.x & 63, s_tf);
+}
+
+typedef void (*kfn_t)(const DImg *, int);
+
+template <int KIND, bool REC>
+kfn_t pick_kernel_nfp(int nfp)
+{
+    switch (nfp) {
+    case 1: return k_render<KIND, 1, REC>;
+    case 2: return k_render<KIND, 2, REC>;
+    case 4: return k_render<KIND, 4, REC>;
+    default: return KIND == 1 ? k_render<KIND, 8, REC> : nullptr;
+    }
+}
+
+kfn_t pick_kernel(int kind, int nfp, bool rec)
+{
+    if (kind == 0) return rec ? pick_kernel_nfp<0, true>(nfp) : pick_kernel_nfp<0, false>(nfp);
+    return rec ? pick_kernel_nfp<1, true>(nfp) : pick_kernel_nfp<1, false>(nfp);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t _e = (expr);                                                             \
+        if (_e != hipSuccess)                                                               \
+            return fail(YTRC_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));   \
+    } while (0)
+
+int nfp_of(int nf) { return nf <= 1 ? 1 : (nf == 2 ? 2 : (nf <= 4 ? 4 : 8)); }
+
+struct Dataset {
+    int n_bricks = 0;
+    int nf = 0;
+    int nfp = 0;
+    std::vector<DBrick> h_bricks;
+    DBrick *d_bricks = nullptr;
+    float *d_pool = nullptr;
+    uint8_t *d_masks = nullptr;
+    uint64_t bytes = 0;
+};
+
+}  // namespace
+
+struct ytrc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    uint64_t used = 0;
+    int n_cu = 0;
+    std::map<int, Dataset> datasets;
+    int next_ds = 1;
+    std::mutex mu;
+};
+
+namespace {
+
+struct ReqHost {
+    ytrc_request r;
+    std::vector<double> image_in;  // nv0*nv1*4 or empty
+    int64_t npix = 0;
+    int ntiles = 0;
+    kfn_t kfn = nullptr;
+    size_t lds = 0;
+    int nch_out = 0;
+    int nfp = 1;
+};
+
+}  // namespace
+
+struct ytrc_batch {
+    ytrc_ctx *ctx = nullptr;
+    uint32_t flags = 0;
+    std::vector<ReqHost> reqs;
+    std::vector<DImg> h_imgs;
+    DImg *d_imgs = nullptr;
+    char *arena = nullptr;
+    uint64_t arena_bytes = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int waves_per_group = 4;
+    int launched = 0;
+    double total_ms = 0.0;
+    int timed = 0;
+};
+
+namespace {
+
+// calculate_extent_plane_parallel (lenses.pyx:22-55) + the +-1 clip of
+// cast_through_kdtree (image_samplers.pyx:159-166), on the host, in FP64.
+void extent_plane_parallel(const ytrc_request &r, const DBrick &b, int64_t rv[4])
+{
+    const double *edges[2] = {b.le, b.re};
+    double ext[4] = {1e300, -1e300, 1e300, -1e300};
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++)
+            for (int k = 0; k < 2; k++) {
+                double t = edges[i][0] * r.x_vec[0];
+                t += edges[j][1] * r.x_vec[1];
+                t += edges[k][2] * r.x_vec[2];
+                if (t < ext[0]) ext[0] = t;
+                if (t > ext[1]) ext[1] = t;
+                t = edges[i][0] * r.y_vec[0];
+                t += edges[j][1] * r.y_vec[1];
+                t += edges[k][2] * r.y_vec[2];
+                if (t < ext[2]) ext[2] = t;
+                if (t > ext[3]) ext[3] = t;
+            }
+    double cx = 0.0, cy = 0.0;
+    for (int i = 0; i < 3; i++) {
+        cx += r.center[i] * r.x_vec[i];
+        cy += r.center[i] * r.y_vec[i];
+    }
+    const double pdx = (r.bounds[1] - r.bounds[0]) / (double)r.nv[0];
+    const double pdy = (r.bounds[3] - r.bounds[2]) / (double)r.nv[1];
+    rv[0] = lrint((ext[0] - cx - r.bounds[0]) / pdx);
+    rv[1] = rv[0] + lrint((ext[1] - ext[0]) / pdx);
+    rv[2] = lrint((ext[2] - cy - r.bounds[2]) / pdy);
+    rv[3] = rv[2] + lrint((ext[3] - ext[2]) / pdy);
+    auto clip = [](int64_t v, int64_t a, int64_t bnd) { return v < a ? a : (v > bnd ? bnd : v); };
+    rv[0] = clip(rv[0] - 1, 0, r.nv[0]);
+    rv[1] = clip(rv[1] + 1, 0, r.nv[0]);
+    rv[2] = clip(rv[2] - 1, 0, r.nv[1]);
+    rv[3] = clip(rv[3] + 1, 0, r.nv[1]);
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+
+extern "C" {
+
+const char *ytrc_last_error(void) { return g_err.c_str(); }
+
+int ytrc_ctx_create(int32_t device, uint64_t budget_bytes, ytrc_ctx **out)
+{
+    if (out == nullptr) return fail(YTRC_EINVAL, "ytrc_ctx_create: out is NULL");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return fail(YTRC_EINVAL, "ytrc_ctx_create: no such device " + std::to_string(device));
+    HIPCHK(hipSetDevice(device));
+    ytrc_ctx *c = new ytrc_ctx();
+    c->device = device;
+    c->budget = budget_bytes;
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(YTRC_EDEVICE, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
+    }
+    c->n_cu = prop.multiProcessorCount;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(YTRC_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return YTRC_OK;
+}
+
+void ytrc_ctx_destroy(ytrc_ctx *ctx)
+{
+    if (ctx == nullptr) return;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->datasets) {
+        hipFree(kv.second.d_bricks);
+        hipFree(kv.second.d_pool);
+        if (kv.second.d_masks) hipFree(kv.second.d_masks);
+    }
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int ytrc_ctx_used_bytes(const ytrc_ctx *ctx, uint64_t *used, uint64_t *budget)
+{
+    if (ctx == nullptr) return fail(YTRC_EINVAL, "ytrc_ctx_used_bytes: ctx is NULL");
+    if (used) *used = ctx->used;
+    if (budget) *budget = ctx->budget;
+    return YTRC_OK;
+}
+
+int ytrc_dataset_upload(ytrc_ctx *ctx, int32_t n_bricks, const ytrc_brick_desc *bricks,
+                        int32_t n_fields, int32_t src_dtype, const void *const *field_ptrs,
+                        const uint8_t *const *mask_ptrs, int32_t *ds_id_out)
+{
+    if (ctx == nullptr || bricks == nullptr || field_ptrs == nullptr || ds_id_out == nullptr)
+        return fail(YTRC_EINVAL, "ytrc_dataset_upload: NULL argument");
+    if (n_bricks <= 0) return fail(YTRC_EINVAL, "ytrc_dataset_upload: n_bricks must be > 0");
+    if (n_fields <= 0 || n_fields > 6)
+        return fail(YTRC_EINVAL, "ytrc_dataset_upload: n_fields must be in 1..6");
+    if (src_dtype != 0 && src_dtype != 1)
+        return fail(YTRC_EINVAL, "ytrc_dataset_upload: src_dtype must be 0 (f64) or 1 (f32)");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    Dataset ds;
+    ds.n_bricks = n_bricks;
+    ds.nf = n_fields;
+    ds.nfp = nfp_of(n_fields);
+    ds.h_bricks.resize(n_bricks);
+    int64_t pool_floats = 0, mask_bytes = 0;
+    for (int b = 0; b < n_bricks; b++) {
+        const ytrc_brick_desc &d = bricks[b];
+        DBrick &h = ds.h_bricks[b];
+        for (int i = 0; i < 3; i++) {
+            if (d.dims[i] <= 0)
+                return fail(YTRC_EINVAL, "ytrc_dataset_upload: brick " + std::to_string(b) +
+                                             " has non-positive dims");
+            if (!(d.right_edge[i] > d.left_edge[i]))
+                return fail(YTRC_EINVAL, "ytrc_dataset_upload: brick " + std::to_string(b) +
+                                             " has right_edge <= left_edge");
+            h.le[i] = d.left_edge[i];
+            h.re[i] = d.right_edge[i];
+            h.dims[i] = d.dims[i];
+            // partitioned_grid.pyx:50-55
+            h.dds[i] = (h.re[i] - h.le[i]) / (double)d.dims[i];
+            h.idds[i] = 1.0 / h.dds[i];
+        }
+        h.vertex = d.vertex_centred ? 1 : 0;
+        const int64_t np = (int64_t)(d.dims[0] + h.vertex) * (d.dims[1] + h.vertex) *
+                           (d.dims[2] + h.vertex);
+        if (np >= (int64_t)1 << 31)
+            return fail(YTRC_EINVAL, "ytrc_dataset_upload: brick " + std::to_string(b) +
+                                         " exceeds 2^31 points (int32 cell index, image_samplers.pyx:341)");
+        h.data_off = pool_floats;
+        pool_floats += align_up((size_t)np * ds.nfp, 64);
+        const bool has_mask = mask_ptrs != nullptr && mask_ptrs[b] != nullptr;
+        h.mask_off = has_mask ? mask_bytes : -1;
+        if (has_mask)
+            mask_bytes += align_up((size_t)d.dims[0] * d.dims[1] * d.dims[2], 256);
+    }
+    const uint64_t bytes = (uint64_t)pool_floats * 4 + (uint64_t)mask_bytes +
+                           (uint64_t)n_bricks * sizeof(DBrick);
+    if (ctx->budget != 0 && ctx->used + bytes > ctx->budget)
+        return fail(YTRC_EBUDGET, "ytrc_dataset_upload: " + std::to_string(bytes) +
+                                      " bytes exceed the HBM budget");
+    HIPCHK(hipMalloc(&ds.d_pool, (size_t)pool_floats * 4));
+    if (mask_bytes > 0) HIPCHK(hipMalloc(&ds.d_masks, (size_t)mask_bytes));
+    HIPCHK(hipMalloc(&ds.d_bricks, (size_t)n_bricks * sizeof(DBrick)));
+    // Interleave fields per point, convert to float32, stage brick by brick.
+    std::vector<float> stage;
+    for (int b = 0; b < n_bricks; b++) {
+        const DBrick &h = ds.h_bricks[b];
+        const int64_t np = (int64_t)(h.dims[0] + h.vertex) * (h.dims[1] + h.vertex) *
+                           (h.dims[2] + h.vertex);
+        stage.assign((size_t)np * ds.nfp, 0.0f);
+        for (int f = 0; f < n_fields; f++) {
+            const void *src = field_ptrs[(size_t)b * n_fields + f];
+            if (src == nullptr)
+                return fail(YTRC_EINVAL, "ytrc_dataset_upload: NULL field pointer");
+            if (src_dtype == 0) {
+                const double *s = (const double *)src;
+                for (int64_t p = 0; p < np; p++) stage[(size_t)p * ds.nfp + f] = (float)s[p];
+            } else {
+                const float *s = (const float *)src;
+                for (int64_t p = 0; p < np; p++) stage[(size_t)p * ds.nfp + f] = s[p];
+            }
+        }
+        HIPCHK(hipMemcpy(ds.d_pool + h.data_off, stage.data(), stage.size() * 4,
+                         hipMemcpyHostToDevice));
+        if (h.mask_off >= 0) {
+            HIPCHK(hipMemcpy(ds.d_masks + h.mask_off, mask_ptrs[b],
+                             (size_t)h.dims[0] * h.dims[1] * h.dims[2], hipMemcpyHostToDevice));
+        }
+    }
+    HIPCHK(hipMemcpy(ds.d_bricks, ds.h_bricks.data(), (size_t)n_bricks * sizeof(DBrick),
+                     hipMemcpyHostToDevice));
+    ds.bytes = bytes;
+    ctx->used += bytes;
+    const int id = ctx->next_ds++;
+    ctx->datasets[id] = std::move(ds);
+    *ds_id_out = id;
+    return YTRC_OK;
+}
+
+int ytrc_dataset_free(ytrc_ctx *ctx, int32_t ds_id)
+{
+    if (ctx == nullptr) return fail(YTRC_EINVAL, "ytrc_dataset_free: ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->datasets.find(ds_id);
+    if (it == ctx->datasets.end())
+        return fail(YTRC_EINVAL, "ytrc_dataset_free: unknown dataset " + std::to_string(ds_id));
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    hipFree(it->second.d_bricks);
+    hipFree(it->second.d_pool);
+    if (it->second.d_masks) hipFree(it->second.d_masks);
+    ctx->used -= it->second.bytes;
+    ctx->datasets.erase(it);
+    return YTRC_OK;
+}
+
+int ytrc_dataset_bytes(const ytrc_ctx *ctx, int32_t ds_id, uint64_t *bytes)
+{
+    if (ctx == nullptr || bytes == nullptr) return fail(YTRC_EINVAL, "ytrc_dataset_bytes: NULL");
+    auto it = ctx->datasets.find(ds_id);
+    if (it == ctx->datasets.end())
+        return fail(YTRC_EINVAL, "ytrc_dataset_bytes: unknown dataset " + std::to_string(ds_id));
+    *bytes = it->second.bytes;
+    return YTRC_OK;
+}
+
+int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
+                      uint32_t batch_flags, ytrc_batch **out)
+{
+    if (ctx == nullptr || reqs == nullptr || out == nullptr)
+        return fail(YTRC_EINVAL, "ytrc_batch_create: NULL argument");
+    if (n_req <= 0) return fail(YTRC_EINVAL, "ytrc_batch_create: n_req must be > 0");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    ytrc_batch *B = new ytrc_batch();
+    B->ctx = ctx;
+    B->flags = batch_flags;
+    B->reqs.resize(n_req);
+    B->h_imgs.resize(n_req);
+    // Pass 1: validate and size the arena.
+    size_t off = 0;
+    struct Offs {
+        size_t order, ext, zbuf, init, out, rv, rh, rs, tf, tft;
+    };
+    std::vector<Offs> offs(n_req);
+    std::vector<std::vector<int4>> exts(n_req);
+    std::vector<DTF> tfs(n_req);
+    std::vector<std::vector<float4>> tft(n_req);
+    auto bad = [&](int code, const std::string &m) {
+        delete B;
+        return fail(code, m);
+    };
+    for (int i = 0; i < n_req; i++) {
+        const ytrc_request &r = reqs[i];
+        ReqHost &q = B->reqs[i];
+        q.r = r;
+        const std::string tag = "request " + std::to_string(i) + ": ";
+        if (r.kind != YTRC_PROJECTION && r.kind != YTRC_VOLUME_RENDER)
+            return bad(YTRC_ENOTIMPL, tag + "unknown sampler kind");
+        if (r.lens != YTRC_LENS_PLANE_PARALLEL)
+            return bad(YTRC_ENOTIMPL, tag + "only the plane-parallel lens is implemented");
+        if (r.nv[0] <= 0 || r.nv[1] <= 0) return bad(YTRC_EINVAL, tag + "image size must be > 0");
+        auto dsit = ctx->datasets.find(r.ds_id);
+        if (dsit == ctx->datasets.end()) return bad(YTRC_EINVAL, tag + "unknown dataset");
+        const Dataset &ds = dsit->second;
+        if (r.n_order < 0 || (r.n_order > 0 && r.order == nullptr))
+            return bad(YTRC_EINVAL, tag + "bad brick order");
+        for (int k = 0; k < r.n_order; k++) {
+            if (r.order[k] < 0 || r.order[k] >= ds.n_bricks)
+                return bad(YTRC_EINVAL, tag + "brick order index out of range");
+            const DBrick &hb = ds.h_bricks[r.order[k]];
+            if (r.kind == YTRC_PROJECTION && hb.vertex)
+                return bad(YTRC_EINVAL, tag + "ProjectionSampler needs cell-centred bricks");
+            if (r.kind == YTRC_VOLUME_RENDER && !hb.vertex)
+                return bad(YTRC_EINVAL, tag + "VolumeRenderSampler needs vertex-centred bricks");
+        }
+        q.npix = (int64_t)r.nv[0] * r.nv[1];
+        const int t0 = (r.nv[0] + TILE - 1) / TILE, t1 = (r.nv[1] + TILE - 1) / TILE;
+        q.ntiles = t0 * t1;
+        q.nfp = ds.nfp;
+        const bool rec = (r.flags & YTRC_FLAG_RECORD) != 0;
+        if (r.kind == YTRC_PROJECTION) {
+            q.nch_out = std::min(4, ds.nf);
+        } else {
+            if (r.tf == nullptr) return bad(YTRC_EINVAL, tag + "VolumeRenderSampler needs a transfer function");
+            const ytrc_tf_desc &tf = *r.tf;
+            if (tf.n_fits < 0 || tf.n_fits > 6)
+                return bad(YTRC_EINVAL, tag + "n_fits must be <= 6 (image_samplers.pyx:430)");
+            if (r.n_samples <= 0) return bad(YTRC_EINVAL, tag + "n_samples must be > 0");
+            DTF &T = tfs[i];
+            memset(&T, 0, sizeof(T));
+            T.n_fits = tf.n_fits;
+            T.grey = tf.grey_opacity == 1;
+            int tot = 0;
+            for (int t = 0; t < tf.n_fits; t++) {
+                if (tf.nbins[t] < 2 || tf.y[t] == nullptr)
+                    return bad(YTRC_EINVAL, tag + "table needs nbins >= 2 and values");
+                if (tf.field_ids[t] < 0 || tf.field_ids[t] >= ds.nf ||
+                    tf.weight_field_ids[t] < -1 || tf.weight_field_ids[t] >= ds.nf ||
+                    tf.weight_table_ids[t] < -1 || tf.weight_table_ids[t] >= tf.n_fits)
+                    return bad(YTRC_EINVAL, tag + "table field/weight ids out of range");
+                T.nbins[t] = tf.nbins[t];
+                T.off[t] = tot;
+                T.field_id[t] = tf.field_ids[t];
+                T.wfield[t] = tf.weight_field_ids[t];
+                T.wtable[t] = tf.weight_table_ids[t];
+                const double b0 = tf.bounds[2 * t], b1 = tf.bounds[2 * t + 1];
+                T.b0[t] = (float)b0;
+                T.b1[t] = (float)b1;
+                // FIT_initialize_table (field_interpolation_tables.pxd:44-61) in FP64.
+                const double dbin = (b1 - b0) / (double)(tf.nbins[t] - 1);
+                const double idbin = 1.0 / dbin;
+                T.idbin[t] = (float)idbin;
+                for (int k = 0; k < tf.nbins[t]; k++) {
+                    float4 q4;
+                    q4.x = (float)tf.y[t][k];
+                    if (k < tf.nbins[t] - 1) {
+                        q4.y = (float)((tf.y[t][k + 1] - tf.y[t][k]) * idbin);
+                        q4.z = (float)(b0 + (double)k * dbin);
+                    } else {
+                        q4.y = 0.0f;
+                        q4.z = 0.0f;
+                    }
+                    q4.w = 0.0f;
+                    tft[i].push_back(q4);
+                }
+                tot += tf.nbins[t];
+            }
+            for (int c = 0; c < 6; c++) {
+                if (tf.field_table_ids[c] < 0 || tf.field_table_ids[c] >= 6)
+                    return bad(YTRC_EINVAL, tag + "field_table_ids out of range");
+                T.ftable[c] = tf.field_table_ids[c];
+            }
+            // Resolve FIT_eval_transfer's in-place weight loop
+            // (field_interpolation_tables.pxd:96-99) into powers of raw values.
+            int pw[6][6];
+            memset(pw, 0, sizeof(pw));
+            for (int t = 0; t < tf.n_fits; t++) pw[t][t] = 1;
+            for (int t = 0; t < tf.n_fits; t++) {
+                const int w = tf.weight_table_ids[t];
+                if (w == -1) continue;
+                int add[6];
+                for (int u = 0; u < 6; u++) add[u] = pw[w][u];
+                for (int u = 0; u < 6; u++) pw[t][u] += add[u];
+            }
+            for (int c = 0; c < 4; c++) {
+                const int t = tf.field_table_ids[c];
+                if (t >= tf.n_fits) {
+                    T.chexp[c] = -1;  // table never evaluated: channel value 0
+                    continue;
+                }
+                int ce = 0;
+                for (int u = 0; u < 6; u++) {
+                    if (pw[t][u] > 3)
+                        return bad(YTRC_ENOTIMPL, tag + "weight-table chain raises a table to a power > 3");
+                    ce |= pw[t][u] << (2 * u);
+                }
+                T.chexp[c] = ce;
+            }
+            T.total_bins = tot;
+            q.lds = (size_t)tot * sizeof(float4);
+            if (q.lds > 64 * 1024) return bad(YTRC_EINVAL, tag + "transfer-function tables exceed 64 KiB of LDS");
+            q.nch_out = T.grey ? 4 : 3;
+        }
+        q.kfn = pick_kernel(r.kind, q.nfp, rec);
+        if (q.kfn == nullptr) return bad(YTRC_ENOTIMPL, tag + "no kernel for this field count");
+        if (r.image_in != nullptr) q.image_in.assign(r.image_in, r.image_in + q.npix * 4);
+        Offs &o = offs[i];
+        exts[i].resize(std::max(1, r.n_order));
+        for (int k = 0; k < r.n_order; k++) {
+            int64_t rv[4];
+            extent_plane_parallel(r, ds.h_bricks[r.order[k]], rv);
+            exts[i][k] = make_int4((int)rv[0], (int)rv[1], (int)rv[2], (int)rv[3]);
+        }
+        o.order = off;
+        off = align_up(off + sizeof(int32_t) * std::max(1, r.n_order), 256);
+        o.ext = off;
+        off = align_up(off + sizeof(int4) * exts[i].size(), 256);
+        o.zbuf = off;
+        if (r.zbuffer) off = align_up(off + sizeof(double) * q.npix, 256);
+        o.init = off;
+        if (r.image_in) off = align_up(off + sizeof(float) * 4 * q.npix, 256);
+        o.out = off;
+        off = align_up(off + sizeof(float) * 4 * q.npix, 256);
+        o.rv = o.rh = o.rs = off;
+        if (rec) {
+            o.rv = off;
+            off = align_up(off + 8 * q.npix, 256);
+            o.rh = off;
+            off = align_up(off + 8 * q.npix, 256);
+            o.rs = off;
+            off = align_up(off + 8 * q.npix, 256);
+        }
+        o.tf = o.tft = off;
+        if (r.kind == YTRC_VOLUME_RENDER) {
+            o.tf = off;
+            off = align_up(off + sizeof(DTF), 256);
+            o.tft = off;
+            off = align_up(off + sizeof(float4) * tft[i].size(), 256);
+        }
+    }
+    const size_t imgs_off = off;
+    off = align_up(off + sizeof(DImg) * n_req, 256);
+    B->arena_bytes = off;
+    if (ctx->budget != 0 && ctx->used + off > ctx->budget)
+        return bad(YTRC_EBUDGET, "ytrc_batch_create: " + std::to_string(off) +
+                                     " bytes exceed the HBM budget");
+    hipError_t e = hipMalloc(&B->arena, off);
+    if (e != hipSuccess) return bad(YTRC_EDEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+    ctx->used += off;
+    char *A = B->arena;
+    // Pass 2: fill descriptors and copy inputs.
+    std::vector<float> stage;
+    for (int i = 0; i < n_req; i++) {
+        const ytrc_request &r = reqs[i];
+        ReqHost &q = B->reqs[i];
+        const Offs &o = offs[i];
+        const Dataset &ds = ctx->datasets[r.ds_id];
+        const bool rec = (r.flags & YTRC_FLAG_RECORD) != 0;
+        DImg &d = B->h_imgs[i];
+        memset(&d, 0, sizeof(d));
+        memcpy(d.vp_pos, r.vp_pos, sizeof(d.vp_pos));
+        memcpy(d.vp_dir, r.vp_dir, sizeof(d.vp_dir));
+        memcpy(d.width, r.width, sizeof(d.width));
+        d.bricks = ds.d_bricks;
+        d.pool = ds.d_pool;
+        d.masks = ds.d_masks;
+        d.order = (const int32_t *)(A + o.order);
+        d.ext = (const int4 *)(A + o.ext);
+        d.zbuf = r.zbuffer ? (const double *)(A + o.zbuf) : nullptr;
+        d.init = r.image_in ? (const float *)(A + o.init) : nullptr;
+        d.out = (float *)(A + o.out);
+        d.rec_visits = rec ? (int64_t *)(A + o.rv) : nullptr;
+        d.rec_hash = rec ? (uint64_t *)(A + o.rh) : nullptr;
+        d.rec_samples = rec ? (int64_t *)(A + o.rs) : nullptr;
+        d.tf = (r.kind == YTRC_VOLUME_RENDER) ? (const DTF *)(A + o.tf) : nullptr;
+        d.nv0 = r.nv[0];
+        d.nv1 = r.nv[1];
+        d.kind = r.kind;
+        d.n_samples = r.n_samples;
+        d.n_order = r.n_order;
+        d.nf = ds.nf;
+        d.nfp = ds.nfp;
+        d.tiles0 = (r.nv[0] + TILE - 1) / TILE;
+        d.tiles1 = (r.nv[1] + TILE - 1) / TILE;
+        d.flags = (int32_t)r.flags;
+        d.nch_out = q.nch_out;
+        d.inv_ns = r.n_samples > 0 ? 1.0 / (double)r.n_samples : 0.0;
+        if (r.n_order > 0) {
+            HIPCHK(hipMemcpy(A + o.order, r.order, sizeof(int32_t) * r.n_order, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(A + o.ext, exts[i].data(), sizeof(int4) * r.n_order, hipMemcpyHostToDevice));
+        }
+        if (r.zbuffer)
+            HIPCHK(hipMemcpy(A + o.zbuf, r.zbuffer, sizeof(double) * q.npix, hipMemcpyHostToDevice));
+        if (r.image_in) {
+            stage.resize((size_t)q.npix * 4);
+            for (int64_t p = 0; p < q.npix; p++)
+                for (int c = 0; c < 4; c++) stage[(size_t)c * q.npix + p] = (float)r.image_in[p * 4 + c];
+            HIPCHK(hipMemcpy(A + o.init, stage.data(), sizeof(float) * 4 * q.npix, hipMemcpyHostToDevice));
+        }
+        if (r.kind == YTRC_VOLUME_RENDER) {
+            tfs[i].table = (const float4 *)(A + o.tft);
+            HIPCHK(hipMemcpy(A + o.tft, tft[i].data(), sizeof(float4) * tft[i].size(), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(A + o.tf, &tfs[i], sizeof(DTF), hipMemcpyHostToDevice));
+        }
+    }
+    B->d_imgs = (DImg *)(A + imgs_off);
+    HIPCHK(hipMemcpy(B->d_imgs, B->h_imgs.data(), sizeof(DImg) * n_req, hipMemcpyHostToDevice));
+    HIPCHK(hipEventCreate(&B->ev0));
+    HIPCHK(hipEventCreate(&B->ev1));
+    *out = B;
+    return YTRC_OK;
+}
+
+int ytrc_batch_set_launch(ytrc_batch *b, int32_t waves_per_group, int32_t groups_per_cu)
+{
+    (void)groups_per_cu;
+    if (b == nullptr) return fail(YTRC_EINVAL, "ytrc_batch_set_launch: NULL batch");
+    if (waves_per_group != 1 && waves_per_group != 2 && waves_per_group != 4)
+        return fail(YTRC_EINVAL, "ytrc_batch_set_launch: waves_per_group must be 1, 2 or 4");
+    b->waves_per_group = waves_per_group;
+    return YTRC_OK;
+}
+
+int ytrc_batch_launch(ytrc_batch *b)
+{
+    if (b == nullptr) return fail(YTRC_EINVAL, "ytrc_batch_launch: NULL batch");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t s = b->ctx->stream;
+    HIPCHK(hipEventRecord(b->ev0, s));
+    const int wpg = b->waves_per_group;
+    for (size_t i = 0; i < b->reqs.size(); i++) {
+        const ReqHost &q = b->reqs[i];
+        const int groups = (q.ntiles + wpg - 1) / wpg;
+        if (groups == 0) continue;
+        hipLaunchKernelGGL(q.kfn, dim3(groups), dim3(64 * wpg), q.lds, s,
+                           (const DImg *)b->d_imgs, (int)i);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(b->ev1, s));
+    b->launched += 1;
+    return YTRC_OK;
+}
+
+int ytrc_batch_sync(ytrc_batch *b, float *ms_out)
+{
+    if (b == nullptr) return fail(YTRC_EINVAL, "ytrc_batch_sync: NULL batch");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipEventSynchronize(b->ev1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->total_ms += ms;
+    b->timed += 1;
+    if (ms_out) *ms_out = ms;
+    return YTRC_OK;
+}
+
+int ytrc_batch_kernel_ms(const ytrc_batch *b, float *avg_ms, int32_t *n_launches)
+{
+    if (b == nullptr) return fail(YTRC_EINVAL, "ytrc_batch_kernel_ms: NULL batch");
+    if (avg_ms) *avg_ms = b->timed ? (float)(b->total_ms / b->timed) : 0.0f;
+    if (n_launches) *n_launches = b->timed;
+    return YTRC_OK;
+}
+
+int ytrc_batch_read_image(ytrc_batch *b, int32_t i, double *image_out)
+{
+    if (b == nullptr || image_out == nullptr) return fail(YTRC_EINVAL, "ytrc_batch_read_image: NULL");
+    if (i < 0 || i >= (int)b->reqs.size()) return fail(YTRC_EINVAL, "ytrc_batch_read_image: bad index");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    const ReqHost &q = b->reqs[i];
+    const DImg &d = b->h_imgs[i];
+    std::vector<float> h((size_t)q.npix * q.nch_out);
+    HIPCHK(hipMemcpy(h.data(), d.out, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
+    for (int64_t p = 0; p < q.npix; p++) {
+        for (int c = 0; c < 4; c++) {
+            double v;
+            if (c < q.nch_out)
+                v = (double)h[(size_t)c * q.npix + p];
+            else
+                v = q.image_in.empty() ? 0.0 : q.image_in[(size_t)p * 4 + c];
+            image_out[(size_t)p * 4 + c] = v;
+        }
+    }
+    return YTRC_OK;
+}
+
+int ytrc_batch_read_record(ytrc_batch *b, int32_t i, int64_t *visits, uint64_t *hashes,
+                           int64_t *samples)
+{
+    if (b == nullptr) return fail(YTRC_EINVAL, "ytrc_batch_read_record: NULL batch");
+    if (i < 0 || i >= (int)b->reqs.size()) return fail(YTRC_EINVAL, "ytrc_batch_read_record: bad index");
+    const DImg &d = b->h_imgs[i];
+    if (d.rec_visits == nullptr)
+        return fail(YTRC_EINVAL, "ytrc_batch_read_record: request was not built with YTRC_FLAG_RECORD");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    const size_t n = (size_t)b->reqs[i].npix;
+    if (visits) HIPCHK(hipMemcpy(visits, d.rec_visits, 8 * n, hipMemcpyDeviceToHost));
+    if (hashes) HIPCHK(hipMemcpy(hashes, d.rec_hash, 8 * n, hipMemcpyDeviceToHost));
+    if (samples) HIPCHK(hipMemcpy(samples, d.rec_samples, 8 * n, hipMemcpyDeviceToHost));
+    return YTRC_OK;
+}
+
+int ytrc_batch_bytes(const ytrc_batch *b, uint64_t *bytes)
+{
+    if (b == nullptr || bytes == nullptr) return fail(YTRC_EINVAL, "ytrc_batch_bytes: NULL");
+    *bytes = b->arena_bytes;
+    return YTRC_OK;
+}
+
+void ytrc_batch_destroy(ytrc_batch *b)
+{
+    if (b == nullptr) return;
+    hipSetDevice(b->ctx->device);
+    hipStreamSynchronize(b->ctx->stream);
+    if (b->arena) {
+        hipFree(b->arena);
+        std::lock_guard<std::mutex> lk(b->ctx->mu);
+        b->ctx->used -= b->arena_bytes;
+    }
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    delete b;
+}
+
+}  // extern "C"

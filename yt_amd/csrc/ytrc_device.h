@@ -1,0 +1,250 @@
+// ytrc_device.h -- device-side data layout and ray-casting core for gfx950.
+//
+// Compiled with -ffp-contract=off: every FP64 operation of the DDA and of
+// the ray set-up is a separately rounded IEEE operation, in the reference's
+// order, so cell-visit sequences are bit-identical to yt's walk_volume
+// (grid_traversal.pyx:26-140).  FP32 sampling code uses explicit fmaf.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ytrc {
+
+// Brick descriptor in HBM (VolumeContainer, volume_container.pxd:13-33).
+// 128 bytes; read with scalar loads because every lane of a wave walks the
+// same brick at the same time.
+struct alignas(16) DBrick {
+    double le[3];
+    double re[3];
+    double dds[3];
+    double idds[3];
+    int32_t dims[3];
+    int32_t vertex;      // 1: (dims+1)^3 points per field
+    int64_t data_off;    // float offset of the brick's points in the pool
+    int64_t mask_off;    // byte offset of the dims^3 mask, -1 = none (all ones)
+};
+
+// Transfer function parameters (FieldInterpolationTable x n_fits,
+// field_interpolation_tables.pxd:27-39).  Tables live in LDS as float4
+// {y, dy, d0, 0} per (table, bin).
+struct alignas(16) DTF {
+    int32_t n_fits;
+    int32_t grey;
+    int32_t total_bins;
+    int32_t pad0;
+    int32_t nbins[6];
+    int32_t off[6];
+    int32_t field_id[6];
+    int32_t wfield[6];
+    int32_t wtable[6];
+    int32_t ftable[6];
+    int32_t chexp[4];     // per channel: 2-bit power of each raw table value, -1 = zero
+    float b0[6];
+    float b1[6];
+    float idbin[6];
+    const float4 *table;  // total_bins float4 in global memory
+};
+
+// One image request on the device.
+struct alignas(16) DImg {
+    double vp_pos[12];
+    double vp_dir[3];
+    double width[3];
+    const DBrick *bricks;
+    const float *pool;
+    const uint8_t *masks;
+    const int32_t *order;   // n_order brick indices
+    const int4 *ext;        // n_order clipped extents {i0, i1, j0, j1}
+    const double *zbuf;     // nv0*nv1 or null
+    const float *init;      // 4 planes of nv0*nv1 or null
+    float *out;             // 4 planes of nv0*nv1
+    int64_t *rec_visits;    // record mode only
+    uint64_t *rec_hash;
+    int64_t *rec_samples;
+    const DTF *tf;
+    int32_t nv0, nv1;
+    int32_t kind, n_samples;
+    int32_t n_order, nf;
+    int32_t nfp, tiles0;
+    int32_t tiles1, flags;
+    int32_t nch_out, tf_slot;
+    double inv_ns;
+};
+
+constexpr int TILE = 8;  // 8x8 rays per 64-lane wavefront
+
+__device__ __forceinline__ double y_fmin(double a, double b) { return (a < b) ? a : b; }
+__device__ __forceinline__ double y_fmax(double a, double b) { return (a > b) ? a : b; }
+
+// generate_vector_info_plane_parallel: lenses.pyx:179-192.
+__device__ __forceinline__ void ray_setup(const DImg &im, int vi, int vj, double pos[3],
+                                          double dir[3])
+{
+    double px = im.width[0] * (double)vi / ((double)im.nv0 - 1.0) - im.width[0] / 2.0;
+    double py = im.width[1] * (double)vj / ((double)im.nv1 - 1.0) - im.width[1] / 2.0;
+    pos[0] = im.vp_pos[0] * px + im.vp_pos[3] * py + im.vp_pos[9];
+    pos[1] = im.vp_pos[1] * px + im.vp_pos[4] * py + im.vp_pos[10];
+    pos[2] = im.vp_pos[2] * px + im.vp_pos[5] * py + im.vp_pos[11];
+    dir[0] = im.vp_dir[0];
+    dir[1] = im.vp_dir[1];
+    dir[2] = im.vp_dir[2];
+}
+
+// DDA state after walk_volume's initialisation (grid_traversal.pyx:42-118).
+struct Walk {
+    double tmax[3];
+    double tdelta[3];
+    int cur[3];
+    int step[3];
+    double enter_t;
+};
+
+// grid_traversal.pyx:42-118.  Returns false where walk_volume returns 0
+// before its loop.  iv[i] = 1/v_dir[i] (0 where v_dir[i] == 0).
+__device__ __forceinline__ bool walk_init(const DBrick &b, const double pos[3],
+                                          const double dir[3], const double iv[3],
+                                          double max_t, Walk &w)
+{
+    double intersect_t = 1.1;
+    int direction = -1;
+    if (b.le[0] <= pos[0] && pos[0] < b.re[0] && b.le[1] <= pos[1] && pos[1] < b.re[1] &&
+        b.le[2] <= pos[2] && pos[2] < b.re[2]) {
+        intersect_t = 0.0;
+        direction = 3;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        w.step[i] = (dir[i] < 0) ? -1 : ((dir[i] == 0.0) ? 0 : 1);
+    }
+    if (direction != 3) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            if (w.step[i] == 0) continue;
+            const int x = (i + 1) % 3;
+            const int y = (i + 2) % 3;
+            double tl = ((w.step[i] > 0) ? (b.le[i] - pos[i]) : (b.re[i] - pos[i])) * iv[i];
+            double tx = (pos[x] + tl * dir[x]);
+            double ty = (pos[y] + tl * dir[y]);
+            if (fabs(tx - b.le[x]) < 1e-10 * b.dds[x])
+                tx = b.le[x];
+            else if (fabs(tx - b.re[x]) < 1e-10 * b.dds[x])
+                tx = b.re[x];
+            if (fabs(ty - b.le[y]) < 1e-10 * b.dds[y])
+                ty = b.le[y];
+            else if (fabs(ty - b.re[y]) < 1e-10 * b.dds[y])
+                ty = b.re[y];
+            if (b.le[x] <= tx && tx <= b.re[x] && b.le[y] <= ty && ty <= b.re[y] && 0.0 <= tl &&
+                tl < intersect_t) {
+                direction = i;
+                intersect_t = tl;
+            }
+        }
+    }
+    if (!((0.0 <= intersect_t) && (intersect_t < max_t))) return false;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        w.tdelta[i] = (double)w.step[i] * iv[i] * b.dds[i];
+        double ty = 0.0;
+        if (i == direction && w.step[i] > 0) {
+            w.cur[i] = 0;
+        } else if (i == direction && w.step[i] < 0) {
+            w.cur[i] = b.dims[i] - 1;
+        } else {
+            double tx = intersect_t * dir[i] + pos[i];
+            ty = ((tx - b.le[i]) * b.idds[i]);
+            if (-1 < ty && ty < 0 && w.step[i] > 0)
+                ty = 0.0;
+            else if ((double)(b.dims[i] - 1) < ty && ty < (double)b.dims[i] && w.step[i] < 0)
+                ty = (double)(b.dims[i] - 1);
+            w.cur[i] = (int)(floor(ty));
+        }
+        if (w.step[i] > 0)
+            ty = (double)(w.cur[i] + 1) * b.dds[i] + b.le[i];
+        else if (w.step[i] < 0)
+            ty = (double)w.cur[i] * b.dds[i] + b.le[i];
+        w.tmax[i] = (w.step[i] == 0) ? 1e60 : (ty - pos[i]) * iv[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if (w.cur[i] == b.dims[i] && w.step[i] >= 0) return false;
+        if (w.cur[i] == -1 && w.step[i] <= -1) return false;
+    }
+    w.enter_t = intersect_t;
+    return true;
+}
+
+// The traversal loop, grid_traversal.pyx:120-138.  `sample` is called with
+// (enter_t, exit_t, i, j, k) once per visited cell, in order.
+template <class S>
+__device__ __forceinline__ int walk_loop(const DBrick &b, Walk &w, double max_t, S &sample)
+{
+    int hit = 0;
+    double t0 = w.tmax[0], t1 = w.tmax[1], t2 = w.tmax[2];
+    int c0 = w.cur[0], c1 = w.cur[1], c2 = w.cur[2];
+    double enter_t = w.enter_t;
+    while (true) {
+        hit += 1;
+        const bool a01 = t0 < t1;
+        const bool a02 = t0 < t2;
+        const bool a12 = t1 < t2;
+        const int ax = a01 ? (a02 ? 0 : 2) : (a12 ? 1 : 2);
+        const double tm = (ax == 0) ? t0 : ((ax == 1) ? t1 : t2);
+        const double exit_t = y_fmin(tm, max_t);
+        sample(enter_t, exit_t, c0, c1, c2);
+        int cn, dn;
+        if (ax == 0) {
+            c0 += w.step[0];
+            t0 += w.tdelta[0];
+            cn = c0;
+            dn = b.dims[0];
+        } else if (ax == 1) {
+            c1 += w.step[1];
+            t1 += w.tdelta[1];
+            cn = c1;
+            dn = b.dims[1];
+        } else {
+            c2 += w.step[2];
+            t2 += w.tdelta[2];
+            cn = c2;
+            dn = b.dims[2];
+        }
+        enter_t = tm;
+        if (cn < 0 || cn >= dn || enter_t >= max_t) break;
+    }
+    return hit;
+}
+
+__device__ __forceinline__ bool in_brick(const DBrick &b, int c0, int c1, int c2)
+{
+    return (unsigned)c0 < (unsigned)b.dims[0] && (unsigned)c1 < (unsigned)b.dims[1] &&
+           (unsigned)c2 < (unsigned)b.dims[2];
+}
+
+// Visit hash, identical to the oracle's: 64-bit mix over
+// (brick id, i, j, k, bits(enter_t), bits(exit_t)).
+__device__ __forceinline__ uint64_t vmix(uint64_t h, uint64_t w)
+{
+    h ^= w;
+    h *= 0x100000001b3ULL;
+    h ^= h >> 29;
+    return h;
+}
+
+struct Rec {
+    int64_t visits;
+    int64_t samples;
+    uint64_t hash;
+    int brick;
+    __device__ __forceinline__ void visit(double en, double ex, int c0, int c1, int c2)
+    {
+        uint64_t h = hash;
+        h = vmix(h, (uint64_t)(uint32_t)brick | ((uint64_t)(uint32_t)c0 << 32));
+        h = vmix(h, (uint64_t)(uint32_t)c1 | ((uint64_t)(uint32_t)c2 << 32));
+        h = vmix(h, (uint64_t)__double_as_longlong(en));
+        h = vmix(h, (uint64_t)__double_as_longlong(ex));
+        hash = h;
+        visits += 1;
+    }
+};
+
+}  // namespace ytrc

@@ -1,0 +1,44 @@
+"""off_axis_projection for grid datasets (the ProjectionSampler caller).
+
+Restates yt/visualization/volume_rendering/off_axis_projection.py:352-547
+for grid indices: camera set-up (:385-411), a ProjectionSampler over the
+child-masked blocks in index order (:505-524), alpha := 1 (:526), and the
+final normalisation (:537-547).  Returns image[:, :, 0] as float64
+(res0, res1).  Lengths are in code units.
+"""
+
+import numpy as np
+
+from .camera import off_axis_camera
+from .image_samplers import ProjectionSampler
+from .partitioned_grid import PartitionedGrid
+from .render import finalize_projection
+
+
+def off_axis_projection(data_source, center, normal_vector, width, resolution, item, weight=None,
+                        volume=None, no_ghost=False, interpolated=False, north_vector=None,
+                        num_threads=1, method="integrate"):
+    if method not in ("integrate", "sum"):
+        raise NotImplementedError("Only 'integrate' or 'sum' methods are valid for off-axis-projections")
+    if interpolated:
+        raise NotImplementedError("interpolated projections are not part of this ray-caster")
+    ds = getattr(data_source, "ds", data_source)
+    cam = off_axis_camera(center, normal_vector, width, resolution, north_vector,
+                          ds.domain_left_edge, ds.domain_right_edge)
+    params = cam._get_sampler_params()
+    image = params["image"]
+    zbuffer = np.ones(image.shape[:2], "float64")
+    sampler = ProjectionSampler(
+        np.atleast_3d(params["vp_pos"]), np.atleast_3d(params["vp_dir"]), params["center"],
+        params["bounds"], image, params["x_vec"], params["y_vec"], params["width"], "KDTree", 10,
+        lens_type=params["lens_type"], zbuffer=zbuffer)
+    for grid, mask in data_source.blocks():
+        if weight is None:
+            data = [np.asarray(grid[item], dtype="float64") * mask]
+        else:
+            w = np.asarray(grid[weight], dtype="float64")
+            data = [np.asarray(grid[item], dtype="float64") * w * mask, w * mask]
+        pg = PartitionedGrid(grid.id, data, mask.astype("uint8"), grid.LeftEdge, grid.RightEdge,
+                             grid.ActiveDimensions.astype("int64"))
+        sampler(pg, num_threads=num_threads)
+    return finalize_projection(sampler.aimage, cam.width, weight is not None, method)

@@ -1,0 +1,124 @@
+"""Device-resident volumes and batched requests.
+
+A ResidentVolume uploads a dataset's bricks to HBM once: the child-masked
+cell-centred blocks the projection path walks (off_axis_projection.py:506-522)
+and/or the vertex-centred log10 kd-tree bricks of the volume-render path
+(amr_kdtree.py:325-374).  Requests then only name a camera; the brick order
+is the reference's call order: blocks order for projections, the kd-tree's
+farthest-from-viewpoint order for volume renders (render_source.py:594-597).
+"""
+
+import numpy as np
+
+from . import _lib
+from .amr_kdtree import AMRKDTree
+from .camera import off_axis_camera
+
+
+class ResidentVolume:
+    def __init__(self, ctx, ds, proj_fields=None, weight=None, vr_field=None, log_field=True):
+        self.ctx = ctx
+        self.ds = ds
+        self.proj_ds_id = None
+        self.vr_ds_id = None
+        self.weighted = weight is not None
+        self.n_proj_bricks = 0
+        self.n_vr_bricks = 0
+        self.proj_cells = 0
+        if proj_fields is not None:
+            if isinstance(proj_fields, str):
+                proj_fields = [proj_fields]
+            bricks, fields = [], []
+            for grid, mask in ds.blocks():
+                m = mask.astype("float64") if mask.dtype == bool else mask
+                if weight is None:
+                    data = [np.asarray(grid[f], dtype="float64") * m for f in proj_fields]
+                else:
+                    f0 = proj_fields[0]
+                    w = np.asarray(grid[weight], dtype="float64")
+                    data = [np.asarray(grid[f0], dtype="float64") * w * m, w * m]
+                bricks.append((grid.LeftEdge, grid.RightEdge, grid.ActiveDimensions, False))
+                fields.append(data)
+                self.proj_cells += int(np.prod(grid.ActiveDimensions))
+            self.proj_ds_id = ctx.upload(bricks, fields)
+            self.n_proj_bricks = len(bricks)
+        if vr_field is not None:
+            self.kd = AMRKDTree(ds)
+            self.kd.set_fields([vr_field], [log_field], no_ghost=True)
+            self.leaves = self.kd.leaves()
+            self._leaf_index = {id(n): i for i, n in enumerate(self.leaves)}
+            bricks, fields, masks = [], [], []
+            for node in self.leaves:
+                pg = self.kd.get_brick_data(node)
+                bricks.append((pg.LeftEdge, pg.RightEdge, pg.dims, True))
+                fields.append(pg.my_data)
+                masks.append(pg.source_mask)
+                node.data = None  # keep only the device copy
+            self.vr_ds_id = ctx.upload(bricks, fields, masks)
+            self.n_vr_bricks = len(bricks)
+            self.kd._vcd_cache = {}
+
+    def device_bytes(self):
+        b = 0
+        for i in (self.proj_ds_id, self.vr_ds_id):
+            if i is not None:
+                b += self.ctx.dataset_bytes(i)
+        return b
+
+    def vr_order(self, viewpoint):
+        return np.array([self._leaf_index[id(n)] for n in
+                         self.kd.trunk.kd_traverse(viewpoint=viewpoint)], dtype=np.int32)
+
+    def projection_request(self, params, record=False, zbuffer=None):
+        if self.proj_ds_id is None:
+            raise ValueError("volume was not uploaded for projections")
+        return _lib.RequestSpec(_lib.PROJECTION, params, self.proj_ds_id,
+                                np.arange(self.n_proj_bricks, dtype=np.int32), record=record,
+                                zbuffer=zbuffer)
+
+    def vr_request(self, params, tf, n_samples=10, record=False, ert=False, zbuffer=None):
+        if self.vr_ds_id is None:
+            raise ValueError("volume was not uploaded for volume rendering")
+        tfc = tf if isinstance(tf, _lib.TF) else _lib.TF.from_tf_obj(tf)
+        return _lib.RequestSpec(_lib.VOLUME_RENDER, params, self.vr_ds_id,
+                                self.vr_order(params["viewpoint"]), n_samples=n_samples, tf=tfc,
+                                record=record, ert=ert, zbuffer=zbuffer)
+
+    def free(self):
+        for i in (self.proj_ds_id, self.vr_ds_id):
+            if i is not None:
+                self.ctx.free(i)
+        self.proj_ds_id = self.vr_ds_id = None
+
+
+def camera_params(center, normal, width, resolution, north=None):
+    """Sampler parameters (+ kd viewpoint) of an off-axis plane-parallel camera."""
+    cam = off_axis_camera(center, normal, width, resolution, north)
+    p = cam._get_sampler_params()
+    p["viewpoint"] = cam.lens.viewpoint.copy()
+    return p
+
+
+def finalize_projection(image, width, weighted, method="integrate"):
+    """off_axis_projection.py:526-547 + VolumeSource.finalize_image
+    (render_source.py:522-537): alpha := 1, then scale or divide."""
+    image = image.copy()
+    image[:, :, 3] = 1
+    if method == "integrate":
+        if not weighted:
+            image *= float(np.asarray(width, dtype="float64").ravel()[-1] if np.ndim(width) else width)
+        else:
+            mask = image[:, :, 1] == 0
+            nmask = np.logical_not(mask)
+            image[:, :, 0][nmask] /= image[:, :, 1][nmask]
+            image[mask] = 0
+    return image[:, :, 0]
+
+
+def finalize_vr(image, grey_opacity=False):
+    """render_source.py:522-537 (alpha := 1 unless grey) and the 180-degree
+    rotation of Scene.composite (scene.py:625)."""
+    image = image.copy()
+    if not grey_opacity:
+        image[:, :, 3] = 1
+    return np.rot90(image, k=2)
