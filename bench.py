@@ -1,0 +1,284 @@
+#!/usr/bin/env python
+"""Headline benchmark: batched off-axis projection + volume rendering of a
+512^3 float32 lognormal field on one MI355X.
+
+Workload (one "step"): 64 plane-parallel cameras on seeded directions
+(seed 1004) with random north vectors, image sides drawn from
+{128, 256, 512, 1024} (seed 1006), alternating unweighted projection and
+volume rendering (8-Gaussian ColorTransferFunction over log10 density,
+10 sub-samples per cell).  The field (seed 1001, P(k) ~ k^-3, sigma_ln = 1)
+is split into 2x2x2 grids of 256^3 (load_uniform_grid), which are the
+projection blocks and the AMRKDTree bricks (vertex-centred, log10).  All
+bricks are resident in HBM before timing starts; a step is one launch of the
+whole batch (persistent tile-queue kernels captured in a hipGraph).
+
+Prints one JSON line.  Progress goes to stderr.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gsamples/s off-axis proj+VR at 512^3 (1 GPU); p50/p99 render latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def build_workload(n, psize, n_cams, sizes=(128, 256, 512, 1024)):
+    from yt_amd import data, render
+    from yt_amd.transfer_functions import gaussian_ctf
+
+    t = time.time()
+    rho = data.lognormal_density(n, seed=1001, dtype=np.float32)
+    log(f"field {n}^3 in {time.time() - t:.1f}s")
+    ds = data.load_uniform_grid({"density": rho}, psize=(psize,) * 3)
+    lo, hi = float(np.log10(rho.min())), float(np.log10(rho.max()))
+    tf = gaussian_ctf((lo, hi), n_gauss=8, nbins=256)
+    rng = np.random.default_rng(1004)
+    rs = np.random.default_rng(1006)
+    cams = []
+    for i in range(n_cams):
+        nv = rng.normal(size=3)
+        nv /= np.linalg.norm(nv)
+        north = rng.normal(size=3)
+        res = int(rs.choice(sizes))
+        p = render.camera_params((0.5, 0.5, 0.5), nv, 1.0, res, north)
+        cams.append((i % 2, p))
+    return rho, ds, tf, cams
+
+
+def make_specs(vol, tf, cams, record=False):
+    specs = []
+    for kind, p in cams:
+        if kind == 0:
+            specs.append(vol.projection_request(p, record=record))
+        else:
+            specs.append(vol.vr_request(p, tf, n_samples=10, record=record))
+    return specs
+
+
+def alg_bytes(kind, samples, npix, nf=1, n_samples=10):
+    """Algorithmic HBM bytes: projection n_fields*4 B per sample, VR
+    (8*n_fields*4 + 1)/n_samples B per sub-sample, plus the image channels
+    written (nf for projection, 3 for VR) at 4 B."""
+    if kind == 0:
+        return samples * nf * 4.0 + npix * nf * 4.0
+    return samples * (8.0 * nf * 4.0 + 1.0) / n_samples + npix * 3 * 4.0
+
+
+def cpu_baseline(ds, vol, tf, cams, samples, budget_s):
+    """FP64 oracle (OpenMP, all host threads of this process, up to 16) on a
+    bounded sample: requests taken smallest first, alternating kinds, until
+    the time budget is used."""
+    from oracle import oracle
+    from tests.helpers import oracle_proj_bricks, tf_dict
+
+    threads = min(16, len(os.sched_getaffinity(0)))
+    pbricks = oracle_proj_bricks(ds, ["density"])
+    vbricks = []
+    for node in vol.leaves:
+        pg = vol.kd.get_brick_data(node)
+        vbricks.append(dict(le=pg.LeftEdge, re=pg.RightEdge, dims=pg.dims, data=pg.my_data,
+                            mask=pg.source_mask))
+        node.data = None
+    tfd = tf_dict(tf)
+    order = sorted(range(len(cams)), key=lambda i: (cams[i][1]["nv"][0], cams[i][0]))
+    tot_s, tot_samp, done = 0.0, 0, []
+    for i in order:
+        kind, p = cams[i]
+        t = time.perf_counter()
+        if kind == 0:
+            oracle.cast(0, p, pbricks, num_threads=threads)
+        else:
+            oracle.cast(1, p, vbricks, order=vol.vr_order(p["viewpoint"]), tf=tfd, n_samples=10,
+                        num_threads=threads)
+        dt = time.perf_counter() - t
+        tot_s += dt
+        tot_samp += samples[i]
+        done.append(f"{'proj' if kind == 0 else 'vr'}{p['nv'][0]}")
+        log(f"cpu oracle {done[-1]}: {dt:.2f}s")
+        if tot_s >= budget_s:
+            break
+    return {
+        "value": tot_samp / tot_s / 1e9,
+        "unit": "Gsamples/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{len(done)} of the 64 requests, smallest first ({', '.join(done)}), "
+                  f"FP64 C oracle with OpenMP static/100, {tot_samp} samples in {tot_s:.1f}s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--psize", type=int, default=2, help="grids (bricks) per axis")
+    ap.add_argument("--cams", type=int, default=64)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--lat-reps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(every GPU box here has one GPU); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from yt_amd import _lib, render
+
+    rho, ds, tf, cams = build_workload(args.n, args.psize, args.cams)
+    ctx = _lib.Context(0)
+    t = time.time()
+    vol = render.ResidentVolume(ctx, ds, proj_fields="density", vr_field="density")
+    del rho
+    log(f"resident: {vol.n_proj_bricks} projection blocks, {vol.n_vr_bricks} VR bricks, "
+        f"{vol.device_bytes() / 2**30:.2f} GiB in {time.time() - t:.1f}s")
+
+    # Sample counts (untimed record pass): visits with dt > 0 for projection,
+    # mask-1 visits x n_samples for VR.
+    rec = _lib.Batch(ctx, make_specs(vol, tf, cams, record=True), _lib.BATCH_PERSISTENT)
+    rec.run()
+    samples, npix, visits = [], [], 0
+    for i in range(len(cams)):
+        v, _, s = rec.read_record(i)
+        samples.append(int(s.sum()))
+        npix.append(int(v.size))
+        visits += int(v.sum())
+    rec.close()
+    step_samples = sum(samples)
+    log(f"samples/step {step_samples:.4e}, visits/step {visits:.4e}, pixels/step {sum(npix)}")
+
+    flags = _lib.BATCH_PERSISTENT | (0 if args.no_graph else _lib.BATCH_GRAPH)
+    specs = make_specs(vol, tf, cams)
+    batch = _lib.Batch(ctx, specs, flags)
+    for _ in range(args.warmup):
+        batch.run()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.launch()
+    batch.sync()
+    wall = time.perf_counter() - t0
+    ms_step = wall * 1e3 / args.steps
+    gsps = step_samples / (wall / args.steps) / 1e9
+    imgs_s = len(cams) / (wall / args.steps)
+    log(f"batched: {ms_step:.2f} ms/step, {gsps:.1f} Gsamples/s, {imgs_s:.1f} images/s")
+    batch.close()
+
+    # Per-kind persistent kernels (dominant-kernel roofline).
+    kinds = {}
+    for kind in (0, 1):
+        idx = [i for i in range(len(cams)) if cams[i][0] == kind]
+        b = _lib.Batch(ctx, [specs[i] for i in idx], _lib.BATCH_PERSISTENT)
+        b.run()
+        tms = []
+        for _ in range(max(3, args.steps)):
+            tms.append(b.run())
+        b.close()
+        ms = float(np.mean(tms))
+        byt = sum(alg_bytes(kind, samples[i], npix[i]) for i in idx)
+        kinds[kind] = dict(ms=ms, bytes=byt, samples=sum(samples[i] for i in idx))
+        log(f"{'projection' if kind == 0 else 'VR'} persistent kernel: {ms:.3f} ms, "
+            f"{byt / ms / 1e6:.1f} GB/s algorithmic")
+    dom = max(kinds, key=lambda k: kinds[k]["ms"])
+    ach = kinds[dom]["bytes"] / kinds[dom]["ms"] / 1e6
+
+    # Plain: one launch + sync per image.
+    plain = None
+    if not args.no_plain:
+        singles = [_lib.Batch(ctx, [s], _lib.BATCH_PLAIN) for s in specs]
+        for b in singles:
+            b.run()
+        t0 = time.perf_counter()
+        for b in singles:
+            b.launch()
+            b.sync()
+        pw = time.perf_counter() - t0
+        for b in singles:
+            b.close()
+        plain = dict(images_per_s=len(cams) / pw, ms_per_step=pw * 1e3)
+        log(f"plain one-call-per-image: {pw * 1e3:.2f} ms for {len(cams)} images "
+            f"({len(cams) / pw:.1f} images/s); batched speed-up {imgs_s / plain['images_per_s']:.2f}x")
+
+    # Latency of single 1024^2 requests (launch -> sync on the host clock).
+    lat = {}
+    for kind, name in ((1, "vr_1024"), (0, "proj_1024")):
+        p = render.camera_params((0.5, 0.5, 0.5), (0.3, 0.5, 0.81), 1.0, 1024)
+        s = vol.projection_request(p) if kind == 0 else vol.vr_request(p, tf)
+        b = _lib.Batch(ctx, [s], _lib.BATCH_PERSISTENT)
+        b.run()
+        xs = []
+        for _ in range(args.lat_reps):
+            t0 = time.perf_counter()
+            b.launch()
+            b.sync()
+            xs.append((time.perf_counter() - t0) * 1e3)
+        b.close()
+        lat[name] = dict(p50_ms=float(np.percentile(xs, 50)), p99_ms=float(np.percentile(xs, 99)))
+    log(f"latency {lat}")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(ds, vol, tf, cams, samples, args.cpu_seconds)
+        log(f"cpu baseline {cpu['value']:.4f} Gsamples/s on {cpu['cores']} threads")
+
+    vol.free()
+    ctx.close()
+    out = {
+        "metric": METRIC,
+        "value": round(gsps, 3),
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.n}^3 float32 lognormal (k^-3, sigma_ln=1), {args.psize}^3 bricks, "
+                        f"{args.cams} cameras (sides 128-1024), alternating projection / VR "
+                        "(8-Gaussian CTF, 10 sub-samples), persistent tile-queue kernels"
+                        + ("" if args.no_graph else " in a hipGraph"),
+            "images": args.cams,
+            "pixels": int(sum(npix)),
+            "samples_per_step": int(step_samples),
+            "visits_per_step": int(visits),
+            "dda": "f64",
+            "parallelism": "1 GPU",
+        },
+        "images_per_s": round(imgs_s, 2),
+        "plain_images_per_s": None if plain is None else round(plain["images_per_s"], 2),
+        "latency_ms": lat,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_persist<VR>" if dom == 1 else "k_persist<projection>",
+            "achieved": round(ach, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel_ms": round(kinds[dom]["ms"], 4),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

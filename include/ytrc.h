@@ -142,8 +142,12 @@ int ytrc_batch_read_image(ytrc_batch *b, int32_t i, double *image_out);
 /* Record mode only: per-pixel visit count, visit hash, sample count. */
 int ytrc_batch_read_record(ytrc_batch *b, int32_t i, int64_t *visits, uint64_t *hashes,
                            int64_t *samples);
-/* Tuning: rays per tile edge (8x8 default) and waves per workgroup. */
+/* Launch shape: waves (8x8 ray tiles) per workgroup (1, 2 or 4) and, for
+ * persistent batches, workgroups per CU (0 = the occupancy limit). */
 int ytrc_batch_set_launch(ytrc_batch *b, int32_t waves_per_group, int32_t groups_per_cu);
+/* Plain, uncaptured batches: GPU time of request i's kernel in the last
+ * launch, from HIP events recorded around it on the launch stream. */
+int ytrc_batch_request_ms(ytrc_batch *b, int32_t i, float *ms);
 /* Device bytes held by the batch (descriptors, extents, outputs). */
 int ytrc_batch_bytes(const ytrc_batch *b, uint64_t *bytes);
 void ytrc_batch_destroy(ytrc_batch *b);

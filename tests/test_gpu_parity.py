@@ -171,6 +171,39 @@ def test_batch_many_images(ctx):
         for i, s in enumerate(specs):
             (alone, _), = _run(ctx, [s])
             assert np.array_equal(alone, together[i][0]), i
+        # The persistent tile queue and its hipGraph replay give the same
+        # images as one launch per request, on every replay.
+        pers = _run(ctx, specs, _lib.BATCH_PERSISTENT)
+        for i in range(len(specs)):
+            assert np.array_equal(pers[i][0], together[i][0]), i
+        b = _lib.Batch(ctx, specs, _lib.BATCH_PERSISTENT | _lib.BATCH_GRAPH)
+        try:
+            for rep in range(3):
+                b.run()
+                for i in range(len(specs)):
+                    assert np.array_equal(b.read_image(i), together[i][0]), (rep, i)
+        finally:
+            b.close()
+    finally:
+        vol.free()
+
+
+def test_persistent_record_matches_oracle(ctx):
+    """Record mode through the persistent kernel: visit sequences of a mixed
+    batch equal the oracle's."""
+    ds = uniform_lognormal(32, seed=31, nprocs=4)
+    tf = vr_tf_for(ds, nbins=64)
+    vol = render.ResidentVolume(ctx, ds, proj_fields="density", vr_field="density")
+    try:
+        pp = c0_params(72, normal=(0.1, 0.9, -0.4))
+        pv = c0_params(56, normal=(-0.5, 0.2, 0.8))
+        specs = [vol.projection_request(pp, record=True), vol.vr_request(pv, tf, record=True)]
+        out = _run(ctx, specs, _lib.BATCH_PERSISTENT)
+        _, ov, oh = oracle.cast(0, pp, oracle_proj_bricks(ds, ["density"]), record=True)
+        _check_visits(out[0][1], ov, oh, "persistent proj")
+        _, ov, oh = oracle.cast(1, pv, oracle_vr_bricks(vol), order=specs[1].order,
+                                tf=tf_dict(tf), record=True)
+        _check_visits(out[1][1], ov, oh, "persistent vr")
     finally:
         vol.free()
 

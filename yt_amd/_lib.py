@@ -49,6 +49,7 @@ EXPORTED = [
     "ytrc_batch_bytes",
     "ytrc_batch_destroy",
     "ytrc_batch_kernel_ms",
+    "ytrc_batch_request_ms",
 ]
 
 
@@ -140,6 +141,7 @@ def _declare(lib):
     lib.ytrc_batch_destroy.argtypes = [P]
     lib.ytrc_batch_destroy.restype = None
     lib.ytrc_batch_kernel_ms.argtypes = [P, C.POINTER(C.c_float), C.POINTER(C.c_int32)]
+    lib.ytrc_batch_request_ms.argtypes = [P, C.c_int32, C.POINTER(C.c_float)]
     for name in EXPORTED:
         f = getattr(lib, name)
         if name not in ("ytrc_ctx_destroy", "ytrc_batch_destroy", "ytrc_last_error"):
@@ -377,6 +379,11 @@ class Batch:
         ms, n = C.c_float(), C.c_int32()
         _check(lib().ytrc_batch_kernel_ms(self._h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def request_ms(self, i):
+        ms = C.c_float()
+        _check(lib().ytrc_batch_request_ms(self._h, int(i), C.byref(ms)))
+        return ms.value
 
     def nbytes(self):
         v = C.c_uint64()

@@ -22,6 +22,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/ytrc.h"
@@ -385,6 +386,57 @@ __global__ __launch_bounds__(256) void k_render(const DImg *__restrict__ imgs, i
     render_tile<KIND, NFP, REC>(im, tile, threadIdx.x & 63, s_tf);
 }
 
+// Persistent launch: every wave repeatedly takes the next (image, tile)
+// item of a global queue until the queue is drained.  All transfer-function
+// tables of the batch sit in LDS; each image reads its own slot.
+template <int KIND, int NFP, bool REC>
+__global__ __launch_bounds__(256) void k_persist(const DImg *__restrict__ imgs,
+                                                 const int2 *__restrict__ items, int n_items,
+                                                 unsigned int *__restrict__ counter,
+                                                 const float4 *__restrict__ tf_all, int tf_bins)
+{
+    extern __shared__ float4 s_tf[];
+    if (KIND == 1) {
+        for (int i = threadIdx.x; i < tf_bins; i += blockDim.x) s_tf[i] = tf_all[i];
+        __syncthreads();
+    }
+    const int lane = threadIdx.x & 63;
+    while (true) {
+        unsigned int it = 0;
+        if (lane == 0) it = atomicAdd(counter, 1u);
+        it = __shfl(it, 0);
+        if (it >= (unsigned int)n_items) break;
+        const int2 w = items[it];
+        const DImg &im = imgs[w.x];
+        render_tile<KIND, NFP, REC>(im, w.y, lane, s_tf + im.tf_slot);
+    }
+}
+
+__global__ void k_zero_counters(unsigned int *c, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) c[i] = 0u;
+}
+
+typedef void (*pfn_t)(const DImg *, const int2 *, int, unsigned int *, const float4 *, int);
+
+template <int KIND, bool REC>
+pfn_t pick_persist_nfp(int nfp)
+{
+    switch (nfp) {
+    case 1: return k_persist<KIND, 1, REC>;
+    case 2: return k_persist<KIND, 2, REC>;
+    case 4: return k_persist<KIND, 4, REC>;
+    default: return KIND == 1 ? k_persist<KIND, 8, REC> : nullptr;
+    }
+}
+
+pfn_t pick_persist(int kind, int nfp, bool rec)
+{
+    if (kind == 0) return rec ? pick_persist_nfp<0, true>(nfp) : pick_persist_nfp<0, false>(nfp);
+    return rec ? pick_persist_nfp<1, true>(nfp) : pick_persist_nfp<1, false>(nfp);
+}
+
 typedef void (*kfn_t)(const DImg *, int);
 
 template <int KIND, bool REC>
@@ -467,6 +519,19 @@ struct ReqHost {
 
 }  // namespace
 
+// One persistent launch: every request of one kernel variant.
+struct PGroup {
+    int kind = 0;
+    int nfp = 1;
+    bool rec = false;
+    pfn_t fn = nullptr;
+    int n_items = 0;
+    const int2 *items = nullptr;
+    int counter = 0;
+    size_t lds = 0;
+    int grid = 0;
+};
+
 struct ytrc_batch {
     ytrc_ctx *ctx = nullptr;
     uint32_t flags = 0;
@@ -476,7 +541,15 @@ struct ytrc_batch {
     char *arena = nullptr;
     uint64_t arena_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> ev_req;  // plain mode: before/after each request
+    std::vector<PGroup> groups;      // persistent mode
+    unsigned int *counters = nullptr;
+    const float4 *tf_all = nullptr;
+    int tf_bins = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
     int waves_per_group = 4;
+    int groups_per_cu = 0;  // 0: occupancy limit
     int launched = 0;
     double total_ms = 0.0;
     int timed = 0;
@@ -715,7 +788,7 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
     // Pass 1: validate and size the arena.
     size_t off = 0;
     struct Offs {
-        size_t order, ext, zbuf, init, out, rv, rh, rs, tf, tft;
+        size_t order, ext, zbuf, init, out, rv, rh, rs, tf;
     };
     std::vector<Offs> offs(n_req);
     std::vector<std::vector<int4>> exts(n_req);
@@ -866,14 +939,60 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
             o.rs = off;
             off = align_up(off + 8 * q.npix, 256);
         }
-        o.tf = o.tft = off;
-        if (r.kind == YTRC_VOLUME_RENDER) {
-            o.tf = off;
-            off = align_up(off + sizeof(DTF), 256);
-            o.tft = off;
-            off = align_up(off + sizeof(float4) * tft[i].size(), 256);
-        }
+        o.tf = off;
+        if (r.kind == YTRC_VOLUME_RENDER) off = align_up(off + sizeof(DTF), 256);
     }
+    // Unique transfer-function tables share one slot of the batch table.
+    std::vector<float4> tf_all;
+    std::map<std::string, int> tf_slot_of;
+    std::vector<int> slot(n_req, 0);
+    for (int i = 0; i < n_req; i++) {
+        if (reqs[i].kind != YTRC_VOLUME_RENDER) continue;
+        std::string key((const char *)tft[i].data(), tft[i].size() * sizeof(float4));
+        auto it = tf_slot_of.find(key);
+        if (it == tf_slot_of.end()) {
+            const int base = (int)tf_all.size();
+            tf_all.insert(tf_all.end(), tft[i].begin(), tft[i].end());
+            it = tf_slot_of.emplace(key, base).first;
+        }
+        slot[i] = it->second;
+    }
+    const size_t tfall_off = off;
+    off = align_up(off + sizeof(float4) * std::max<size_t>(1, tf_all.size()), 256);
+    // Persistent groups: one per kernel variant, items = (request, tile),
+    // larger images first.
+    std::map<std::tuple<int, int, int>, std::vector<int>> gmap;
+    for (int i = 0; i < n_req; i++) {
+        const bool rec = (reqs[i].flags & YTRC_FLAG_RECORD) != 0;
+        gmap[std::make_tuple(reqs[i].kind, B->reqs[i].nfp, rec ? 1 : 0)].push_back(i);
+    }
+    std::vector<std::vector<int2>> gitems;
+    std::vector<size_t> gitems_off;
+    for (auto &kv : gmap) {
+        std::vector<int> ids = kv.second;
+        std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) {
+            return B->reqs[a].ntiles > B->reqs[b].ntiles;
+        });
+        std::vector<int2> items;
+        for (int i : ids)
+            for (int t = 0; t < B->reqs[i].ntiles; t++) items.push_back(make_int2(i, t));
+        PGroup g;
+        g.kind = std::get<0>(kv.first);
+        g.nfp = std::get<1>(kv.first);
+        g.rec = std::get<2>(kv.first) != 0;
+        g.fn = pick_persist(g.kind, g.nfp, g.rec);
+        g.n_items = (int)items.size();
+        g.counter = (int)B->groups.size();
+        g.lds = (g.kind == YTRC_VOLUME_RENDER) ? sizeof(float4) * tf_all.size() : 0;
+        if (g.lds > 64 * 1024)
+            return bad(YTRC_EINVAL, "ytrc_batch_create: the batch's distinct transfer functions exceed 64 KiB of LDS");
+        B->groups.push_back(g);
+        gitems_off.push_back(off);
+        off = align_up(off + sizeof(int2) * std::max<size_t>(1, items.size()), 256);
+        gitems.push_back(std::move(items));
+    }
+    const size_t counters_off = off;
+    off = align_up(off + sizeof(unsigned int) * std::max<size_t>(1, B->groups.size()), 256);
     const size_t imgs_off = off;
     off = align_up(off + sizeof(DImg) * n_req, 256);
     B->arena_bytes = off;
@@ -934,46 +1053,135 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
             HIPCHK(hipMemcpy(A + o.init, stage.data(), sizeof(float) * 4 * q.npix, hipMemcpyHostToDevice));
         }
         if (r.kind == YTRC_VOLUME_RENDER) {
-            tfs[i].table = (const float4 *)(A + o.tft);
-            HIPCHK(hipMemcpy(A + o.tft, tft[i].data(), sizeof(float4) * tft[i].size(), hipMemcpyHostToDevice));
+            tfs[i].table = (const float4 *)(A + tfall_off) + slot[i];
+            d.tf_slot = slot[i];
             HIPCHK(hipMemcpy(A + o.tf, &tfs[i], sizeof(DTF), hipMemcpyHostToDevice));
         }
     }
+    B->tf_all = (const float4 *)(A + tfall_off);
+    B->tf_bins = (int)tf_all.size();
+    if (!tf_all.empty())
+        HIPCHK(hipMemcpy(A + tfall_off, tf_all.data(), sizeof(float4) * tf_all.size(), hipMemcpyHostToDevice));
+    for (size_t g = 0; g < B->groups.size(); g++) {
+        B->groups[g].items = (const int2 *)(A + gitems_off[g]);
+        if (!gitems[g].empty())
+            HIPCHK(hipMemcpy(A + gitems_off[g], gitems[g].data(), sizeof(int2) * gitems[g].size(),
+                             hipMemcpyHostToDevice));
+    }
+    B->counters = (unsigned int *)(A + counters_off);
     B->d_imgs = (DImg *)(A + imgs_off);
     HIPCHK(hipMemcpy(B->d_imgs, B->h_imgs.data(), sizeof(DImg) * n_req, hipMemcpyHostToDevice));
     HIPCHK(hipEventCreate(&B->ev0));
     HIPCHK(hipEventCreate(&B->ev1));
+    if (!(batch_flags & YTRC_BATCH_PERSISTENT)) {
+        B->ev_req.resize(2 * n_req);
+        for (auto &e : B->ev_req) HIPCHK(hipEventCreate(&e));
+    }
     *out = B;
     return YTRC_OK;
 }
 
 int ytrc_batch_set_launch(ytrc_batch *b, int32_t waves_per_group, int32_t groups_per_cu)
 {
-    (void)groups_per_cu;
     if (b == nullptr) return fail(YTRC_EINVAL, "ytrc_batch_set_launch: NULL batch");
     if (waves_per_group != 1 && waves_per_group != 2 && waves_per_group != 4)
         return fail(YTRC_EINVAL, "ytrc_batch_set_launch: waves_per_group must be 1, 2 or 4");
+    if (groups_per_cu < 0 || groups_per_cu > 64)
+        return fail(YTRC_EINVAL, "ytrc_batch_set_launch: groups_per_cu must be in 0..64");
     b->waves_per_group = waves_per_group;
+    b->groups_per_cu = groups_per_cu;
+    if (b->gexec) {
+        hipGraphExecDestroy(b->gexec);
+        hipGraphDestroy(b->graph);
+        b->gexec = nullptr;
+        b->graph = nullptr;
+    }
     return YTRC_OK;
 }
+
+namespace {
+
+int enqueue_persistent(ytrc_batch *b, hipStream_t s)
+{
+    const int ng = (int)b->groups.size();
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, b->counters, ng);
+    HIPCHK(hipGetLastError());
+    const int threads = 64 * b->waves_per_group;
+    for (auto &g : b->groups) {
+        if (g.n_items == 0) continue;
+        int per_cu = b->groups_per_cu;
+        if (per_cu == 0) {
+            int nb = 0;
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, g.fn, threads, g.lds));
+            per_cu = std::max(1, nb);
+        }
+        g.grid = std::max(1, std::min(per_cu * b->ctx->n_cu,
+                                      (g.n_items + b->waves_per_group - 1) / b->waves_per_group));
+        hipLaunchKernelGGL(g.fn, dim3(g.grid), dim3(threads), g.lds, s, (const DImg *)b->d_imgs,
+                           g.items, g.n_items, b->counters + g.counter, b->tf_all, b->tf_bins);
+        HIPCHK(hipGetLastError());
+    }
+    return YTRC_OK;
+}
+
+int enqueue_plain(ytrc_batch *b, hipStream_t s, bool events)
+{
+    const int wpg = b->waves_per_group;
+    for (size_t i = 0; i < b->reqs.size(); i++) {
+        const ReqHost &q = b->reqs[i];
+        const int groups = (q.ntiles + wpg - 1) / wpg;
+        if (events) HIPCHK(hipEventRecord(b->ev_req[2 * i], s));
+        if (groups > 0) {
+            hipLaunchKernelGGL(q.kfn, dim3(groups), dim3(64 * wpg), q.lds, s,
+                               (const DImg *)b->d_imgs, (int)i);
+            HIPCHK(hipGetLastError());
+        }
+        if (events) HIPCHK(hipEventRecord(b->ev_req[2 * i + 1], s));
+    }
+    return YTRC_OK;
+}
+
+}  // namespace
 
 int ytrc_batch_launch(ytrc_batch *b)
 {
     if (b == nullptr) return fail(YTRC_EINVAL, "ytrc_batch_launch: NULL batch");
     HIPCHK(hipSetDevice(b->ctx->device));
     hipStream_t s = b->ctx->stream;
-    HIPCHK(hipEventRecord(b->ev0, s));
-    const int wpg = b->waves_per_group;
-    for (size_t i = 0; i < b->reqs.size(); i++) {
-        const ReqHost &q = b->reqs[i];
-        const int groups = (q.ntiles + wpg - 1) / wpg;
-        if (groups == 0) continue;
-        hipLaunchKernelGGL(q.kfn, dim3(groups), dim3(64 * wpg), q.lds, s,
-                           (const DImg *)b->d_imgs, (int)i);
-        HIPCHK(hipGetLastError());
+    const bool persistent = (b->flags & YTRC_BATCH_PERSISTENT) != 0;
+    if ((b->flags & YTRC_BATCH_GRAPH) && b->gexec == nullptr) {
+        // Capture the launch sequence once; replays reuse it.
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        int rc = persistent ? enqueue_persistent(b, s) : enqueue_plain(b, s, false);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc != YTRC_OK) return rc;
+        if (e != hipSuccess) return fail(YTRC_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        b->graph = g;
+        HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
     }
+    HIPCHK(hipEventRecord(b->ev0, s));
+    int rc;
+    if (b->gexec) {
+        HIPCHK(hipGraphLaunch(b->gexec, s));
+        rc = YTRC_OK;
+    } else {
+        rc = persistent ? enqueue_persistent(b, s) : enqueue_plain(b, s, true);
+    }
+    if (rc != YTRC_OK) return rc;
     HIPCHK(hipEventRecord(b->ev1, s));
     b->launched += 1;
+    return YTRC_OK;
+}
+
+int ytrc_batch_request_ms(ytrc_batch *b, int32_t i, float *ms)
+{
+    if (b == nullptr || ms == nullptr) return fail(YTRC_EINVAL, "ytrc_batch_request_ms: NULL");
+    if (i < 0 || i >= (int)b->reqs.size()) return fail(YTRC_EINVAL, "ytrc_batch_request_ms: bad index");
+    if (b->ev_req.empty() || b->gexec)
+        return fail(YTRC_EINVAL, "ytrc_batch_request_ms: per-request times exist for plain, uncaptured batches only");
+    HIPCHK(hipEventSynchronize(b->ev_req[2 * i + 1]));
+    HIPCHK(hipEventElapsedTime(ms, b->ev_req[2 * i], b->ev_req[2 * i + 1]));
     return YTRC_OK;
 }
 
@@ -1057,6 +1265,9 @@ void ytrc_batch_destroy(ytrc_batch *b)
     }
     if (b->ev0) hipEventDestroy(b->ev0);
     if (b->ev1) hipEventDestroy(b->ev1);
+    for (auto &e : b->ev_req) hipEventDestroy(e);
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    if (b->graph) hipGraphDestroy(b->graph);
     delete b;
 }
 
