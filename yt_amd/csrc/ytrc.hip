@@ -123,16 +123,69 @@ __device__ __forceinline__ float pick(const float (&a)[N], int i)
     return r;
 }
 
-// FIT_get_value + FIT_eval_transfer (field_interpolation_tables.pxd:66-110)
-// with the tables in LDS as float4 {y, dy, d0, 0}.  The reference's
-// sequential weight-table products (istorage[i] *= istorage[weight_table_id])
-// and the channel links (trgba[c] = istorage[field_table_ids[c]]) are
-// resolved on the host into chexp[c]: 2 bits per raw table u giving the power
-// of table u's value in channel c, so the device never indexes a register
-// array with a runtime value.
+// Transfer-function parameters of one image, loaded once per tile into
+// (uniform) registers.  mode 1 is the ColorTransferFunction pattern
+// (transfer_functions.py:385-408): four tables on one field with shared
+// bounds, red/green/blue weighted by the alpha table, channels linked 0..3;
+// its LDS table holds per bin {y_r, y_g, y_b, y_a}, {dy_r..dy_a} and d0.
+// mode 0 is the general FieldInterpolationTable set.
+struct TFCache {
+    int mode;
+    int n_fits;
+    int grey;
+    int fid;       // mode 1: the one field
+    float b0, b1, idbin;
+    int nbm2;      // mode 1: nbins - 2
+    int nbins[6];
+    int off[6];
+    int field_id[6];
+    int wfield[6];
+    int chexp[4];
+    float tb0[6], tb1[6], tidbin[6];
+};
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float unif(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// Loads the (wave-uniform) table parameters into scalar registers.
+__device__ __forceinline__ void load_tf(const DTF &T, TFCache &c)
+{
+    c.mode = uni(T.mode);
+    c.n_fits = uni(T.n_fits);
+    c.grey = uni(T.grey);
+    c.fid = uni(T.field_id[0]);
+    c.b0 = unif(T.b0[0]);
+    c.b1 = unif(T.b1[0]);
+    c.idbin = unif(T.idbin[0]);
+    c.nbm2 = uni(T.nbins[0] - 2);
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        c.nbins[t] = uni(T.nbins[t]);
+        c.off[t] = uni(T.off[t]);
+        c.field_id[t] = uni(T.field_id[t]);
+        c.wfield[t] = uni(T.wfield[t]);
+        c.tb0[t] = unif(T.b0[t]);
+        c.tb1[t] = unif(T.b1[t]);
+        c.tidbin[t] = unif(T.idbin[t]);
+    }
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++) c.chexp[ch] = uni(T.chexp[ch]);
+}
+
+// FIT_get_value + FIT_eval_transfer (field_interpolation_tables.pxd:66-110),
+// general table set, tables in LDS as float4 {y, dy, d0, 0}.  The
+// reference's sequential weight-table products (istorage[i] *=
+// istorage[weight_table_id]) and channel links (trgba[c] =
+// istorage[field_table_ids[c]]) are resolved on the host into chexp[c]:
+// 2 bits per raw table u giving the power of table u's value in channel c.
+// Table reads are branch-free: out-of-range values read a clamped bin and
+// are then zeroed.
 template <int NFP>
-__device__ __forceinline__ void fit_eval(const DTF &tf, const float4 *__restrict__ lt,
-                                         const float (&dvs)[NFP], float e[4])
+__device__ __forceinline__ void fit_eval_general(const TFCache &tf, const float4 *__restrict__ lt,
+                                                 const float (&dvs)[NFP], float e[4])
 {
     float raw[6];
 #pragma unroll
@@ -140,14 +193,13 @@ __device__ __forceinline__ void fit_eval(const DTF &tf, const float4 *__restrict
         raw[t] = 0.0f;
         if (t < tf.n_fits) {
             const float v = (NFP == 1) ? dvs[0] : pick(dvs, tf.field_id[t]);
-            if (v < tf.b1[t] && v > tf.b0[t] && __builtin_isnormal(v)) {
-                int bin = (int)((v - tf.b0[t]) * tf.idbin[t]);
-                bin = bin < 0 ? 0 : (bin > tf.nbins[t] - 2 ? tf.nbins[t] - 2 : bin);
-                const float4 q = lt[tf.off[t] + bin];
-                float r = __builtin_fmaf(v - q.z, q.y, q.x);
-                if (tf.wfield[t] != -1) r *= (NFP == 1) ? dvs[0] : pick(dvs, tf.wfield[t]);
-                raw[t] = r;
-            }
+            const bool ok = (v < tf.tb1[t]) && (v > tf.tb0[t]) && __builtin_isnormal(v);
+            int bin = (int)((v - tf.tb0[t]) * tf.tidbin[t]);
+            bin = bin < 0 ? 0 : (bin > tf.nbins[t] - 2 ? tf.nbins[t] - 2 : bin);
+            const float4 q = lt[tf.off[t] + bin];
+            float r = __builtin_fmaf(v - q.z, q.y, q.x);
+            if (tf.wfield[t] != -1) r *= (NFP == 1) ? dvs[0] : pick(dvs, tf.wfield[t]);
+            raw[t] = ok ? r : 0.0f;
         }
     }
 #pragma unroll
@@ -165,16 +217,63 @@ __device__ __forceinline__ void fit_eval(const DTF &tf, const float4 *__restrict
     }
 }
 
-// VolumeRenderSampler.sample (image_samplers.pyx:453-487).
-template <int NFP, bool REC>
+// The ColorTransferFunction case of the same evaluation: one bin for the
+// four tables, e_c = table_c(v) * table_alpha(v) for c < 3, e_3 = alpha.
+__device__ __forceinline__ void fit_eval_ctf(const TFCache &tf, const float4 *__restrict__ lt,
+                                             int nb, float v, float e[4])
+{
+    const bool ok = (v < tf.b1) && (v > tf.b0) && __builtin_isnormal(v);
+    int bin = (int)((v - tf.b0) * tf.idbin);
+    bin = bin < 0 ? 0 : (bin > tf.nbm2 ? tf.nbm2 : bin);
+    const float4 y = lt[bin];
+    const float4 dy = lt[nb + bin];
+    const float d0 = reinterpret_cast<const float *>(lt + 2 * nb)[bin];
+    const float dd = v - d0;
+    const float a = ok ? __builtin_fmaf(dd, dy.w, y.w) : 0.0f;
+    const float r = ok ? __builtin_fmaf(dd, dy.x, y.x) : 0.0f;
+    const float g = ok ? __builtin_fmaf(dd, dy.y, y.y) : 0.0f;
+    const float b = ok ? __builtin_fmaf(dd, dy.z, y.z) : 0.0f;
+    e[0] = r * a;
+    e[1] = g * a;
+    e[2] = b * a;
+    e[3] = a;
+}
+
+// Brick fields the samplers read, copied by value (uniform across a wave).
+struct BrickReg {
+    double le[3];
+    double dds[3];
+    double idds[3];
+    int d0, d1, d2;
+};
+
+__device__ __forceinline__ BrickReg brick_reg(const DBrick &b)
+{
+    BrickReg r;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        r.le[i] = b.le[i];
+        r.dds[i] = b.dds[i];
+        r.idds[i] = b.idds[i];
+    }
+    r.d0 = b.dims[0];
+    r.d1 = b.dims[1];
+    r.d2 = b.dims[2];
+    return r;
+}
+
+// VolumeRenderSampler.sample (image_samplers.pyx:453-487).  MODE is the
+// transfer-function path (1 = ColorTransferFunction, 0 = general).
+template <int NFP, bool REC, int MODE>
 struct VRSampler {
     const float *data;
     const uint8_t *mask;
-    const DBrick *b;
+    BrickReg b;
     double pos[3];
     double dir[3];
-    const DTF *tf;
+    const TFCache *tf;
     const float4 *lt;
+    int nb;
     double inv_ns;
     int n_samples;
     double rgba[4];
@@ -182,8 +281,8 @@ struct VRSampler {
     __device__ __forceinline__ void operator()(double en, double ex, int c0, int c1, int c2)
     {
         if (REC) rec->visit(en, ex, c0, c1, c2);
-        const int d0 = b->dims[0], d1 = b->dims[1], d2 = b->dims[2];
-        if (!((unsigned)c0 < (unsigned)d0 && (unsigned)c1 < (unsigned)d1 &&
+        const int d1 = b.d1, d2 = b.d2;
+        if (!((unsigned)c0 < (unsigned)b.d0 && (unsigned)c1 < (unsigned)d1 &&
               (unsigned)c2 < (unsigned)d2))
             return;
         const int cell = (c0 * d1 + c1) * d2 + c2;
@@ -196,16 +295,14 @@ struct VRSampler {
         const double dt = (ex - en) * inv_ns;
         const double tt = en + 0.5 * dt;
         float dp[3], ds[3];
-        {
 #pragma unroll
-            for (int i = 0; i < 3; i++) {
-                const int ci = (i == 0) ? c0 : ((i == 1) ? c1 : c2);
-                double p = tt * dir[i] + pos[i];
-                p -= (double)ci * b->dds[i] + b->le[i];
-                p *= b->idds[i];
-                dp[i] = (float)p;
-                ds[i] = (float)(dir[i] * b->idds[i] * dt);
-            }
+        for (int i = 0; i < 3; i++) {
+            const int ci = (i == 0) ? c0 : ((i == 1) ? c1 : c2);
+            double p = tt * dir[i] + pos[i];
+            p -= (double)ci * b.dds[i] + b.le[i];
+            p *= b.idds[i];
+            dp[i] = (float)p;
+            ds[i] = (float)(dir[i] * b.idds[i] * dt);
         }
         float cv[8][NFP];
         {
@@ -219,7 +316,7 @@ struct VRSampler {
             FieldLoad<NFP>::get(p0 + (size_t)(s0 + s1) * NFP, cv[6]);
             FieldLoad<NFP>::get(p0 + (size_t)(s0 + s1 + 1) * NFP, cv[7]);
         }
-        const DTF &T = *tf;
+        const TFCache &T = *tf;
         for (int s = 0; s < n_samples; s++) {
             float dvs[NFP];
             const float dv2 = 1.0f - dp[2];
@@ -236,7 +333,10 @@ struct VRSampler {
                 dvs[f] = __builtin_fmaf(dp[0], y1, dv0 * y0);
             }
             float e[4];
-            fit_eval<NFP>(T, lt, dvs, e);
+            if (MODE == 1)
+                fit_eval_ctf(T, lt, nb, (NFP == 1) ? dvs[0] : pick(dvs, T.fid), e);
+            else
+                fit_eval_general<NFP>(T, lt, dvs, e);
             if (T.grey) {
                 const double ta = y_fmax(1.0 - dt * (double)e[3], 0.0);
 #pragma unroll
@@ -256,8 +356,9 @@ struct VRSampler {
     }
 };
 
-// Casts the rays of one 8x8 tile of image `im` through its bricks.
-template <int KIND, int NFP, bool REC>
+// Casts the rays of one 8x8 tile of image `im` through its bricks.  `im`
+// and `tile` are wave-uniform.
+template <int KIND, int NFP, bool REC, int MODE>
 __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
                                             const float4 *lt)
 {
@@ -282,6 +383,10 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
     rec.samples = 0;
     rec.hash = 0;
     rec.brick = 0;
+
+    TFCache tfc;
+    if (KIND == 1) load_tf(*im.tf, tfc);
+    const int nb = (KIND == 1) ? im.tf->nbins[0] : 0;
 
     float ps[NFP], pc[NFP];
     double rgba[4];
@@ -331,17 +436,18 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
                 pc[f] = s.c[f];
             }
         } else {
-            VRSampler<NFP, REC> s;
+            VRSampler<NFP, REC, MODE> s;
             s.data = im.pool + b.data_off;
             s.mask = (b.mask_off >= 0) ? im.masks + b.mask_off : nullptr;
-            s.b = &b;
+            s.b = brick_reg(b);
 #pragma unroll
             for (int i = 0; i < 3; i++) {
                 s.pos[i] = pos[i];
                 s.dir[i] = dir[i];
             }
-            s.tf = im.tf;
+            s.tf = &tfc;
             s.lt = lt;
+            s.nb = nb;
             s.inv_ns = im.inv_ns;
             s.n_samples = im.n_samples;
 #pragma unroll
@@ -370,7 +476,7 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
 }
 
 // Plain launch: one request; blockDim.x/64 tiles per workgroup.
-template <int KIND, int NFP, bool REC>
+template <int KIND, int NFP, bool REC, int MODE>
 __global__ __launch_bounds__(256) void k_render(const DImg *__restrict__ imgs, int idx)
 {
     extern __shared__ float4 s_tf[];
@@ -383,13 +489,15 @@ __global__ __launch_bounds__(256) void k_render(const DImg *__restrict__ imgs, i
     const int waves = blockDim.x >> 6;
     const int tile = blockIdx.x * waves + (threadIdx.x >> 6);
     if (tile >= im.tiles0 * im.tiles1) return;
-    render_tile<KIND, NFP, REC>(im, tile, threadIdx.x & 63, s_tf);
+    render_tile<KIND, NFP, REC, MODE>(im, tile, threadIdx.x & 63, s_tf);
 }
 
 // Persistent launch: every wave repeatedly takes the next (image, tile)
 // item of a global queue until the queue is drained.  All transfer-function
-// tables of the batch sit in LDS; each image reads its own slot.
-template <int KIND, int NFP, bool REC>
+// tables of the batch sit in LDS; each image reads its own slot.  The item
+// index is made wave-uniform so that image, brick and table descriptors are
+// read with scalar loads.
+template <int KIND, int NFP, bool REC, int MODE>
 __global__ __launch_bounds__(256) void k_persist(const DImg *__restrict__ imgs,
                                                  const int2 *__restrict__ items, int n_items,
                                                  unsigned int *__restrict__ counter,
@@ -404,11 +512,13 @@ __global__ __launch_bounds__(256) void k_persist(const DImg *__restrict__ imgs,
     while (true) {
         unsigned int it = 0;
         if (lane == 0) it = atomicAdd(counter, 1u);
-        it = __shfl(it, 0);
+        it = (unsigned int)__builtin_amdgcn_readfirstlane((int)__shfl(it, 0));
         if (it >= (unsigned int)n_items) break;
         const int2 w = items[it];
-        const DImg &im = imgs[w.x];
-        render_tile<KIND, NFP, REC>(im, w.y, lane, s_tf + im.tf_slot);
+        const int img = __builtin_amdgcn_readfirstlane(w.x);
+        const int tile = __builtin_amdgcn_readfirstlane(w.y);
+        const DImg &im = imgs[img];
+        render_tile<KIND, NFP, REC, MODE>(im, tile, lane, s_tf + im.tf_slot);
     }
 }
 
@@ -419,41 +529,42 @@ __global__ void k_zero_counters(unsigned int *c, int n)
 }
 
 typedef void (*pfn_t)(const DImg *, const int2 *, int, unsigned int *, const float4 *, int);
+typedef void (*kfn_t)(const DImg *, int);
 
-template <int KIND, bool REC>
+template <int KIND, bool REC, int MODE>
 pfn_t pick_persist_nfp(int nfp)
 {
     switch (nfp) {
-    case 1: return k_persist<KIND, 1, REC>;
-    case 2: return k_persist<KIND, 2, REC>;
-    case 4: return k_persist<KIND, 4, REC>;
-    default: return KIND == 1 ? k_persist<KIND, 8, REC> : nullptr;
+    case 1: return k_persist<KIND, 1, REC, MODE>;
+    case 2: return k_persist<KIND, 2, REC, MODE>;
+    case 4: return k_persist<KIND, 4, REC, MODE>;
+    default: return KIND == 1 ? k_persist<KIND, 8, REC, MODE> : nullptr;
     }
 }
 
-pfn_t pick_persist(int kind, int nfp, bool rec)
+pfn_t pick_persist(int kind, int nfp, bool rec, int mode)
 {
-    if (kind == 0) return rec ? pick_persist_nfp<0, true>(nfp) : pick_persist_nfp<0, false>(nfp);
-    return rec ? pick_persist_nfp<1, true>(nfp) : pick_persist_nfp<1, false>(nfp);
+    if (kind == 0) return rec ? pick_persist_nfp<0, true, 0>(nfp) : pick_persist_nfp<0, false, 0>(nfp);
+    if (mode == 1) return rec ? pick_persist_nfp<1, true, 1>(nfp) : pick_persist_nfp<1, false, 1>(nfp);
+    return rec ? pick_persist_nfp<1, true, 0>(nfp) : pick_persist_nfp<1, false, 0>(nfp);
 }
 
-typedef void (*kfn_t)(const DImg *, int);
-
-template <int KIND, bool REC>
+template <int KIND, bool REC, int MODE>
 kfn_t pick_kernel_nfp(int nfp)
 {
     switch (nfp) {
-    case 1: return k_render<KIND, 1, REC>;
-    case 2: return k_render<KIND, 2, REC>;
-    case 4: return k_render<KIND, 4, REC>;
-    default: return KIND == 1 ? k_render<KIND, 8, REC> : nullptr;
+    case 1: return k_render<KIND, 1, REC, MODE>;
+    case 2: return k_render<KIND, 2, REC, MODE>;
+    case 4: return k_render<KIND, 4, REC, MODE>;
+    default: return KIND == 1 ? k_render<KIND, 8, REC, MODE> : nullptr;
     }
 }
 
-kfn_t pick_kernel(int kind, int nfp, bool rec)
+kfn_t pick_kernel(int kind, int nfp, bool rec, int mode)
 {
-    if (kind == 0) return rec ? pick_kernel_nfp<0, true>(nfp) : pick_kernel_nfp<0, false>(nfp);
-    return rec ? pick_kernel_nfp<1, true>(nfp) : pick_kernel_nfp<1, false>(nfp);
+    if (kind == 0) return rec ? pick_kernel_nfp<0, true, 0>(nfp) : pick_kernel_nfp<0, false, 0>(nfp);
+    if (mode == 1) return rec ? pick_kernel_nfp<1, true, 1>(nfp) : pick_kernel_nfp<1, false, 1>(nfp);
+    return rec ? pick_kernel_nfp<1, true, 0>(nfp) : pick_kernel_nfp<1, false, 0>(nfp);
 }
 
 }  // namespace
@@ -515,6 +626,7 @@ struct ReqHost {
     size_t lds = 0;
     int nch_out = 0;
     int nfp = 1;
+    int mode = 0;
 };
 
 }  // namespace
@@ -522,6 +634,7 @@ struct ReqHost {
 // One persistent launch: every request of one kernel variant.
 struct PGroup {
     int kind = 0;
+    int mode = 0;
     int nfp = 1;
     bool rec = false;
     pfn_t fn = nullptr;
@@ -905,12 +1018,37 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
                 }
                 T.chexp[c] = ce;
             }
+            // ColorTransferFunction pattern (transfer_functions.py:385-408):
+            // one shared bin per sample, tables laid out per bin.
+            bool ctf = tf.n_fits == 4;
+            for (int t = 0; ctf && t < 4; t++)
+                ctf = tf.field_ids[t] == tf.field_ids[0] && tf.weight_field_ids[t] == -1 &&
+                      tf.nbins[t] == tf.nbins[0] && tf.bounds[2 * t] == tf.bounds[0] &&
+                      tf.bounds[2 * t + 1] == tf.bounds[1];
+            ctf = ctf && T.chexp[0] == (1 | (1 << 6)) && T.chexp[1] == ((1 << 2) | (1 << 6)) &&
+                  T.chexp[2] == ((1 << 4) | (1 << 6)) && T.chexp[3] == (1 << 6);
+            if (ctf) {
+                const int nb = tf.nbins[0];
+                std::vector<float4> lay((size_t)2 * nb + (nb + 3) / 4);
+                float *d0 = reinterpret_cast<float *>(lay.data() + 2 * nb);
+                for (int k = 0; k < nb; k++) {
+                    const float4 a0 = tft[i][k], a1 = tft[i][nb + k], a2 = tft[i][2 * nb + k],
+                                 a3 = tft[i][3 * nb + k];
+                    lay[k] = make_float4(a0.x, a1.x, a2.x, a3.x);
+                    lay[nb + k] = make_float4(a0.y, a1.y, a2.y, a3.y);
+                    d0[k] = a0.z;
+                }
+                tft[i] = std::move(lay);
+                tot = (int)tft[i].size();
+                T.mode = 1;
+            }
+            q.mode = T.mode;
             T.total_bins = tot;
             q.lds = (size_t)tot * sizeof(float4);
             if (q.lds > 64 * 1024) return bad(YTRC_EINVAL, tag + "transfer-function tables exceed 64 KiB of LDS");
             q.nch_out = T.grey ? 4 : 3;
         }
-        q.kfn = pick_kernel(r.kind, q.nfp, rec);
+        q.kfn = pick_kernel(r.kind, q.nfp, rec, q.mode);
         if (q.kfn == nullptr) return bad(YTRC_ENOTIMPL, tag + "no kernel for this field count");
         if (r.image_in != nullptr) q.image_in.assign(r.image_in, r.image_in + q.npix * 4);
         Offs &o = offs[i];
@@ -961,10 +1099,10 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
     off = align_up(off + sizeof(float4) * std::max<size_t>(1, tf_all.size()), 256);
     // Persistent groups: one per kernel variant, items = (request, tile),
     // larger images first.
-    std::map<std::tuple<int, int, int>, std::vector<int>> gmap;
+    std::map<std::tuple<int, int, int, int>, std::vector<int>> gmap;
     for (int i = 0; i < n_req; i++) {
         const bool rec = (reqs[i].flags & YTRC_FLAG_RECORD) != 0;
-        gmap[std::make_tuple(reqs[i].kind, B->reqs[i].nfp, rec ? 1 : 0)].push_back(i);
+        gmap[std::make_tuple(reqs[i].kind, B->reqs[i].nfp, rec ? 1 : 0, B->reqs[i].mode)].push_back(i);
     }
     std::vector<std::vector<int2>> gitems;
     std::vector<size_t> gitems_off;
@@ -980,7 +1118,8 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
         g.kind = std::get<0>(kv.first);
         g.nfp = std::get<1>(kv.first);
         g.rec = std::get<2>(kv.first) != 0;
-        g.fn = pick_persist(g.kind, g.nfp, g.rec);
+        g.mode = std::get<3>(kv.first);
+        g.fn = pick_persist(g.kind, g.nfp, g.rec, g.mode);
         g.n_items = (int)items.size();
         g.counter = (int)B->groups.size();
         g.lds = (g.kind == YTRC_VOLUME_RENDER) ? sizeof(float4) * tf_all.size() : 0;

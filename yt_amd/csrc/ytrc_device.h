@@ -31,7 +31,7 @@ struct alignas(16) DTF {
     int32_t n_fits;
     int32_t grey;
     int32_t total_bins;
-    int32_t pad0;
+    int32_t mode;         // 1: ColorTransferFunction layout, 0: general
     int32_t nbins[6];
     int32_t off[6];
     int32_t field_id[6];
