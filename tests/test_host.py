@@ -1,0 +1,156 @@
+"""Host-side logic that feeds the kernels (no GPU): camera basis and sampler
+parameters, vertex centring, child masks, the kd-tree's brick set and
+viewpoint order, the uniform-grid split, and the C ABI's exported symbols."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from yt_amd import _lib, amr_kdtree, camera, data, render
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_default_north_matches_on_axis():
+    """off_axis_projection.py:402-407: for normal +z the image x axis is +x
+    and y is +y (the on-axis projection convention)."""
+    cam = camera.off_axis_camera((0.5, 0.5, 0.5), (0, 0, 1), 1.0, 32)
+    assert np.allclose(cam.unit_vectors[0], [1, 0, 0])
+    assert np.allclose(cam.unit_vectors[1], [0, 1, 0])
+    assert np.allclose(cam.unit_vectors[2], [0, 0, 1])
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_basis_and_sampler_params(seed):
+    rng = np.random.default_rng(seed)
+    n = rng.normal(size=3)
+    c = rng.uniform(0, 1, 3)
+    w = float(rng.uniform(0.5, 2))
+    p = render.camera_params(c, n, w, (31, 17), rng.normal(size=3))
+    uv = np.array([p["x_vec"], p["y_vec"], np.ravel(p["vp_dir"]) / w])
+    assert np.allclose(uv @ uv.T, np.eye(3), atol=1e-12)
+    vp = np.ravel(p["vp_pos"])
+    inv = vp[:9].reshape(3, 3, order="F")
+    assert np.allclose(inv, np.linalg.pinv(uv), atol=1e-12)
+    assert np.allclose(vp[9:], c - 0.5 * w * uv[2])  # back_center (lens.py:57)
+    assert np.allclose(p["center"], vp[9:])
+    assert p["bounds"] == (-w / 2, w / 2, -w / 2, w / 2)
+    # the viewpoint is 1e6 widths beyond the front plane (lens.py:114-121)
+    assert np.allclose(p["viewpoint"], c + 0.5 * w * uv[2] + uv[2] * 1e6 * w)
+
+
+def test_vertex_centring_exact_on_linear_fields():
+    """test_interpolators.py:107-157: no-ghost vertex centring reproduces a
+    linear field exactly at every vertex."""
+    n = (6, 5, 7)
+    i, j, k = np.meshgrid(*[np.arange(x) + 0.5 for x in n], indexing="ij")
+    f = 1.5 + 2.0 * i - 0.75 * j + 0.25 * k
+    v = data.ghost_zone_interpolate_no_ghost(f)
+    I, J, K = np.meshgrid(*[np.arange(x + 1.0) for x in n], indexing="ij")
+    assert v.shape == tuple(x + 1 for x in n)
+    assert np.array_equal(v, 1.5 + 2.0 * I - 0.75 * J + 0.25 * K)
+
+
+def test_uniform_split_matches_decompose():
+    rho = np.arange(64 ** 3, dtype=float).reshape((64,) * 3)
+    ds = data.load_uniform_grid({"density": rho}, nprocs=8)
+    assert len(ds.grids) == 8
+    assert all(tuple(g.ActiveDimensions) == (32, 32, 32) for g in ds.grids)
+    # split_array order: i-major (decompose.py:120-150)
+    assert np.allclose(ds.grids[1].LeftEdge, [0, 0, 0.5])
+    assert np.allclose(ds.grids[4].LeftEdge, [0.5, 0, 0])
+    assert np.array_equal(ds.grids[5]["density"], rho[32:, :32, 32:])
+
+
+def _small_amr():
+    from tests.helpers import small_amr
+
+    return small_amr(root_n=16, levels=2, seed=11)
+
+
+def test_child_mask_and_blocks():
+    ds = _small_amr()
+    total = 0.0
+    for g, m in ds.blocks():
+        total += m.sum() * np.prod(g.dds)
+    assert abs(total - 1.0) < 1e-12  # unmasked cells tile the domain once
+
+
+def test_kdtree_invariants():
+    ds = _small_amr()
+    kd = amr_kdtree.AMRKDTree(ds)
+    kd.set_fields(["density"], [True])
+    leaves = kd.leaves()
+    assert abs(kd.trunk.kd_sum_volume() - 1.0) < 1e-12
+    by_id = {g.id: g for g in ds.grids}
+    for node in leaves:
+        g = by_id[node.grid]
+        assert np.all(g.LeftEdge <= node.left_edge) and np.all(g.RightEdge >= node.right_edge)
+        # finest covering grid: no finer grid overlaps the leaf interior
+        for h in ds.grids:
+            if h.Level > g.Level:
+                lo = np.maximum(h.LeftEdge, node.left_edge)
+                hi = np.minimum(h.RightEdge, node.right_edge)
+                assert np.any(hi <= lo)
+        pg = kd.get_brick_data(node)
+        assert pg.my_data[0].shape == tuple(pg.dims + 1)
+        assert pg.source_mask.shape == tuple(pg.dims)
+
+
+def test_kdtree_viewpoint_order_is_front_to_back_along_rays():
+    """Farthest-from-viewpoint first: along a ray travelling toward the
+    viewpoint the leaves it crosses come in increasing t."""
+    ds = _small_amr()
+    kd = amr_kdtree.AMRKDTree(ds)
+    rng = np.random.default_rng(5)
+    for _ in range(10):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        vp = 0.5 + n * 1e6
+        order = list(kd.trunk.kd_traverse(viewpoint=vp))
+        pos = rng.uniform(0.2, 0.8, 3) - n
+        tin = []
+        for node in order:
+            t0, t1 = -np.inf, np.inf
+            for a in range(3):
+                ta = (node.left_edge[a] - pos[a]) / n[a]
+                tb = (node.right_edge[a] - pos[a]) / n[a]
+                t0, t1 = max(t0, min(ta, tb)), min(t1, max(ta, tb))
+            if t1 > t0:
+                tin.append(t0)
+        assert len(tin) > 1
+        assert all(a <= b + 1e-12 for a, b in zip(tin, tin[1:]))
+
+
+def test_tf_from_ctf_object():
+    from yt_amd.transfer_functions import gaussian_ctf
+
+    tf = gaussian_ctf((-1.0, 2.0), n_gauss=8)
+    t = _lib.TF.from_tf_obj(tf)
+    assert t.desc.n_fits == 4
+    assert list(t.desc.weight_table_ids)[:4] == [3, 3, 3, -1]
+    assert t.desc.nbins[0] == 256
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "ytrc.h")).read()
+    return sorted(set(re.findall(r"\b(ytrc_[a-z_]+)\s*\(", src)))
+
+
+def test_abi_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    decl = _declared()
+    assert len(decl) >= 15
+    for name in decl:
+        assert hasattr(lib, name), name
+    assert sorted(decl) == sorted(_lib.EXPORTED)
+
+
+def test_library_missing_fails_loudly(monkeypatch):
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", "/nonexistent/libytrc.so")
+    with pytest.raises(_lib.YtrcLibraryError):
+        _lib.lib()
