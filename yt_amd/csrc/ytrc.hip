@@ -144,7 +144,6 @@ struct TFCache {
     float tb0[6], tb1[6], tidbin[6];
 };
 
-__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ float unif(float v)
 {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
@@ -219,23 +218,24 @@ __device__ __forceinline__ void fit_eval_general(const TFCache &tf, const float4
 
 // The ColorTransferFunction case of the same evaluation: one bin for the
 // four tables, e_c = table_c(v) * table_alpha(v) for c < 3, e_3 = alpha.
+// An out-of-range or non-normal v reads bin 0 and gets alpha 0, so every
+// channel is 0 without a select per channel.
 __device__ __forceinline__ void fit_eval_ctf(const TFCache &tf, const float4 *__restrict__ lt,
                                              int nb, float v, float e[4])
 {
     const bool ok = (v < tf.b1) && (v > tf.b0) && __builtin_isnormal(v);
-    int bin = (int)((v - tf.b0) * tf.idbin);
-    bin = bin < 0 ? 0 : (bin > tf.nbm2 ? tf.nbm2 : bin);
+    const float vs = ok ? v : tf.b0;
+    int bin = (int)((vs - tf.b0) * tf.idbin);
+    bin = bin > tf.nbm2 ? tf.nbm2 : bin;
     const float4 y = lt[bin];
     const float4 dy = lt[nb + bin];
     const float d0 = reinterpret_cast<const float *>(lt + 2 * nb)[bin];
-    const float dd = v - d0;
-    const float a = ok ? __builtin_fmaf(dd, dy.w, y.w) : 0.0f;
-    const float r = ok ? __builtin_fmaf(dd, dy.x, y.x) : 0.0f;
-    const float g = ok ? __builtin_fmaf(dd, dy.y, y.y) : 0.0f;
-    const float b = ok ? __builtin_fmaf(dd, dy.z, y.z) : 0.0f;
-    e[0] = r * a;
-    e[1] = g * a;
-    e[2] = b * a;
+    const float dd = vs - d0;
+    float a = __builtin_fmaf(dd, dy.w, y.w);
+    a = ok ? a : 0.0f;
+    e[0] = __builtin_fmaf(dd, dy.x, y.x) * a;
+    e[1] = __builtin_fmaf(dd, dy.y, y.y) * a;
+    e[2] = __builtin_fmaf(dd, dy.z, y.z) * a;
     e[3] = a;
 }
 
@@ -262,8 +262,43 @@ __device__ __forceinline__ BrickReg brick_reg(const DBrick &b)
     return r;
 }
 
-// VolumeRenderSampler.sample (image_samplers.pyx:453-487).  MODE is the
-// transfer-function path (1 = ColorTransferFunction, 0 = general).
+// offset_interpolate (fixed_interpolator.cpp:34-52) restricted to the
+// sub-sample positions u(s) = dp + s*ds of one visit is a cubic in s.  Its
+// coefficients come from the 8 corners c[4i+2j+k] once per visit; each
+// sub-sample then costs 3 FMAs.
+__device__ __forceinline__ void trilinear_cubic(const float (&c)[8], const float dp[3],
+                                                const float ds[3], float k[4])
+{
+    const float A = c[0];
+    const float Bz = c[1] - c[0];
+    const float By = c[2] - c[0];
+    const float Bx = c[4] - c[0];
+    const float Cyz = (c[3] - c[2]) - Bz;
+    const float Cxz = (c[5] - c[4]) - Bz;
+    const float Cxy = (c[6] - c[4]) - By;
+    const float D = ((c[7] - c[6]) - (c[5] - c[4])) - Cyz;
+    const float x0 = dp[0], y0 = dp[1], z0 = dp[2];
+    const float dx = ds[0], dy = ds[1], dz = ds[2];
+    const float fxy = __builtin_fmaf(D, z0, Cxy);
+    const float fxz = __builtin_fmaf(D, y0, Cxz);
+    const float fyz = __builtin_fmaf(D, x0, Cyz);
+    const float tyz = __builtin_fmaf(Cyz, z0, By);
+    const float ex = __builtin_fmaf(y0, fxy, __builtin_fmaf(Cxz, z0, Bx));
+    const float ey = __builtin_fmaf(x0, fxy, tyz);
+    const float ez = __builtin_fmaf(x0, fxz, __builtin_fmaf(Cyz, y0, Bz));
+    k[0] = __builtin_fmaf(x0, ex, __builtin_fmaf(y0, tyz, __builtin_fmaf(z0, Bz, A)));
+    k[1] = __builtin_fmaf(ex, dx, __builtin_fmaf(ey, dy, ez * dz));
+    const float dydz = dy * dz;
+    k[2] = __builtin_fmaf(fxy, dx * dy, __builtin_fmaf(fxz, dx * dz, fyz * dydz));
+    k[3] = D * dx * dydz;
+}
+
+// VolumeRenderSampler.sample (image_samplers.pyx:453-487).
+// MODE bit 0: ColorTransferFunction path; bit 1: n_samples == 10 unrolled.
+// st[] is the compositing state: rgba, or, when `tform` is set (non-grey
+// and the host has shown dt * e <= 1 for every sub-sample), the per-channel
+// transmittance T = 1 - rgba, for which rgba' = a + max(1 - a, 0) rgba
+// (field_interpolation_tables.pxd:107-110) is exactly T' = T (1 - a).
 template <int NFP, bool REC, int MODE>
 struct VRSampler {
     const float *data;
@@ -274,10 +309,74 @@ struct VRSampler {
     const TFCache *tf;
     const float4 *lt;
     int nb;
+    int tform;
     double inv_ns;
     int n_samples;
-    double rgba[4];
+    double st[4];
     Rec *rec;
+
+    __device__ __forceinline__ void eval(float v0, const float (&dvs)[NFP], float e[4]) const
+    {
+        if (MODE & 1)
+            fit_eval_ctf(*tf, lt, nb, v0, e);
+        else
+            fit_eval_general<NFP>(*tf, lt, dvs, e);
+    }
+
+    template <int NS, class Body>
+    __device__ __forceinline__ static void each_sample(int ns, Body &&body)
+    {
+        if constexpr (NS > 0) {
+#pragma unroll
+            for (int s = 0; s < NS; s++) body(s);
+        } else {
+            for (int s = 0; s < ns; s++) body(s);
+        }
+    }
+
+    template <int NS>
+    __device__ __forceinline__ void samples(const float (&k)[NFP][4], double dt, int ns)
+    {
+        const TFCache &T = *tf;
+        auto value = [&](int s, float (&dvs)[NFP]) {
+            const float sf = (float)s;
+#pragma unroll
+            for (int f = 0; f < NFP; f++)
+                dvs[f] = __builtin_fmaf(
+                    __builtin_fmaf(__builtin_fmaf(k[f][3], sf, k[f][2]), sf, k[f][1]), sf, k[f][0]);
+        };
+        if (tform) {
+            each_sample<NS>(ns, [&](int s) {
+                float dvs[NFP], e[4];
+                value(s, dvs);
+                eval((NFP == 1) ? dvs[0] : pick(dvs, T.fid), dvs, e);
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++) st[ch] *= __builtin_fma(-dt, (double)e[ch], 1.0);
+            });
+        } else if (T.grey) {
+            each_sample<NS>(ns, [&](int s) {
+                float dvs[NFP], e[4];
+                value(s, dvs);
+                eval((NFP == 1) ? dvs[0] : pick(dvs, T.fid), dvs, e);
+                const double ta = y_fmax(1.0 - dt * (double)e[3], 0.0);
+#pragma unroll
+                for (int ch = 0; ch < 4; ch++) st[ch] = __builtin_fma(ta, st[ch], dt * (double)e[ch]);
+            });
+        } else {
+            each_sample<NS>(ns, [&](int s) {
+                float dvs[NFP], e[4];
+                value(s, dvs);
+                eval((NFP == 1) ? dvs[0] : pick(dvs, T.fid), dvs, e);
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++) {
+                    const double a = dt * (double)e[ch];
+                    const double ta = y_fmax(1.0 - a, 0.0);
+                    st[ch] = __builtin_fma(ta, st[ch], a);
+                }
+            });
+        }
+    }
+
     __device__ __forceinline__ void operator()(double en, double ex, int c0, int c1, int c2)
     {
         if (REC) rec->visit(en, ex, c0, c1, c2);
@@ -291,7 +390,8 @@ struct VRSampler {
         const int s1 = d2 + 1;
         const int s0 = (d1 + 1) * (d2 + 1);
         const int voff = c0 * s0 + c1 * s1 + c2;
-        // dt = (exit_t - enter_t) / n_samples; positions from FP64 enter_t.
+        // dt = (exit_t - enter_t) / n_samples; positions from FP64 enter_t
+        // in the order of image_samplers.pyx:475-479.
         const double dt = (ex - en) * inv_ns;
         const double tt = en + 0.5 * dt;
         float dp[3], ds[3];
@@ -316,43 +416,17 @@ struct VRSampler {
             FieldLoad<NFP>::get(p0 + (size_t)(s0 + s1) * NFP, cv[6]);
             FieldLoad<NFP>::get(p0 + (size_t)(s0 + s1 + 1) * NFP, cv[7]);
         }
-        const TFCache &T = *tf;
-        for (int s = 0; s < n_samples; s++) {
-            float dvs[NFP];
-            const float dv2 = 1.0f - dp[2];
-            const float dv1 = 1.0f - dp[1];
-            const float dv0 = 1.0f - dp[0];
+        float k[NFP][4];
 #pragma unroll
-            for (int f = 0; f < NFP; f++) {
-                float z0 = __builtin_fmaf(dp[2], cv[1][f], dv2 * cv[0][f]);
-                float z1 = __builtin_fmaf(dp[2], cv[3][f], dv2 * cv[2][f]);
-                float z2 = __builtin_fmaf(dp[2], cv[5][f], dv2 * cv[4][f]);
-                float z3 = __builtin_fmaf(dp[2], cv[7][f], dv2 * cv[6][f]);
-                float y0 = __builtin_fmaf(dp[1], z1, dv1 * z0);
-                float y1 = __builtin_fmaf(dp[1], z3, dv1 * z2);
-                dvs[f] = __builtin_fmaf(dp[0], y1, dv0 * y0);
-            }
-            float e[4];
-            if (MODE == 1)
-                fit_eval_ctf(T, lt, nb, (NFP == 1) ? dvs[0] : pick(dvs, T.fid), e);
-            else
-                fit_eval_general<NFP>(T, lt, dvs, e);
-            if (T.grey) {
-                const double ta = y_fmax(1.0 - dt * (double)e[3], 0.0);
-#pragma unroll
-                for (int ch = 0; ch < 4; ch++) rgba[ch] = __builtin_fma(ta, rgba[ch], dt * (double)e[ch]);
-            } else {
-#pragma unroll
-                for (int ch = 0; ch < 3; ch++) {
-                    const double a = dt * (double)e[ch];
-                    const double ta = y_fmax(1.0 - a, 0.0);
-                    rgba[ch] = __builtin_fma(ta, rgba[ch], a);
-                }
-            }
-            dp[0] = dp[0] + ds[0];
-            dp[1] = dp[1] + ds[1];
-            dp[2] = dp[2] + ds[2];
+        for (int f = 0; f < NFP; f++) {
+            const float cf[8] = {cv[0][f], cv[1][f], cv[2][f], cv[3][f],
+                                 cv[4][f], cv[5][f], cv[6][f], cv[7][f]};
+            trilinear_cubic(cf, dp, ds, k[f]);
         }
+        if (MODE & 2)
+            samples<10>(k, dt, 10);
+        else
+            samples<0>(k, dt, n_samples);
     }
 };
 
@@ -374,7 +448,7 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
     double pos[3], dir[3], iv[3];
     ray_setup(im, vi, vj, pos, dir);
 #pragma unroll
-    for (int i = 0; i < 3; i++) iv[i] = (dir[i] != 0.0) ? 1.0 / dir[i] : 0.0;
+    for (int i = 0; i < 3; i++) iv[i] = uni_d((dir[i] != 0.0) ? 1.0 / dir[i] : 0.0);
     double max_t = 1.0;
     if (im.zbuf != nullptr && active) max_t = y_fmin(y_fmax(im.zbuf[pix], 0.0), 1.0);
 
@@ -385,18 +459,22 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
     rec.brick = 0;
 
     TFCache tfc;
-    if (KIND == 1) load_tf(*im.tf, tfc);
-    const int nb = (KIND == 1) ? im.tf->nbins[0] : 0;
+    int nb = 0, tform = 0;
+    if (KIND == 1) {
+        load_tf(*im.tf, tfc);
+        nb = uni(im.tf->nbins[0]);
+        tform = uni(im.tform);
+    }
 
     float ps[NFP], pc[NFP];
-    double rgba[4];
+    double st[4];
 #pragma unroll
     for (int f = 0; f < NFP; f++) {
         ps[f] = 0.0f;
         pc[f] = 0.0f;
     }
 #pragma unroll
-    for (int c = 0; c < 4; c++) rgba[c] = 0.0;
+    for (int c = 0; c < 4; c++) st[c] = 0.0;
     if (im.init != nullptr && active) {
         if (KIND == 0) {
 #pragma unroll
@@ -404,8 +482,12 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
                 if (f < 4) ps[f] = im.init[f * npix + pix];
         } else {
 #pragma unroll
-            for (int c = 0; c < 4; c++) rgba[c] = (double)im.init[c * npix + pix];
+            for (int c = 0; c < 4; c++) st[c] = (double)im.init[c * npix + pix];
         }
+    }
+    if (KIND == 1 && tform) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) st[c] = 1.0 - st[c];
     }
 
     for (int k = 0; k < im.n_order; k++) {
@@ -448,14 +530,15 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
             s.tf = &tfc;
             s.lt = lt;
             s.nb = nb;
+            s.tform = tform;
             s.inv_ns = im.inv_ns;
             s.n_samples = im.n_samples;
 #pragma unroll
-            for (int c = 0; c < 4; c++) s.rgba[c] = rgba[c];
+            for (int c = 0; c < 4; c++) s.st[c] = st[c];
             s.rec = &rec;
             walk_loop(b, w, max_t, s);
 #pragma unroll
-            for (int c = 0; c < 4; c++) rgba[c] = s.rgba[c];
+            for (int c = 0; c < 4; c++) st[c] = s.st[c];
         }
     }
     if (!active) return;
@@ -464,9 +547,13 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
         for (int f = 0; f < NFP; f++)
             if (f < im.nch_out) im.out[f * npix + pix] = ps[f] - pc[f];
     } else {
+        if (tform) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) st[c] = 1.0 - st[c];
+        }
 #pragma unroll
         for (int c = 0; c < 4; c++)
-            if (c < im.nch_out) im.out[c * npix + pix] = (float)rgba[c];
+            if (c < im.nch_out) im.out[c * npix + pix] = (float)st[c];
     }
     if (REC) {
         im.rec_visits[pix] = rec.visits;
@@ -537,16 +624,9 @@ pfn_t pick_persist_nfp(int nfp)
     switch (nfp) {
     case 1: return k_persist<KIND, 1, REC, MODE>;
     case 2: return k_persist<KIND, 2, REC, MODE>;
-    case 4: return k_persist<KIND, 4, REC, MODE>;
-    default: return KIND == 1 ? k_persist<KIND, 8, REC, MODE> : nullptr;
+    case 4: return k_persist<KIND, 4, REC, (MODE & 1)>;
+    default: return KIND == 1 ? k_persist<KIND, 8, REC, (MODE & 1)> : nullptr;
     }
-}
-
-pfn_t pick_persist(int kind, int nfp, bool rec, int mode)
-{
-    if (kind == 0) return rec ? pick_persist_nfp<0, true, 0>(nfp) : pick_persist_nfp<0, false, 0>(nfp);
-    if (mode == 1) return rec ? pick_persist_nfp<1, true, 1>(nfp) : pick_persist_nfp<1, false, 1>(nfp);
-    return rec ? pick_persist_nfp<1, true, 0>(nfp) : pick_persist_nfp<1, false, 0>(nfp);
 }
 
 template <int KIND, bool REC, int MODE>
@@ -555,16 +635,32 @@ kfn_t pick_kernel_nfp(int nfp)
     switch (nfp) {
     case 1: return k_render<KIND, 1, REC, MODE>;
     case 2: return k_render<KIND, 2, REC, MODE>;
-    case 4: return k_render<KIND, 4, REC, MODE>;
-    default: return KIND == 1 ? k_render<KIND, 8, REC, MODE> : nullptr;
+    case 4: return k_render<KIND, 4, REC, (MODE & 1)>;
+    default: return KIND == 1 ? k_render<KIND, 8, REC, (MODE & 1)> : nullptr;
+    }
+}
+
+// mode: bit 0 ColorTransferFunction path, bit 1 n_samples == 10 (VR only).
+pfn_t pick_persist(int kind, int nfp, bool rec, int mode)
+{
+    if (kind == 0) return rec ? pick_persist_nfp<0, true, 0>(nfp) : pick_persist_nfp<0, false, 0>(nfp);
+    switch (mode & 3) {
+    case 0: return rec ? pick_persist_nfp<1, true, 0>(nfp) : pick_persist_nfp<1, false, 0>(nfp);
+    case 1: return rec ? pick_persist_nfp<1, true, 1>(nfp) : pick_persist_nfp<1, false, 1>(nfp);
+    case 2: return rec ? pick_persist_nfp<1, true, 2>(nfp) : pick_persist_nfp<1, false, 2>(nfp);
+    default: return rec ? pick_persist_nfp<1, true, 3>(nfp) : pick_persist_nfp<1, false, 3>(nfp);
     }
 }
 
 kfn_t pick_kernel(int kind, int nfp, bool rec, int mode)
 {
     if (kind == 0) return rec ? pick_kernel_nfp<0, true, 0>(nfp) : pick_kernel_nfp<0, false, 0>(nfp);
-    if (mode == 1) return rec ? pick_kernel_nfp<1, true, 1>(nfp) : pick_kernel_nfp<1, false, 1>(nfp);
-    return rec ? pick_kernel_nfp<1, true, 0>(nfp) : pick_kernel_nfp<1, false, 0>(nfp);
+    switch (mode & 3) {
+    case 0: return rec ? pick_kernel_nfp<1, true, 0>(nfp) : pick_kernel_nfp<1, false, 0>(nfp);
+    case 1: return rec ? pick_kernel_nfp<1, true, 1>(nfp) : pick_kernel_nfp<1, false, 1>(nfp);
+    case 2: return rec ? pick_kernel_nfp<1, true, 2>(nfp) : pick_kernel_nfp<1, false, 2>(nfp);
+    default: return rec ? pick_kernel_nfp<1, true, 3>(nfp) : pick_kernel_nfp<1, false, 3>(nfp);
+    }
 }
 
 }  // namespace
@@ -627,6 +723,7 @@ struct ReqHost {
     int nch_out = 0;
     int nfp = 1;
     int mode = 0;
+    int tform = 0;
 };
 
 }  // namespace
@@ -1042,7 +1139,35 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
                 tot = (int)tft[i].size();
                 T.mode = 1;
             }
-            q.mode = T.mode;
+            q.mode = T.mode | (r.n_samples == 10 ? 2 : 0);
+            // Transmittance-form compositing is exact when every sub-sample
+            // has a = dt * e <= 1: bound e by the tables' largest magnitudes
+            // and dt by the longest in-cell t-interval / n_samples.
+            if (!T.grey) {
+                bool bounded = true;
+                double emax = 0.0;
+                for (int t = 0; t < tf.n_fits; t++) bounded = bounded && tf.weight_field_ids[t] == -1;
+                for (int c = 0; bounded && c < 3; c++) {
+                    if (T.chexp[c] < 0) continue;
+                    double ec = 1.0;
+                    for (int u = 0; u < tf.n_fits; u++) {
+                        const int pwr = (T.chexp[c] >> (2 * u)) & 3;
+                        double m = 0.0;
+                        for (int kk = 0; kk < tf.nbins[u]; kk++) m = std::max(m, std::fabs(tf.y[u][kk]));
+                        for (int q2 = 0; q2 < pwr; q2++) ec *= m;
+                    }
+                    emax = std::max(emax, ec);
+                }
+                double tcell = 0.0;
+                for (int k2 = 0; bounded && k2 < r.n_order; k2++) {
+                    const DBrick &hb = ds.h_bricks[r.order[k2]];
+                    double cmin = 1.0;
+                    for (int a = 0; a < 3; a++)
+                        if (r.vp_dir[a] != 0.0) cmin = std::min(cmin, hb.dds[a] / std::fabs(r.vp_dir[a]));
+                    tcell = std::max(tcell, cmin);
+                }
+                q.tform = bounded && (emax * (tcell / (double)r.n_samples) * 1.001 <= 1.0);
+            }
             T.total_bins = tot;
             q.lds = (size_t)tot * sizeof(float4);
             if (q.lds > 64 * 1024) return bad(YTRC_EINVAL, tag + "transfer-function tables exceed 64 KiB of LDS");
@@ -1179,6 +1304,7 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
         d.flags = (int32_t)r.flags;
         d.nch_out = q.nch_out;
         d.inv_ns = r.n_samples > 0 ? 1.0 / (double)r.n_samples : 0.0;
+        d.tform = q.tform;
         if (r.n_order > 0) {
             HIPCHK(hipMemcpy(A + o.order, r.order, sizeof(int32_t) * r.n_order, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(A + o.ext, exts[i].data(), sizeof(int4) * r.n_order, hipMemcpyHostToDevice));

@@ -68,12 +68,24 @@ struct alignas(16) DImg {
     int32_t nfp, tiles0;
     int32_t tiles1, flags;
     int32_t nch_out, tf_slot;
+    int32_t tform, pad1;    // 1: VR compositing state kept as transmittance
     double inv_ns;
 };
 
 constexpr int TILE = 8;  // 8x8 rays per 64-lane wavefront
 
 __device__ __forceinline__ double y_fmin(double a, double b) { return (a < b) ? a : b; }
+
+// Marks a value the compiler cannot prove uniform as wave-uniform (scalar
+// register); only for values that are equal in every lane.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uni_d(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 __device__ __forceinline__ double y_fmax(double a, double b) { return (a > b) ? a : b; }
 
 // generate_vector_info_plane_parallel: lenses.pyx:179-192.
@@ -112,9 +124,10 @@ __device__ __forceinline__ bool walk_init(const DBrick &b, const double pos[3],
         intersect_t = 0.0;
         direction = 3;
     }
+    // Plane-parallel rays share v_dir, so step and tdelta are wave-uniform.
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        w.step[i] = (dir[i] < 0) ? -1 : ((dir[i] == 0.0) ? 0 : 1);
+        w.step[i] = uni((dir[i] < 0) ? -1 : ((dir[i] == 0.0) ? 0 : 1));
     }
     if (direction != 3) {
 #pragma unroll
@@ -143,7 +156,7 @@ __device__ __forceinline__ bool walk_init(const DBrick &b, const double pos[3],
     if (!((0.0 <= intersect_t) && (intersect_t < max_t))) return false;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        w.tdelta[i] = (double)w.step[i] * iv[i] * b.dds[i];
+        w.tdelta[i] = uni_d((double)w.step[i] * iv[i] * b.dds[i]);
         double ty = 0.0;
         if (i == direction && w.step[i] > 0) {
             w.cur[i] = 0;
