@@ -132,6 +132,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--lat-reps", type=int, default=20)
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="only run the HBM copy probe on this many GiB and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -140,8 +142,17 @@ def main():
 
     from yt_amd import _lib, render
 
+    if args.copy_probe > 0:
+        ctx = _lib.Context(0)
+        g, ms = ctx.hbm_probe(int(args.copy_probe * 2**30), 20)
+        print(json.dumps({"copy_bytes": int(args.copy_probe * 2**30), "gbps": g, "ms": ms}))
+        ctx.close()
+        return
+
     rho, ds, tf, cams = build_workload(args.n, args.psize, args.cams)
     ctx = _lib.Context(0)
+    copy_gbps, _ = ctx.hbm_probe(1 << 30, 10)
+    log(f"HBM copy probe: {copy_gbps:.0f} GB/s")
     t = time.time()
     vol = render.ResidentVolume(ctx, ds, proj_fields="density", vr_field="density")
     del rho
@@ -215,7 +226,7 @@ def main():
 
     # Latency of single 1024^2 requests (launch -> sync on the host clock).
     lat = {}
-    for kind, name in ((1, "vr_1024"), (0, "proj_1024")):
+    for kind, name in (((1, "vr_1024"), (0, "proj_1024")) if args.lat_reps > 0 else ()):
         p = render.camera_params((0.5, 0.5, 0.5), (0.3, 0.5, 0.81), 1.0, 1024)
         s = vol.projection_request(p) if kind == 0 else vol.vr_request(p, tf)
         b = _lib.Batch(ctx, [s], _lib.BATCH_PERSISTENT)
@@ -274,6 +285,7 @@ def main():
             "frac": round(ach / HBM_PEAK_GBS, 4),
             "traffic": None,
             "kernel_ms": round(kinds[dom]["ms"], 4),
+            "copy_probe_gbps": round(copy_gbps, 1),
         },
         "cpu_baseline": cpu,
     }

@@ -152,6 +152,11 @@ int ytrc_batch_request_ms(ytrc_batch *b, int32_t i, float *ms);
 int ytrc_batch_bytes(const ytrc_batch *b, uint64_t *bytes);
 void ytrc_batch_destroy(ytrc_batch *b);
 
+/* Achievable HBM bandwidth: a streaming copy of `bytes` (read + write
+ * counted) timed over `iters` launches with HIP events.  No counterpart in
+ * the reference; used to confirm the bandwidth roofline on the device. */
+int ytrc_hbm_probe(ytrc_ctx *ctx, uint64_t bytes, int32_t iters, float *gbps, float *ms_per_copy);
+
 /* Average duration (ms) of the dominant kernel over the launches since
  * batch creation, from HIP events on the launch stream. */
 int ytrc_batch_kernel_ms(const ytrc_batch *b, float *avg_ms, int32_t *n_launches);

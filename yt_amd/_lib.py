@@ -50,6 +50,7 @@ EXPORTED = [
     "ytrc_batch_destroy",
     "ytrc_batch_kernel_ms",
     "ytrc_batch_request_ms",
+    "ytrc_hbm_probe",
 ]
 
 
@@ -142,6 +143,8 @@ def _declare(lib):
     lib.ytrc_batch_destroy.restype = None
     lib.ytrc_batch_kernel_ms.argtypes = [P, C.POINTER(C.c_float), C.POINTER(C.c_int32)]
     lib.ytrc_batch_request_ms.argtypes = [P, C.c_int32, C.POINTER(C.c_float)]
+    lib.ytrc_hbm_probe.argtypes = [P, C.c_uint64, C.c_int32, C.POINTER(C.c_float),
+                                   C.POINTER(C.c_float)]
     for name in EXPORTED:
         f = getattr(lib, name)
         if name not in ("ytrc_ctx_destroy", "ytrc_batch_destroy", "ytrc_last_error"):
@@ -205,6 +208,12 @@ class Context:
         u, b = C.c_uint64(), C.c_uint64()
         _check(lib().ytrc_ctx_used_bytes(self._h, C.byref(u), C.byref(b)))
         return u.value, b.value
+
+    def hbm_probe(self, nbytes=1 << 30, iters=20):
+        """(GB/s, ms per copy) of a streaming device copy of nbytes."""
+        g, ms = C.c_float(), C.c_float()
+        _check(lib().ytrc_hbm_probe(self._h, int(nbytes), int(iters), C.byref(g), C.byref(ms)))
+        return g.value, ms.value
 
     def upload(self, bricks, fields, masks=None):
         """bricks: list of (left_edge, right_edge, dims, vertex_centred);

@@ -96,23 +96,87 @@ struct ProjSampler {
     float s[NFP];
     float c[NFP];
     Rec *rec;
-    __device__ __forceinline__ void operator()(double en, double ex, int c0, int c1, int c2)
-    {
-        if (REC) {
-            rec->visit(en, ex, c0, c1, c2);
-            if (ex > en) rec->samples += 1;
-        }
-        if (!((unsigned)c0 < (unsigned)dims0 && (unsigned)c1 < (unsigned)d1 &&
-              (unsigned)c2 < (unsigned)d2))
-            return;
-        const float dl = (float)(ex - en);
-        const int di = (c0 * d1 + c1) * d2 + c2;
-        float v[NFP];
-        FieldLoad<NFP>::get(data + (size_t)di * NFP, v);
-#pragma unroll
-        for (int f = 0; f < NFP; f++) kahan_add(s[f], c[f], v[f] * dl);
-    }
 };
+
+// walk_volume's loop (grid_traversal.pyx:120-138) fused with
+// ProjectionSampler.sample and software-pipelined: the DDA never depends
+// on the data, so each round takes up to PF steps, issues their PF loads
+// together and then accumulates them in visit order (the same additions, in
+// the same order, as one visit at a time).
+template <int NFP, bool REC>
+__device__ __forceinline__ void walk_projection(const DBrick &b, Walk &w, double max_t,
+                                                ProjSampler<NFP, REC> &S)
+{
+    constexpr int PF = 4;
+    double t0 = w.tmax[0], t1 = w.tmax[1], t2 = w.tmax[2];
+    int c0 = w.cur[0], c1 = w.cur[1], c2 = w.cur[2];
+    double enter_t = w.enter_t;
+    const int st0 = w.step[0], st1 = w.step[1], st2 = w.step[2];
+    const double td0 = w.tdelta[0], td1 = w.tdelta[1], td2 = w.tdelta[2];
+    bool live = true;
+    while (live) {
+        int off[PF];
+        float dl[PF];
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            off[u] = -1;
+            dl[u] = 0.0f;
+            if (live) {
+                const bool a01 = t0 < t1;
+                const bool a02 = t0 < t2;
+                const bool a12 = t1 < t2;
+                const int ax = a01 ? (a02 ? 0 : 2) : (a12 ? 1 : 2);
+                const double tm = (ax == 0) ? t0 : ((ax == 1) ? t1 : t2);
+                const double exit_t = y_fmin(tm, max_t);
+                if (REC) {
+                    S.rec->visit(enter_t, exit_t, c0, c1, c2);
+                    if (exit_t > enter_t) S.rec->samples += 1;
+                }
+                if ((unsigned)c0 < (unsigned)S.dims0 && (unsigned)c1 < (unsigned)S.d1 &&
+                    (unsigned)c2 < (unsigned)S.d2) {
+                    off[u] = (c0 * S.d1 + c1) * S.d2 + c2;
+                    dl[u] = (float)(exit_t - enter_t);
+                }
+                int cn, dn;
+                if (ax == 0) {
+                    c0 += st0;
+                    t0 += td0;
+                    cn = c0;
+                    dn = b.dims[0];
+                } else if (ax == 1) {
+                    c1 += st1;
+                    t1 += td1;
+                    cn = c1;
+                    dn = b.dims[1];
+                } else {
+                    c2 += st2;
+                    t2 += td2;
+                    cn = c2;
+                    dn = b.dims[2];
+                }
+                enter_t = tm;
+                live = !(cn < 0 || cn >= dn || enter_t >= max_t);
+            }
+        }
+        float v[PF][NFP];
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            if (off[u] >= 0) {
+                FieldLoad<NFP>::get(S.data + (size_t)off[u] * NFP, v[u]);
+            } else {
+#pragma unroll
+                for (int f = 0; f < NFP; f++) v[u][f] = 0.0f;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            if (off[u] >= 0) {
+#pragma unroll
+                for (int f = 0; f < NFP; f++) kahan_add(S.s[f], S.c[f], v[u][f] * dl[u]);
+            }
+        }
+    }
+}
 
 template <int N>
 __device__ __forceinline__ float pick(const float (&a)[N], int i)
@@ -385,11 +449,12 @@ struct VRSampler {
               (unsigned)c2 < (unsigned)d2))
             return;
         const int cell = (c0 * d1 + c1) * d2 + c2;
-        if (mask != nullptr && mask[cell] != 1) return;
-        if (REC) rec->samples += n_samples;
         const int s1 = d2 + 1;
         const int s0 = (d1 + 1) * (d2 + 1);
         const int voff = c0 * s0 + c1 * s1 + c2;
+        // The mask byte and the corner values are independent loads: issue
+        // both before testing the mask.
+        const int mval = (mask != nullptr) ? (int)mask[cell] : 1;
         // dt = (exit_t - enter_t) / n_samples; positions from FP64 enter_t
         // in the order of image_samplers.pyx:475-479.
         const double dt = (ex - en) * inv_ns;
@@ -416,6 +481,8 @@ struct VRSampler {
             FieldLoad<NFP>::get(p0 + (size_t)(s0 + s1) * NFP, cv[6]);
             FieldLoad<NFP>::get(p0 + (size_t)(s0 + s1 + 1) * NFP, cv[7]);
         }
+        if (mval != 1) return;
+        if (REC) rec->samples += n_samples;
         float k[NFP][4];
 #pragma unroll
         for (int f = 0; f < NFP; f++) {
@@ -511,7 +578,7 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
                 s.c[f] = pc[f];
             }
             s.rec = &rec;
-            walk_loop(b, w, max_t, s);
+            walk_projection(b, w, max_t, s);
 #pragma unroll
             for (int f = 0; f < NFP; f++) {
                 ps[f] = s.s[f];
@@ -607,6 +674,15 @@ __global__ __launch_bounds__(256) void k_persist(const DImg *__restrict__ imgs,
         const DImg &im = imgs[img];
         render_tile<KIND, NFP, REC, MODE>(im, tile, lane, s_tf + im.tf_slot);
     }
+}
+
+// Streaming copy used to measure the achievable HBM bandwidth and to
+// calibrate the FETCH_SIZE / WRITE_SIZE counters against a known byte count.
+__global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src,
+                                              float4 *__restrict__ dst, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
 __global__ void k_zero_counters(unsigned int *c, int n)
@@ -906,7 +982,15 @@ int ytrc_dataset_upload(ytrc_ctx *ctx, int32_t n_bricks, const ytrc_brick_desc *
                                          " exceeds 2^31 points (int32 cell index, image_samplers.pyx:341)");
         h.data_off = pool_floats;
         pool_floats += align_up((size_t)np * ds.nfp, 64);
-        const bool has_mask = mask_ptrs != nullptr && mask_ptrs[b] != nullptr;
+        bool has_mask = mask_ptrs != nullptr && mask_ptrs[b] != nullptr;
+        if (has_mask) {
+            // An all-ones mask is the same as none: skip its per-visit load.
+            const size_t nc = (size_t)d.dims[0] * d.dims[1] * d.dims[2];
+            const uint8_t *m = mask_ptrs[b];
+            bool all1 = true;
+            for (size_t q = 0; q < nc && all1; q++) all1 = (m[q] == 1);
+            has_mask = !all1;
+        }
         h.mask_off = has_mask ? mask_bytes : -1;
         if (has_mask)
             mask_bytes += align_up((size_t)d.dims[0] * d.dims[1] * d.dims[2], 256);
@@ -1508,6 +1592,40 @@ int ytrc_batch_read_record(ytrc_batch *b, int32_t i, int64_t *visits, uint64_t *
     if (visits) HIPCHK(hipMemcpy(visits, d.rec_visits, 8 * n, hipMemcpyDeviceToHost));
     if (hashes) HIPCHK(hipMemcpy(hashes, d.rec_hash, 8 * n, hipMemcpyDeviceToHost));
     if (samples) HIPCHK(hipMemcpy(samples, d.rec_samples, 8 * n, hipMemcpyDeviceToHost));
+    return YTRC_OK;
+}
+
+int ytrc_hbm_probe(ytrc_ctx *ctx, uint64_t bytes, int32_t iters, float *gbps, float *ms_per_copy)
+{
+    if (ctx == nullptr || bytes < 16 || iters <= 0) return fail(YTRC_EINVAL, "ytrc_hbm_probe: bad argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)(bytes / 16);
+    float4 *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, n * 16));
+    hipError_t e = hipMalloc(&b, n * 16);
+    if (e != hipSuccess) {
+        hipFree(a);
+        return fail(YTRC_EDEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const int grid = ctx->n_cu * 8;
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, ctx->stream, (const float4 *)a, b, n);
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    for (int i = 0; i < iters; i++)
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, ctx->stream, (const float4 *)a, b, n);
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    const double per = ms / iters;
+    if (ms_per_copy) *ms_per_copy = (float)per;
+    if (gbps) *gbps = (float)(2.0 * (double)n * 16.0 / (per * 1e-3) / 1e9);
     return YTRC_OK;
 }
 
