@@ -107,7 +107,7 @@ template <int NFP, bool REC>
 __device__ __forceinline__ void walk_projection(const DBrick &b, Walk &w, double max_t,
                                                 ProjSampler<NFP, REC> &S)
 {
-    constexpr int PF = 4;
+    constexpr int PF = 8;
     double t0 = w.tmax[0], t1 = w.tmax[1], t2 = w.tmax[2];
     int c0 = w.cur[0], c1 = w.cur[1], c2 = w.cur[2];
     double enter_t = w.enter_t;
@@ -134,7 +134,9 @@ __device__ __forceinline__ void walk_projection(const DBrick &b, Walk &w, double
                 }
                 if ((unsigned)c0 < (unsigned)S.dims0 && (unsigned)c1 < (unsigned)S.d1 &&
                     (unsigned)c2 < (unsigned)S.d2) {
-                    off[u] = (c0 * S.d1 + c1) * S.d2 + c2;
+                    // in-range indices of a brick below 2^24 points per axis
+                    // product: full-rate 24-bit multiplies are exact here
+                    off[u] = (int)__umul24(__umul24(c0, S.d1) + c1, S.d2) + c2;
                     dl[u] = (float)(exit_t - enter_t);
                 }
                 int cn, dn;
@@ -198,7 +200,7 @@ struct TFCache {
     int n_fits;
     int grey;
     int fid;       // mode 1: the one field
-    float b0, b1, idbin;
+    float b0, b1, idbin, nb0idbin;  // nb0idbin = -b0 * idbin
     int nbm2;      // mode 1: nbins - 2
     int nbins[6];
     int off[6];
@@ -223,6 +225,7 @@ __device__ __forceinline__ void load_tf(const DTF &T, TFCache &c)
     c.b0 = unif(T.b0[0]);
     c.b1 = unif(T.b1[0]);
     c.idbin = unif(T.idbin[0]);
+    c.nb0idbin = -c.b0 * c.idbin;
     c.nbm2 = uni(T.nbins[0] - 2);
 #pragma unroll
     for (int t = 0; t < 6; t++) {
@@ -289,11 +292,13 @@ __device__ __forceinline__ void fit_eval_ctf(const TFCache &tf, const float4 *__
 {
     const bool ok = (v < tf.b1) && (v > tf.b0) && __builtin_isnormal(v);
     const float vs = ok ? v : tf.b0;
-    int bin = (int)((vs - tf.b0) * tf.idbin);
+    int bin = (int)__builtin_fmaf(vs, tf.idbin, tf.nb0idbin);
     bin = bin > tf.nbm2 ? tf.nbm2 : bin;
-    const float4 y = lt[bin];
-    const float4 dy = lt[nb + bin];
-    const float d0 = reinterpret_cast<const float *>(lt + 2 * nb)[bin];
+    // Bins are interleaved {y_rgba}, {dy_rgba}, {d0, -, -, -}: one address.
+    const float4 *q = lt + 3 * bin;
+    const float4 y = q[0];
+    const float4 dy = q[1];
+    const float d0 = q[2].x;
     const float dd = vs - d0;
     float a = __builtin_fmaf(dd, dy.w, y.w);
     a = ok ? a : 0.0f;
@@ -368,8 +373,8 @@ struct VRSampler {
     const float *data;
     const uint8_t *mask;
     BrickReg b;
-    double pos[3];
-    double dir[3];
+    double pa[3];  // v_dir * idds (wave-uniform)
+    double pb[3];  // (v_pos - left_edge) * idds
     const TFCache *tf;
     const float4 *lt;
     int nb;
@@ -448,10 +453,10 @@ struct VRSampler {
         if (!((unsigned)c0 < (unsigned)b.d0 && (unsigned)c1 < (unsigned)d1 &&
               (unsigned)c2 < (unsigned)d2))
             return;
-        const int cell = (c0 * d1 + c1) * d2 + c2;
+        const int cell = (int)__umul24(__umul24(c0, d1) + c1, d2) + c2;
         const int s1 = d2 + 1;
         const int s0 = (d1 + 1) * (d2 + 1);
-        const int voff = c0 * s0 + c1 * s1 + c2;
+        const int voff = (int)__umul24(c0, s0) + (int)__umul24(c1, s1) + c2;
         // The mask byte and the corner values are independent loads: issue
         // both before testing the mask.
         const int mval = (mask != nullptr) ? (int)mask[cell] : 1;
@@ -459,15 +464,14 @@ struct VRSampler {
         // in the order of image_samplers.pyx:475-479.
         const double dt = (ex - en) * inv_ns;
         const double tt = en + 0.5 * dt;
+        // Cell-relative position (tt*v_dir + v_pos - (idx*dds + LE)) * idds,
+        // as tt*pa + pb - idx with pa = v_dir*idds, pb = (v_pos - LE)*idds.
         float dp[3], ds[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const int ci = (i == 0) ? c0 : ((i == 1) ? c1 : c2);
-            double p = tt * dir[i] + pos[i];
-            p -= (double)ci * b.dds[i] + b.le[i];
-            p *= b.idds[i];
-            dp[i] = (float)p;
-            ds[i] = (float)(dir[i] * b.idds[i] * dt);
+            dp[i] = (float)(__builtin_fma(tt, pa[i], pb[i]) - (double)ci);
+            ds[i] = (float)(pa[i] * dt);
         }
         float cv[8][NFP];
         {
@@ -591,8 +595,8 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
             s.b = brick_reg(b);
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                s.pos[i] = pos[i];
-                s.dir[i] = dir[i];
+                s.pa[i] = uni_d(dir[i] * b.idds[i]);
+                s.pb[i] = (pos[i] - b.le[i]) * b.idds[i];
             }
             s.tf = &tfc;
             s.lt = lt;
@@ -980,6 +984,11 @@ int ytrc_dataset_upload(ytrc_ctx *ctx, int32_t n_bricks, const ytrc_brick_desc *
         if (np >= (int64_t)1 << 31)
             return fail(YTRC_EINVAL, "ytrc_dataset_upload: brick " + std::to_string(b) +
                                          " exceeds 2^31 points (int32 cell index, image_samplers.pyx:341)");
+        // The kernels form cell and vertex offsets with 24-bit multiplies.
+        if ((int64_t)(d.dims[0] + 1) * (d.dims[1] + 1) >= (int64_t)1 << 24 ||
+            (int64_t)(d.dims[1] + 1) * (d.dims[2] + 1) >= (int64_t)1 << 24)
+            return fail(YTRC_EINVAL, "ytrc_dataset_upload: brick " + std::to_string(b) +
+                                         " has a face of 2^24 or more points; split it into smaller bricks");
         h.data_off = pool_floats;
         pool_floats += align_up((size_t)np * ds.nfp, 64);
         bool has_mask = mask_ptrs != nullptr && mask_ptrs[b] != nullptr;
@@ -1210,14 +1219,13 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
                   T.chexp[2] == ((1 << 4) | (1 << 6)) && T.chexp[3] == (1 << 6);
             if (ctf) {
                 const int nb = tf.nbins[0];
-                std::vector<float4> lay((size_t)2 * nb + (nb + 3) / 4);
-                float *d0 = reinterpret_cast<float *>(lay.data() + 2 * nb);
+                std::vector<float4> lay((size_t)3 * nb);
                 for (int k = 0; k < nb; k++) {
                     const float4 a0 = tft[i][k], a1 = tft[i][nb + k], a2 = tft[i][2 * nb + k],
                                  a3 = tft[i][3 * nb + k];
-                    lay[k] = make_float4(a0.x, a1.x, a2.x, a3.x);
-                    lay[nb + k] = make_float4(a0.y, a1.y, a2.y, a3.y);
-                    d0[k] = a0.z;
+                    lay[3 * k] = make_float4(a0.x, a1.x, a2.x, a3.x);
+                    lay[3 * k + 1] = make_float4(a0.y, a1.y, a2.y, a3.y);
+                    lay[3 * k + 2] = make_float4(a0.z, 0.0f, 0.0f, 0.0f);
                 }
                 tft[i] = std::move(lay);
                 tot = (int)tft[i].size();
