@@ -119,6 +119,46 @@ def cpu_baseline(ds, vol, tf, cams, samples, budget_s):
     }
 
 
+def serve_leg(ctx, vol512, tf512, rates, n):
+    """Continuous-batching server over three resident datasets: 256^3 and
+    512^3 uniform lognormal fields and the seeded AMR hierarchy (128^3 root,
+    3 levels); requests of side uniform in [256, 2048], half projection and
+    half VR (8-Gaussian CTF per dataset), Poisson arrivals (seed 1005)."""
+    from yt_amd import _lib, data, render, serve
+    from yt_amd.transfer_functions import gaussian_ctf
+
+    t = time.time()
+    rho = data.lognormal_density(256, seed=1001, dtype=np.float32)
+    v256 = render.ResidentVolume(ctx, data.load_uniform_grid({"density": rho}),
+                                 proj_fields="density", vr_field="density")
+    amr = data.load_amr_grids(data.make_amr_hierarchy(128, 3), [128] * 3)
+    vamr = render.ResidentVolume(ctx, amr, proj_fields="density", vr_field="density")
+    log(f"serving datasets resident in {time.time() - t:.1f}s "
+        f"(AMR: {len(amr.grids)} grids, {vamr.n_vr_bricks} kd bricks)")
+
+    def tf_for(ds):
+        lo = min(float(np.log10(np.min(g["density"]))) for g in ds.grids)
+        hi = max(float(np.log10(np.max(g["density"]))) for g in ds.grids)
+        return gaussian_ctf((lo, hi), n_gauss=8, nbins=256)
+
+    vols = [v256, vol512, vamr]
+    tfs = {id(v256): tf_for(v256.ds), id(vol512): tf512, id(vamr): tf_for(amr)}
+    srv = _lib.Server(ctx, max_requests=256, budget_bytes=8 << 30, chunk_tiles=16384)
+    out = {}
+    for k, lam in enumerate(rates):
+        specs = serve.make_requests(vols, n, seed=1006 + k, tfs=tfs, rng_cam_seed=1004 + k)
+        res = serve.run_poisson(srv, specs, lam, seed=1005 + k)
+        out[f"{lam:g}"] = {kk: (round(vv, 3) if isinstance(vv, float) else vv)
+                           for kk, vv in serve.summarize(res, specs).items()}
+        log(f"serving rate {lam:g}/s: {out[f'{lam:g}']}")
+    srv.close()
+    v256.free()
+    vamr.free()
+    return {"datasets": ["256^3 uniform", "512^3 uniform (8 bricks)",
+                         f"AMR 128^3+3 levels ({len(amr.grids)} grids)"],
+            "sides": [256, 2048], "budget_gib": 8, "rates": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +172,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--lat-reps", type=int, default=20)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rates", default="10,50,200",
+                    help="Poisson arrival rates (req/s) of the serving leg")
+    ap.add_argument("--serve-n", type=int, default=48, help="requests per rate")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run the HBM copy probe on this many GiB and exit")
     args = ap.parse_args()
@@ -241,6 +285,11 @@ def main():
         lat[name] = dict(p50_ms=float(np.percentile(xs, 50)), p99_ms=float(np.percentile(xs, 99)))
     log(f"latency {lat}")
 
+    serving = None
+    if not args.no_serve:
+        serving = serve_leg(ctx, vol, tf, [float(x) for x in args.serve_rates.split(",")],
+                            args.serve_n)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(ds, vol, tf, cams, samples, args.cpu_seconds)
@@ -276,6 +325,7 @@ def main():
         "images_per_s": round(imgs_s, 2),
         "plain_images_per_s": None if plain is None else round(plain["images_per_s"], 2),
         "latency_ms": lat,
+        "serving": serving,
         "roofline": {
             "bound": "hbm",
             "kernel": "k_persist<VR>" if dom == 1 else "k_persist<projection>",

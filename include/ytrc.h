@@ -152,6 +152,35 @@ int ytrc_batch_request_ms(ytrc_batch *b, int32_t i, float *ms);
 int ytrc_batch_bytes(const ytrc_batch *b, uint64_t *bytes);
 void ytrc_batch_destroy(ytrc_batch *b);
 
+/* Serving: continuous batching against resident datasets.  The reference
+ * renders one sampler at a time with no queue (render_source.py:594-599,
+ * off_axis_projection.py:506-526); a server admits requests while others
+ * are on the GPU and runs the 8x8 ray tiles of every admitted request, in
+ * arrival order, in chunks of `chunk_tiles` persistent-kernel work items
+ * (up to two chunks in flight).  `budget_bytes` of device memory is
+ * allocated once; per-request descriptors and output images come from it.
+ * submit returns YTRC_EBUDGET when the request does not fit now (no free
+ * slot or budget): the caller retries after reading finished requests. */
+typedef struct ytrc_server ytrc_server;
+int ytrc_server_create(ytrc_ctx *ctx, int32_t max_requests, uint64_t budget_bytes,
+                       int32_t chunk_tiles, ytrc_server **out);
+int ytrc_server_submit(ytrc_server *s, const ytrc_request *r, int64_t *id_out);
+/* Retires finished chunks, launches new ones, and returns up to `cap` ids
+ * of requests finished since the last call (non-blocking). */
+int ytrc_server_progress(ytrc_server *s, int64_t *done_ids, int32_t cap, int32_t *n_done);
+/* As progress, but first waits for the oldest chunk in flight. */
+int ytrc_server_wait(ytrc_server *s, int64_t *done_ids, int32_t cap, int32_t *n_done);
+/* Copies a finished request's image (float64 nv0*nv1*4) and releases it. */
+int ytrc_server_read(ytrc_server *s, int64_t id, double *image_out);
+/* Same, as the float32 planes the kernels write (channel-major,
+ * nch * nv0 * nv1): the written channels only, no layout conversion. */
+int ytrc_server_read_planar(ytrc_server *s, int64_t id, float *planes, int32_t cap_ch, int32_t *nch);
+int ytrc_server_cancel(ytrc_server *s, int64_t id);
+/* stats[8]: requests held, requests with tiles queued, chunks in flight,
+ * bytes used, budget, tiles launched, chunks launched, free slots. */
+int ytrc_server_stats(const ytrc_server *s, int64_t *stats);
+void ytrc_server_destroy(ytrc_server *s);
+
 /* Achievable HBM bandwidth: a streaming copy of `bytes` (read + write
  * counted) timed over `iters` launches with HIP events.  No counterpart in
  * the reference; used to confirm the bandwidth roofline on the device. */

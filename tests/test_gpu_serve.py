@@ -1,0 +1,119 @@
+"""GPU tests of the continuous-batching server (ytrc_server_*).
+
+Each request's image is bit-identical whether it runs alone (one plain
+launch) or mixed with other requests of other sizes, kinds and datasets in
+the server's chunks: every pixel is computed by the same per-ray code, so
+the tolerance written here is exact equality.
+"""
+
+import numpy as np
+import pytest
+
+from yt_amd import _lib, render
+
+from .helpers import c0_params, small_amr, uniform_lognormal, vr_tf_for
+
+pytestmark = pytest.mark.gpu
+
+
+def _alone(ctx, spec):
+    b = _lib.Batch(ctx, [spec], _lib.BATCH_PLAIN)
+    try:
+        b.run()
+        return b.read_image(0)
+    finally:
+        b.close()
+
+
+@pytest.fixture(scope="module")
+def volumes(ctx):
+    a = uniform_lognormal(40, seed=41)
+    b = small_amr(root_n=16, levels=2, seed=43)
+    va = render.ResidentVolume(ctx, a, proj_fields="density", vr_field="density")
+    vb = render.ResidentVolume(ctx, b, proj_fields="density", vr_field="density")
+    tfs = {id(va): vr_tf_for(a, nbins=128), id(vb): vr_tf_for(b, nbins=256)}
+    yield [va, vb], tfs
+    va.free()
+    vb.free()
+
+
+def _specs(volumes, tfs, n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        vol = volumes[i % len(volumes)]
+        nrm = rng.normal(size=3)
+        p = c0_params(int(rng.integers(8, 140)), normal=tuple(nrm / np.linalg.norm(nrm)))
+        out.append(vol.vr_request(p, tfs[id(vol)]) if rng.random() < 0.5
+                   else vol.projection_request(p))
+    return out
+
+
+def test_server_mix_equals_alone(ctx, volumes):
+    vols, tfs = volumes
+    specs = _specs(vols, tfs, 10, seed=7)
+    alone = [_alone(ctx, s) for s in specs]
+    srv = _lib.Server(ctx, max_requests=16, budget_bytes=256 << 20, chunk_tiles=256)
+    try:
+        ids = {srv.submit(s): i for i, s in enumerate(specs)}
+        assert None not in ids
+        got = {}
+        while len(got) < len(specs):
+            for rid in srv.wait():
+                got[ids[rid]] = srv.read(rid)
+        for i in range(len(specs)):
+            nch = 1 if specs[i].kind == _lib.PROJECTION else 3
+            assert np.array_equal(got[i][..., :nch], alone[i][..., :nch]), i
+        st = srv.stats()
+        assert st["held"] == 0 and st["used"] == 0 and st["chunks"] > 1
+    finally:
+        srv.close()
+
+
+def test_server_admission_waits_for_budget(ctx, volumes):
+    vols, tfs = volumes
+    specs = _specs(vols, tfs, 6, seed=9)
+    # a budget that holds one 140^2 request region at a time
+    srv = _lib.Server(ctx, max_requests=4, budget_bytes=1 << 20, chunk_tiles=128)
+    try:
+        pending = list(range(len(specs)))
+        out = {}
+        live = {}
+        rejected = 0
+        while pending or live:
+            while pending:
+                rid = srv.submit(specs[pending[0]])
+                if rid is None:
+                    rejected += 1
+                    break
+                live[rid] = pending.pop(0)
+            for rid in srv.wait():
+                out[live.pop(rid)] = srv.read(rid)
+        assert rejected > 0
+        for i, s in enumerate(specs):
+            nch = 1 if s.kind == _lib.PROJECTION else 3
+            assert np.array_equal(out[i][..., :nch], _alone(ctx, s)[..., :nch]), i
+    finally:
+        srv.close()
+
+
+def test_server_cancel(ctx, volumes):
+    vols, tfs = volumes
+    specs = _specs(vols, tfs, 5, seed=11)
+    srv = _lib.Server(ctx, max_requests=8, budget_bytes=64 << 20, chunk_tiles=64)
+    try:
+        ids = [srv.submit(s) for s in specs]
+        srv.cancel(ids[2])
+        done = set()
+        while len(done) < 4:
+            for rid in srv.wait():
+                done.add(rid)
+                srv.read_planar(rid)
+        assert ids[2] not in done
+        for _ in range(10):
+            srv.progress()
+            if srv.stats()["held"] == 0:
+                break
+        assert srv.stats()["used"] == 0
+    finally:
+        srv.close()

@@ -51,6 +51,15 @@ EXPORTED = [
     "ytrc_batch_kernel_ms",
     "ytrc_batch_request_ms",
     "ytrc_hbm_probe",
+    "ytrc_server_create",
+    "ytrc_server_submit",
+    "ytrc_server_progress",
+    "ytrc_server_wait",
+    "ytrc_server_read",
+    "ytrc_server_read_planar",
+    "ytrc_server_cancel",
+    "ytrc_server_stats",
+    "ytrc_server_destroy",
 ]
 
 
@@ -145,9 +154,22 @@ def _declare(lib):
     lib.ytrc_batch_request_ms.argtypes = [P, C.c_int32, C.POINTER(C.c_float)]
     lib.ytrc_hbm_probe.argtypes = [P, C.c_uint64, C.c_int32, C.POINTER(C.c_float),
                                    C.POINTER(C.c_float)]
+    I64P = C.POINTER(C.c_int64)
+    lib.ytrc_server_create.argtypes = [P, C.c_int32, C.c_uint64, C.c_int32, C.POINTER(P)]
+    lib.ytrc_server_submit.argtypes = [P, C.POINTER(Request), I64P]
+    lib.ytrc_server_progress.argtypes = [P, I64P, C.c_int32, C.POINTER(C.c_int32)]
+    lib.ytrc_server_wait.argtypes = [P, I64P, C.c_int32, C.POINTER(C.c_int32)]
+    lib.ytrc_server_read.argtypes = [P, C.c_int64, C.POINTER(C.c_double)]
+    lib.ytrc_server_read_planar.argtypes = [P, C.c_int64, C.POINTER(C.c_float), C.c_int32,
+                                            C.POINTER(C.c_int32)]
+    lib.ytrc_server_cancel.argtypes = [P, C.c_int64]
+    lib.ytrc_server_stats.argtypes = [P, I64P]
+    lib.ytrc_server_destroy.argtypes = [P]
+    lib.ytrc_server_destroy.restype = None
     for name in EXPORTED:
         f = getattr(lib, name)
-        if name not in ("ytrc_ctx_destroy", "ytrc_batch_destroy", "ytrc_last_error"):
+        if name not in ("ytrc_ctx_destroy", "ytrc_batch_destroy", "ytrc_last_error",
+                        "ytrc_server_destroy"):
             f.restype = C.c_int
 
 
@@ -418,6 +440,79 @@ class Batch:
     def close(self):
         if self._h:
             lib().ytrc_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching server (ytrc_server_*): submit requests as they
+    arrive, call progress() to advance, read() finished images."""
+
+    def __init__(self, ctx, max_requests=256, budget_bytes=8 << 30, chunk_tiles=16384):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        _check(lib().ytrc_server_create(ctx._h, int(max_requests), int(budget_bytes),
+                                        int(chunk_tiles), C.byref(self._h)))
+        self._specs = {}
+        self._ids = (C.c_int64 * 1024)()
+
+    def submit(self, spec):
+        """Returns the request id, or None when it is not admitted now."""
+        rid = C.c_int64()
+        r = spec.to_c()
+        rc = lib().ytrc_server_submit(self._h, C.byref(r), C.byref(rid))
+        if rc == YTRC_EBUDGET:
+            return None
+        _check(rc)
+        self._specs[rid.value] = spec
+        return rid.value
+
+    def _done(self, fn):
+        n = C.c_int32()
+        _check(fn(self._h, self._ids, 1024, C.byref(n)))
+        return [self._ids[i] for i in range(n.value)]
+
+    def progress(self):
+        return self._done(lib().ytrc_server_progress)
+
+    def wait(self):
+        return self._done(lib().ytrc_server_wait)
+
+    def read(self, rid):
+        spec = self._specs.pop(rid)
+        out = np.empty((spec.nv[0], spec.nv[1], 4), dtype=np.float64)
+        _check(lib().ytrc_server_read(self._h, int(rid), _dptr(out)))
+        return out
+
+    def read_planar(self, rid):
+        """(nch, nv0, nv1) float32: the channels the kernel wrote."""
+        spec = self._specs.pop(rid)
+        out = np.empty((4, spec.nv[0], spec.nv[1]), dtype=np.float32)
+        n = C.c_int32()
+        _check(lib().ytrc_server_read_planar(self._h, int(rid),
+                                             out.ctypes.data_as(C.POINTER(C.c_float)), 4,
+                                             C.byref(n)))
+        return out[:n.value]
+
+    def cancel(self, rid):
+        _check(lib().ytrc_server_cancel(self._h, int(rid)))
+        self._specs.pop(rid, None)
+
+    def stats(self):
+        s = (C.c_int64 * 8)()
+        _check(lib().ytrc_server_stats(self._h, s))
+        keys = ("held", "queued", "chunks_inflight", "used", "budget", "tiles", "chunks",
+                "free_slots")
+        return dict(zip(keys, list(s)))
+
+    def close(self):
+        if self._h:
+            lib().ytrc_server_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

@@ -19,6 +19,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <mutex>
 #include <string>
@@ -1075,6 +1076,305 @@ int ytrc_dataset_bytes(const ytrc_ctx *ctx, int32_t ds_id, uint64_t *bytes)
     return YTRC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Host-side preparation of one request: validation, per-brick extents,
+// transfer-function tables, kernel choice and the layout of its device
+// region (offsets relative to the region start).
+struct PrepReq {
+    ReqHost q;
+    std::vector<int4> ext;
+    DTF T;
+    std::vector<float4> tft;
+    size_t o_order = 0, o_ext = 0, o_zbuf = 0, o_init = 0, o_out = 0;
+    size_t o_rv = 0, o_rh = 0, o_rs = 0, o_tf = 0;
+    size_t bytes = 0;
+};
+
+int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, PrepReq &P)
+{
+    ReqHost &q = P.q;
+    q.r = r;
+    if (r.kind != YTRC_PROJECTION && r.kind != YTRC_VOLUME_RENDER)
+        return fail(YTRC_ENOTIMPL, tag + "unknown sampler kind");
+    if (r.lens != YTRC_LENS_PLANE_PARALLEL)
+        return fail(YTRC_ENOTIMPL, tag + "only the plane-parallel lens is implemented");
+    if (r.nv[0] <= 0 || r.nv[1] <= 0) return fail(YTRC_EINVAL, tag + "image size must be > 0");
+    auto dsit = ctx->datasets.find(r.ds_id);
+    if (dsit == ctx->datasets.end()) return fail(YTRC_EINVAL, tag + "unknown dataset");
+    const Dataset &ds = dsit->second;
+    if (r.n_order < 0 || (r.n_order > 0 && r.order == nullptr))
+        return fail(YTRC_EINVAL, tag + "bad brick order");
+    for (int k = 0; k < r.n_order; k++) {
+        if (r.order[k] < 0 || r.order[k] >= ds.n_bricks)
+            return fail(YTRC_EINVAL, tag + "brick order index out of range");
+        const DBrick &hb = ds.h_bricks[r.order[k]];
+        if (r.kind == YTRC_PROJECTION && hb.vertex)
+            return fail(YTRC_EINVAL, tag + "ProjectionSampler needs cell-centred bricks");
+        if (r.kind == YTRC_VOLUME_RENDER && !hb.vertex)
+            return fail(YTRC_EINVAL, tag + "VolumeRenderSampler needs vertex-centred bricks");
+    }
+    q.npix = (int64_t)r.nv[0] * r.nv[1];
+    const int t0 = (r.nv[0] + TILE - 1) / TILE, t1 = (r.nv[1] + TILE - 1) / TILE;
+    q.ntiles = t0 * t1;
+    q.nfp = ds.nfp;
+    const bool rec = (r.flags & YTRC_FLAG_RECORD) != 0;
+    DTF &T = P.T;
+    memset(&T, 0, sizeof(T));
+    if (r.kind == YTRC_PROJECTION) {
+        q.nch_out = std::min(4, ds.nf);
+    } else {
+        if (r.tf == nullptr) return fail(YTRC_EINVAL, tag + "VolumeRenderSampler needs a transfer function");
+        const ytrc_tf_desc &tf = *r.tf;
+        if (tf.n_fits < 0 || tf.n_fits > 6)
+            return fail(YTRC_EINVAL, tag + "n_fits must be <= 6 (image_samplers.pyx:430)");
+        if (r.n_samples <= 0) return fail(YTRC_EINVAL, tag + "n_samples must be > 0");
+        T.n_fits = tf.n_fits;
+        T.grey = tf.grey_opacity == 1;
+        int tot = 0;
+        for (int t = 0; t < tf.n_fits; t++) {
+            if (tf.nbins[t] < 2 || tf.y[t] == nullptr)
+                return fail(YTRC_EINVAL, tag + "table needs nbins >= 2 and values");
+            if (tf.field_ids[t] < 0 || tf.field_ids[t] >= ds.nf || tf.weight_field_ids[t] < -1 ||
+                tf.weight_field_ids[t] >= ds.nf || tf.weight_table_ids[t] < -1 ||
+                tf.weight_table_ids[t] >= tf.n_fits)
+                return fail(YTRC_EINVAL, tag + "table field/weight ids out of range");
+            T.nbins[t] = tf.nbins[t];
+            T.off[t] = tot;
+            T.field_id[t] = tf.field_ids[t];
+            T.wfield[t] = tf.weight_field_ids[t];
+            T.wtable[t] = tf.weight_table_ids[t];
+            const double b0 = tf.bounds[2 * t], b1 = tf.bounds[2 * t + 1];
+            T.b0[t] = (float)b0;
+            T.b1[t] = (float)b1;
+            // FIT_initialize_table (field_interpolation_tables.pxd:44-61) in FP64.
+            const double dbin = (b1 - b0) / (double)(tf.nbins[t] - 1);
+            const double idbin = 1.0 / dbin;
+            T.idbin[t] = (float)idbin;
+            for (int k = 0; k < tf.nbins[t]; k++) {
+                float4 q4;
+                q4.x = (float)tf.y[t][k];
+                if (k < tf.nbins[t] - 1) {
+                    q4.y = (float)((tf.y[t][k + 1] - tf.y[t][k]) * idbin);
+                    q4.z = (float)(b0 + (double)k * dbin);
+                } else {
+                    q4.y = 0.0f;
+                    q4.z = 0.0f;
+                }
+                q4.w = 0.0f;
+                P.tft.push_back(q4);
+            }
+            tot += tf.nbins[t];
+        }
+        for (int c = 0; c < 6; c++) {
+            if (tf.field_table_ids[c] < 0 || tf.field_table_ids[c] >= 6)
+                return fail(YTRC_EINVAL, tag + "field_table_ids out of range");
+            T.ftable[c] = tf.field_table_ids[c];
+        }
+        // Resolve FIT_eval_transfer's in-place weight loop
+        // (field_interpolation_tables.pxd:96-99) into powers of raw values.
+        int pw[6][6];
+        memset(pw, 0, sizeof(pw));
+        for (int t = 0; t < tf.n_fits; t++) pw[t][t] = 1;
+        for (int t = 0; t < tf.n_fits; t++) {
+            const int w = tf.weight_table_ids[t];
+            if (w == -1) continue;
+            int add[6];
+            for (int u = 0; u < 6; u++) add[u] = pw[w][u];
+            for (int u = 0; u < 6; u++) pw[t][u] += add[u];
+        }
+        for (int c = 0; c < 4; c++) {
+            const int t = tf.field_table_ids[c];
+            if (t >= tf.n_fits) {
+                T.chexp[c] = -1;  // table never evaluated: channel value 0
+                continue;
+            }
+            int ce = 0;
+            for (int u = 0; u < 6; u++) {
+                if (pw[t][u] > 3)
+                    return fail(YTRC_ENOTIMPL, tag + "weight-table chain raises a table to a power > 3");
+                ce |= pw[t][u] << (2 * u);
+            }
+            T.chexp[c] = ce;
+        }
+        // ColorTransferFunction pattern (transfer_functions.py:385-408):
+        // one shared bin per sample, tables interleaved per bin.
+        bool ctf = tf.n_fits == 4;
+        for (int t = 0; ctf && t < 4; t++)
+            ctf = tf.field_ids[t] == tf.field_ids[0] && tf.weight_field_ids[t] == -1 &&
+                  tf.nbins[t] == tf.nbins[0] && tf.bounds[2 * t] == tf.bounds[0] &&
+                  tf.bounds[2 * t + 1] == tf.bounds[1];
+        ctf = ctf && T.chexp[0] == (1 | (1 << 6)) && T.chexp[1] == ((1 << 2) | (1 << 6)) &&
+              T.chexp[2] == ((1 << 4) | (1 << 6)) && T.chexp[3] == (1 << 6);
+        if (ctf) {
+            const int nb = tf.nbins[0];
+            std::vector<float4> lay((size_t)3 * nb);
+            for (int k = 0; k < nb; k++) {
+                const float4 a0 = P.tft[k], a1 = P.tft[nb + k], a2 = P.tft[2 * nb + k],
+                             a3 = P.tft[3 * nb + k];
+                lay[3 * k] = make_float4(a0.x, a1.x, a2.x, a3.x);
+                lay[3 * k + 1] = make_float4(a0.y, a1.y, a2.y, a3.y);
+                lay[3 * k + 2] = make_float4(a0.z, 0.0f, 0.0f, 0.0f);
+            }
+            P.tft = std::move(lay);
+            tot = (int)P.tft.size();
+            T.mode = 1;
+        }
+        q.mode = T.mode | (r.n_samples == 10 ? 2 : 0);
+        // Transmittance-form compositing is exact when every sub-sample has
+        // a = dt * e <= 1: bound e by the tables' largest magnitudes and dt
+        // by the longest in-cell t-interval / n_samples.
+        if (!T.grey) {
+            bool bounded = true;
+            double emax = 0.0;
+            for (int t = 0; t < tf.n_fits; t++) bounded = bounded && tf.weight_field_ids[t] == -1;
+            for (int c = 0; bounded && c < 3; c++) {
+                if (T.chexp[c] < 0) continue;
+                double ec = 1.0;
+                for (int u = 0; u < tf.n_fits; u++) {
+                    const int pwr = (T.chexp[c] >> (2 * u)) & 3;
+                    double m = 0.0;
+                    for (int kk = 0; kk < tf.nbins[u]; kk++) m = std::max(m, std::fabs(tf.y[u][kk]));
+                    for (int q2 = 0; q2 < pwr; q2++) ec *= m;
+                }
+                emax = std::max(emax, ec);
+            }
+            double tcell = 0.0;
+            for (int k2 = 0; bounded && k2 < r.n_order; k2++) {
+                const DBrick &hb = ds.h_bricks[r.order[k2]];
+                double cmin = 1.0;
+                for (int a = 0; a < 3; a++)
+                    if (r.vp_dir[a] != 0.0) cmin = std::min(cmin, hb.dds[a] / std::fabs(r.vp_dir[a]));
+                tcell = std::max(tcell, cmin);
+            }
+            q.tform = bounded && (emax * (tcell / (double)r.n_samples) * 1.001 <= 1.0);
+        }
+        T.total_bins = tot;
+        q.lds = (size_t)tot * sizeof(float4);
+        if (q.lds > 64 * 1024) return fail(YTRC_EINVAL, tag + "transfer-function tables exceed 64 KiB of LDS");
+        q.nch_out = T.grey ? 4 : 3;
+    }
+    q.kfn = pick_kernel(r.kind, q.nfp, rec, q.mode);
+    if (q.kfn == nullptr) return fail(YTRC_ENOTIMPL, tag + "no kernel for this field count");
+    if (r.image_in != nullptr) q.image_in.assign(r.image_in, r.image_in + q.npix * 4);
+    P.ext.resize(std::max(1, r.n_order));
+    for (int k = 0; k < r.n_order; k++) {
+        int64_t rv[4];
+        extent_plane_parallel(r, ds.h_bricks[r.order[k]], rv);
+        P.ext[k] = make_int4((int)rv[0], (int)rv[1], (int)rv[2], (int)rv[3]);
+    }
+    // Device region layout.
+    size_t off = 0;
+    P.o_order = off;
+    off = align_up(off + sizeof(int32_t) * std::max(1, r.n_order), 256);
+    P.o_ext = off;
+    off = align_up(off + sizeof(int4) * P.ext.size(), 256);
+    P.o_zbuf = off;
+    if (r.zbuffer) off = align_up(off + sizeof(double) * q.npix, 256);
+    P.o_init = off;
+    if (r.image_in) off = align_up(off + sizeof(float) * 4 * q.npix, 256);
+    P.o_out = off;
+    off = align_up(off + sizeof(float) * 4 * q.npix, 256);
+    P.o_rv = P.o_rh = P.o_rs = off;
+    if (rec) {
+        P.o_rv = off;
+        off = align_up(off + 8 * q.npix, 256);
+        P.o_rh = off;
+        off = align_up(off + 8 * q.npix, 256);
+        P.o_rs = off;
+        off = align_up(off + 8 * q.npix, 256);
+    }
+    P.o_tf = off;
+    if (r.kind == YTRC_VOLUME_RENDER) off = align_up(off + sizeof(DTF), 256);
+    P.bytes = off;
+    return YTRC_OK;
+}
+
+// Copies a prepared request into its device region `A` and fills its
+// device descriptor.  `tf_table` is the device address of its (deduplicated)
+// transfer-function table, at float4 offset `tf_slot` of the launch's table.
+int stage_request(ytrc_ctx *ctx, PrepReq &P, char *A, const float4 *tf_table, int tf_slot, DImg &d,
+                  std::vector<float> &stage)
+{
+    const ytrc_request &r = P.q.r;
+    const ReqHost &q = P.q;
+    const Dataset &ds = ctx->datasets[r.ds_id];
+    const bool rec = (r.flags & YTRC_FLAG_RECORD) != 0;
+    memset(&d, 0, sizeof(d));
+    memcpy(d.vp_pos, r.vp_pos, sizeof(d.vp_pos));
+    memcpy(d.vp_dir, r.vp_dir, sizeof(d.vp_dir));
+    memcpy(d.width, r.width, sizeof(d.width));
+    d.bricks = ds.d_bricks;
+    d.pool = ds.d_pool;
+    d.masks = ds.d_masks;
+    d.order = (const int32_t *)(A + P.o_order);
+    d.ext = (const int4 *)(A + P.o_ext);
+    d.zbuf = r.zbuffer ? (const double *)(A + P.o_zbuf) : nullptr;
+    d.init = r.image_in ? (const float *)(A + P.o_init) : nullptr;
+    d.out = (float *)(A + P.o_out);
+    d.rec_visits = rec ? (int64_t *)(A + P.o_rv) : nullptr;
+    d.rec_hash = rec ? (uint64_t *)(A + P.o_rh) : nullptr;
+    d.rec_samples = rec ? (int64_t *)(A + P.o_rs) : nullptr;
+    d.tf = (r.kind == YTRC_VOLUME_RENDER) ? (const DTF *)(A + P.o_tf) : nullptr;
+    d.nv0 = r.nv[0];
+    d.nv1 = r.nv[1];
+    d.kind = r.kind;
+    d.n_samples = r.n_samples;
+    d.n_order = r.n_order;
+    d.nf = ds.nf;
+    d.nfp = ds.nfp;
+    d.tiles0 = (r.nv[0] + TILE - 1) / TILE;
+    d.tiles1 = (r.nv[1] + TILE - 1) / TILE;
+    d.flags = (int32_t)r.flags;
+    d.nch_out = q.nch_out;
+    d.inv_ns = r.n_samples > 0 ? 1.0 / (double)r.n_samples : 0.0;
+    d.tform = q.tform;
+    if (r.n_order > 0) {
+        HIPCHK(hipMemcpy(A + P.o_order, r.order, sizeof(int32_t) * r.n_order, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(A + P.o_ext, P.ext.data(), sizeof(int4) * r.n_order, hipMemcpyHostToDevice));
+    }
+    if (r.zbuffer)
+        HIPCHK(hipMemcpy(A + P.o_zbuf, r.zbuffer, sizeof(double) * q.npix, hipMemcpyHostToDevice));
+    if (r.image_in) {
+        stage.resize((size_t)q.npix * 4);
+        for (int64_t p = 0; p < q.npix; p++)
+            for (int c = 0; c < 4; c++) stage[(size_t)c * q.npix + p] = (float)r.image_in[p * 4 + c];
+        HIPCHK(hipMemcpy(A + P.o_init, stage.data(), sizeof(float) * 4 * q.npix, hipMemcpyHostToDevice));
+    }
+    if (r.kind == YTRC_VOLUME_RENDER) {
+        DTF T = P.T;
+        T.table = tf_table + tf_slot;
+        d.tf_slot = tf_slot;
+        HIPCHK(hipMemcpy(A + P.o_tf, &T, sizeof(DTF), hipMemcpyHostToDevice));
+    }
+    return YTRC_OK;
+}
+
+// Copies a finished request's output (and image_in for channels it does
+// not write) into the caller's float64 (nv0, nv1, 4) image.
+int read_request_image(const ReqHost &q, const DImg &d, double *image_out)
+{
+    std::vector<float> h((size_t)q.npix * q.nch_out);
+    HIPCHK(hipMemcpy(h.data(), d.out, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
+    for (int64_t p = 0; p < q.npix; p++) {
+        for (int c = 0; c < 4; c++) {
+            double v;
+            if (c < q.nch_out)
+                v = (double)h[(size_t)c * q.npix + p];
+            else
+                v = q.image_in.empty() ? 0.0 : q.image_in[(size_t)p * 4 + c];
+            image_out[(size_t)p * 4 + c] = v;
+        }
+    }
+    return YTRC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
                       uint32_t batch_flags, ytrc_batch **out)
 {
@@ -1083,232 +1383,41 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
     if (n_req <= 0) return fail(YTRC_EINVAL, "ytrc_batch_create: n_req must be > 0");
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
+    std::vector<PrepReq> P(n_req);
+    size_t off = 0;
+    std::vector<size_t> base(n_req);
+    for (int i = 0; i < n_req; i++) {
+        const int rc = prep_request(ctx, reqs[i], "request " + std::to_string(i) + ": ", P[i]);
+        if (rc != YTRC_OK) return rc;
+        base[i] = off;
+        off = align_up(off + P[i].bytes, 256);
+    }
     ytrc_batch *B = new ytrc_batch();
     B->ctx = ctx;
     B->flags = batch_flags;
     B->reqs.resize(n_req);
     B->h_imgs.resize(n_req);
-    // Pass 1: validate and size the arena.
-    size_t off = 0;
-    struct Offs {
-        size_t order, ext, zbuf, init, out, rv, rh, rs, tf;
-    };
-    std::vector<Offs> offs(n_req);
-    std::vector<std::vector<int4>> exts(n_req);
-    std::vector<DTF> tfs(n_req);
-    std::vector<std::vector<float4>> tft(n_req);
+    for (int i = 0; i < n_req; i++) B->reqs[i] = P[i].q;
     auto bad = [&](int code, const std::string &m) {
+        if (B->arena) {
+            hipFree(B->arena);
+            ctx->used -= B->arena_bytes;
+        }
         delete B;
         return fail(code, m);
     };
-    for (int i = 0; i < n_req; i++) {
-        const ytrc_request &r = reqs[i];
-        ReqHost &q = B->reqs[i];
-        q.r = r;
-        const std::string tag = "request " + std::to_string(i) + ": ";
-        if (r.kind != YTRC_PROJECTION && r.kind != YTRC_VOLUME_RENDER)
-            return bad(YTRC_ENOTIMPL, tag + "unknown sampler kind");
-        if (r.lens != YTRC_LENS_PLANE_PARALLEL)
-            return bad(YTRC_ENOTIMPL, tag + "only the plane-parallel lens is implemented");
-        if (r.nv[0] <= 0 || r.nv[1] <= 0) return bad(YTRC_EINVAL, tag + "image size must be > 0");
-        auto dsit = ctx->datasets.find(r.ds_id);
-        if (dsit == ctx->datasets.end()) return bad(YTRC_EINVAL, tag + "unknown dataset");
-        const Dataset &ds = dsit->second;
-        if (r.n_order < 0 || (r.n_order > 0 && r.order == nullptr))
-            return bad(YTRC_EINVAL, tag + "bad brick order");
-        for (int k = 0; k < r.n_order; k++) {
-            if (r.order[k] < 0 || r.order[k] >= ds.n_bricks)
-                return bad(YTRC_EINVAL, tag + "brick order index out of range");
-            const DBrick &hb = ds.h_bricks[r.order[k]];
-            if (r.kind == YTRC_PROJECTION && hb.vertex)
-                return bad(YTRC_EINVAL, tag + "ProjectionSampler needs cell-centred bricks");
-            if (r.kind == YTRC_VOLUME_RENDER && !hb.vertex)
-                return bad(YTRC_EINVAL, tag + "VolumeRenderSampler needs vertex-centred bricks");
-        }
-        q.npix = (int64_t)r.nv[0] * r.nv[1];
-        const int t0 = (r.nv[0] + TILE - 1) / TILE, t1 = (r.nv[1] + TILE - 1) / TILE;
-        q.ntiles = t0 * t1;
-        q.nfp = ds.nfp;
-        const bool rec = (r.flags & YTRC_FLAG_RECORD) != 0;
-        if (r.kind == YTRC_PROJECTION) {
-            q.nch_out = std::min(4, ds.nf);
-        } else {
-            if (r.tf == nullptr) return bad(YTRC_EINVAL, tag + "VolumeRenderSampler needs a transfer function");
-            const ytrc_tf_desc &tf = *r.tf;
-            if (tf.n_fits < 0 || tf.n_fits > 6)
-                return bad(YTRC_EINVAL, tag + "n_fits must be <= 6 (image_samplers.pyx:430)");
-            if (r.n_samples <= 0) return bad(YTRC_EINVAL, tag + "n_samples must be > 0");
-            DTF &T = tfs[i];
-            memset(&T, 0, sizeof(T));
-            T.n_fits = tf.n_fits;
-            T.grey = tf.grey_opacity == 1;
-            int tot = 0;
-            for (int t = 0; t < tf.n_fits; t++) {
-                if (tf.nbins[t] < 2 || tf.y[t] == nullptr)
-                    return bad(YTRC_EINVAL, tag + "table needs nbins >= 2 and values");
-                if (tf.field_ids[t] < 0 || tf.field_ids[t] >= ds.nf ||
-                    tf.weight_field_ids[t] < -1 || tf.weight_field_ids[t] >= ds.nf ||
-                    tf.weight_table_ids[t] < -1 || tf.weight_table_ids[t] >= tf.n_fits)
-                    return bad(YTRC_EINVAL, tag + "table field/weight ids out of range");
-                T.nbins[t] = tf.nbins[t];
-                T.off[t] = tot;
-                T.field_id[t] = tf.field_ids[t];
-                T.wfield[t] = tf.weight_field_ids[t];
-                T.wtable[t] = tf.weight_table_ids[t];
-                const double b0 = tf.bounds[2 * t], b1 = tf.bounds[2 * t + 1];
-                T.b0[t] = (float)b0;
-                T.b1[t] = (float)b1;
-                // FIT_initialize_table (field_interpolation_tables.pxd:44-61) in FP64.
-                const double dbin = (b1 - b0) / (double)(tf.nbins[t] - 1);
-                const double idbin = 1.0 / dbin;
-                T.idbin[t] = (float)idbin;
-                for (int k = 0; k < tf.nbins[t]; k++) {
-                    float4 q4;
-                    q4.x = (float)tf.y[t][k];
-                    if (k < tf.nbins[t] - 1) {
-                        q4.y = (float)((tf.y[t][k + 1] - tf.y[t][k]) * idbin);
-                        q4.z = (float)(b0 + (double)k * dbin);
-                    } else {
-                        q4.y = 0.0f;
-                        q4.z = 0.0f;
-                    }
-                    q4.w = 0.0f;
-                    tft[i].push_back(q4);
-                }
-                tot += tf.nbins[t];
-            }
-            for (int c = 0; c < 6; c++) {
-                if (tf.field_table_ids[c] < 0 || tf.field_table_ids[c] >= 6)
-                    return bad(YTRC_EINVAL, tag + "field_table_ids out of range");
-                T.ftable[c] = tf.field_table_ids[c];
-            }
-            // Resolve FIT_eval_transfer's in-place weight loop
-            // (field_interpolation_tables.pxd:96-99) into powers of raw values.
-            int pw[6][6];
-            memset(pw, 0, sizeof(pw));
-            for (int t = 0; t < tf.n_fits; t++) pw[t][t] = 1;
-            for (int t = 0; t < tf.n_fits; t++) {
-                const int w = tf.weight_table_ids[t];
-                if (w == -1) continue;
-                int add[6];
-                for (int u = 0; u < 6; u++) add[u] = pw[w][u];
-                for (int u = 0; u < 6; u++) pw[t][u] += add[u];
-            }
-            for (int c = 0; c < 4; c++) {
-                const int t = tf.field_table_ids[c];
-                if (t >= tf.n_fits) {
-                    T.chexp[c] = -1;  // table never evaluated: channel value 0
-                    continue;
-                }
-                int ce = 0;
-                for (int u = 0; u < 6; u++) {
-                    if (pw[t][u] > 3)
-                        return bad(YTRC_ENOTIMPL, tag + "weight-table chain raises a table to a power > 3");
-                    ce |= pw[t][u] << (2 * u);
-                }
-                T.chexp[c] = ce;
-            }
-            // ColorTransferFunction pattern (transfer_functions.py:385-408):
-            // one shared bin per sample, tables laid out per bin.
-            bool ctf = tf.n_fits == 4;
-            for (int t = 0; ctf && t < 4; t++)
-                ctf = tf.field_ids[t] == tf.field_ids[0] && tf.weight_field_ids[t] == -1 &&
-                      tf.nbins[t] == tf.nbins[0] && tf.bounds[2 * t] == tf.bounds[0] &&
-                      tf.bounds[2 * t + 1] == tf.bounds[1];
-            ctf = ctf && T.chexp[0] == (1 | (1 << 6)) && T.chexp[1] == ((1 << 2) | (1 << 6)) &&
-                  T.chexp[2] == ((1 << 4) | (1 << 6)) && T.chexp[3] == (1 << 6);
-            if (ctf) {
-                const int nb = tf.nbins[0];
-                std::vector<float4> lay((size_t)3 * nb);
-                for (int k = 0; k < nb; k++) {
-                    const float4 a0 = tft[i][k], a1 = tft[i][nb + k], a2 = tft[i][2 * nb + k],
-                                 a3 = tft[i][3 * nb + k];
-                    lay[3 * k] = make_float4(a0.x, a1.x, a2.x, a3.x);
-                    lay[3 * k + 1] = make_float4(a0.y, a1.y, a2.y, a3.y);
-                    lay[3 * k + 2] = make_float4(a0.z, 0.0f, 0.0f, 0.0f);
-                }
-                tft[i] = std::move(lay);
-                tot = (int)tft[i].size();
-                T.mode = 1;
-            }
-            q.mode = T.mode | (r.n_samples == 10 ? 2 : 0);
-            // Transmittance-form compositing is exact when every sub-sample
-            // has a = dt * e <= 1: bound e by the tables' largest magnitudes
-            // and dt by the longest in-cell t-interval / n_samples.
-            if (!T.grey) {
-                bool bounded = true;
-                double emax = 0.0;
-                for (int t = 0; t < tf.n_fits; t++) bounded = bounded && tf.weight_field_ids[t] == -1;
-                for (int c = 0; bounded && c < 3; c++) {
-                    if (T.chexp[c] < 0) continue;
-                    double ec = 1.0;
-                    for (int u = 0; u < tf.n_fits; u++) {
-                        const int pwr = (T.chexp[c] >> (2 * u)) & 3;
-                        double m = 0.0;
-                        for (int kk = 0; kk < tf.nbins[u]; kk++) m = std::max(m, std::fabs(tf.y[u][kk]));
-                        for (int q2 = 0; q2 < pwr; q2++) ec *= m;
-                    }
-                    emax = std::max(emax, ec);
-                }
-                double tcell = 0.0;
-                for (int k2 = 0; bounded && k2 < r.n_order; k2++) {
-                    const DBrick &hb = ds.h_bricks[r.order[k2]];
-                    double cmin = 1.0;
-                    for (int a = 0; a < 3; a++)
-                        if (r.vp_dir[a] != 0.0) cmin = std::min(cmin, hb.dds[a] / std::fabs(r.vp_dir[a]));
-                    tcell = std::max(tcell, cmin);
-                }
-                q.tform = bounded && (emax * (tcell / (double)r.n_samples) * 1.001 <= 1.0);
-            }
-            T.total_bins = tot;
-            q.lds = (size_t)tot * sizeof(float4);
-            if (q.lds > 64 * 1024) return bad(YTRC_EINVAL, tag + "transfer-function tables exceed 64 KiB of LDS");
-            q.nch_out = T.grey ? 4 : 3;
-        }
-        q.kfn = pick_kernel(r.kind, q.nfp, rec, q.mode);
-        if (q.kfn == nullptr) return bad(YTRC_ENOTIMPL, tag + "no kernel for this field count");
-        if (r.image_in != nullptr) q.image_in.assign(r.image_in, r.image_in + q.npix * 4);
-        Offs &o = offs[i];
-        exts[i].resize(std::max(1, r.n_order));
-        for (int k = 0; k < r.n_order; k++) {
-            int64_t rv[4];
-            extent_plane_parallel(r, ds.h_bricks[r.order[k]], rv);
-            exts[i][k] = make_int4((int)rv[0], (int)rv[1], (int)rv[2], (int)rv[3]);
-        }
-        o.order = off;
-        off = align_up(off + sizeof(int32_t) * std::max(1, r.n_order), 256);
-        o.ext = off;
-        off = align_up(off + sizeof(int4) * exts[i].size(), 256);
-        o.zbuf = off;
-        if (r.zbuffer) off = align_up(off + sizeof(double) * q.npix, 256);
-        o.init = off;
-        if (r.image_in) off = align_up(off + sizeof(float) * 4 * q.npix, 256);
-        o.out = off;
-        off = align_up(off + sizeof(float) * 4 * q.npix, 256);
-        o.rv = o.rh = o.rs = off;
-        if (rec) {
-            o.rv = off;
-            off = align_up(off + 8 * q.npix, 256);
-            o.rh = off;
-            off = align_up(off + 8 * q.npix, 256);
-            o.rs = off;
-            off = align_up(off + 8 * q.npix, 256);
-        }
-        o.tf = off;
-        if (r.kind == YTRC_VOLUME_RENDER) off = align_up(off + sizeof(DTF), 256);
-    }
     // Unique transfer-function tables share one slot of the batch table.
     std::vector<float4> tf_all;
     std::map<std::string, int> tf_slot_of;
     std::vector<int> slot(n_req, 0);
     for (int i = 0; i < n_req; i++) {
         if (reqs[i].kind != YTRC_VOLUME_RENDER) continue;
-        std::string key((const char *)tft[i].data(), tft[i].size() * sizeof(float4));
+        std::string key((const char *)P[i].tft.data(), P[i].tft.size() * sizeof(float4));
         auto it = tf_slot_of.find(key);
         if (it == tf_slot_of.end()) {
-            const int base = (int)tf_all.size();
-            tf_all.insert(tf_all.end(), tft[i].begin(), tft[i].end());
-            it = tf_slot_of.emplace(key, base).first;
+            const int b0 = (int)tf_all.size();
+            tf_all.insert(tf_all.end(), P[i].tft.begin(), P[i].tft.end());
+            it = tf_slot_of.emplace(key, b0).first;
         }
         slot[i] = it->second;
     }
@@ -1319,18 +1428,17 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
     std::map<std::tuple<int, int, int, int>, std::vector<int>> gmap;
     for (int i = 0; i < n_req; i++) {
         const bool rec = (reqs[i].flags & YTRC_FLAG_RECORD) != 0;
-        gmap[std::make_tuple(reqs[i].kind, B->reqs[i].nfp, rec ? 1 : 0, B->reqs[i].mode)].push_back(i);
+        gmap[std::make_tuple(reqs[i].kind, P[i].q.nfp, rec ? 1 : 0, P[i].q.mode)].push_back(i);
     }
     std::vector<std::vector<int2>> gitems;
     std::vector<size_t> gitems_off;
     for (auto &kv : gmap) {
         std::vector<int> ids = kv.second;
-        std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) {
-            return B->reqs[a].ntiles > B->reqs[b].ntiles;
-        });
+        std::stable_sort(ids.begin(), ids.end(),
+                         [&](int a, int b) { return P[a].q.ntiles > P[b].q.ntiles; });
         std::vector<int2> items;
         for (int i : ids)
-            for (int t = 0; t < B->reqs[i].ntiles; t++) items.push_back(make_int2(i, t));
+            for (int t = 0; t < P[i].q.ntiles; t++) items.push_back(make_int2(i, t));
         PGroup g;
         g.kind = std::get<0>(kv.first);
         g.nfp = std::get<1>(kv.first);
@@ -1351,88 +1459,42 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
     off = align_up(off + sizeof(unsigned int) * std::max<size_t>(1, B->groups.size()), 256);
     const size_t imgs_off = off;
     off = align_up(off + sizeof(DImg) * n_req, 256);
-    B->arena_bytes = off;
     if (ctx->budget != 0 && ctx->used + off > ctx->budget)
-        return bad(YTRC_EBUDGET, "ytrc_batch_create: " + std::to_string(off) +
-                                     " bytes exceed the HBM budget");
+        return bad(YTRC_EBUDGET, "ytrc_batch_create: " + std::to_string(off) + " bytes exceed the HBM budget");
     hipError_t e = hipMalloc(&B->arena, off);
     if (e != hipSuccess) return bad(YTRC_EDEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+    B->arena_bytes = off;
     ctx->used += off;
     char *A = B->arena;
-    // Pass 2: fill descriptors and copy inputs.
     std::vector<float> stage;
     for (int i = 0; i < n_req; i++) {
-        const ytrc_request &r = reqs[i];
-        ReqHost &q = B->reqs[i];
-        const Offs &o = offs[i];
-        const Dataset &ds = ctx->datasets[r.ds_id];
-        const bool rec = (r.flags & YTRC_FLAG_RECORD) != 0;
-        DImg &d = B->h_imgs[i];
-        memset(&d, 0, sizeof(d));
-        memcpy(d.vp_pos, r.vp_pos, sizeof(d.vp_pos));
-        memcpy(d.vp_dir, r.vp_dir, sizeof(d.vp_dir));
-        memcpy(d.width, r.width, sizeof(d.width));
-        d.bricks = ds.d_bricks;
-        d.pool = ds.d_pool;
-        d.masks = ds.d_masks;
-        d.order = (const int32_t *)(A + o.order);
-        d.ext = (const int4 *)(A + o.ext);
-        d.zbuf = r.zbuffer ? (const double *)(A + o.zbuf) : nullptr;
-        d.init = r.image_in ? (const float *)(A + o.init) : nullptr;
-        d.out = (float *)(A + o.out);
-        d.rec_visits = rec ? (int64_t *)(A + o.rv) : nullptr;
-        d.rec_hash = rec ? (uint64_t *)(A + o.rh) : nullptr;
-        d.rec_samples = rec ? (int64_t *)(A + o.rs) : nullptr;
-        d.tf = (r.kind == YTRC_VOLUME_RENDER) ? (const DTF *)(A + o.tf) : nullptr;
-        d.nv0 = r.nv[0];
-        d.nv1 = r.nv[1];
-        d.kind = r.kind;
-        d.n_samples = r.n_samples;
-        d.n_order = r.n_order;
-        d.nf = ds.nf;
-        d.nfp = ds.nfp;
-        d.tiles0 = (r.nv[0] + TILE - 1) / TILE;
-        d.tiles1 = (r.nv[1] + TILE - 1) / TILE;
-        d.flags = (int32_t)r.flags;
-        d.nch_out = q.nch_out;
-        d.inv_ns = r.n_samples > 0 ? 1.0 / (double)r.n_samples : 0.0;
-        d.tform = q.tform;
-        if (r.n_order > 0) {
-            HIPCHK(hipMemcpy(A + o.order, r.order, sizeof(int32_t) * r.n_order, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(A + o.ext, exts[i].data(), sizeof(int4) * r.n_order, hipMemcpyHostToDevice));
-        }
-        if (r.zbuffer)
-            HIPCHK(hipMemcpy(A + o.zbuf, r.zbuffer, sizeof(double) * q.npix, hipMemcpyHostToDevice));
-        if (r.image_in) {
-            stage.resize((size_t)q.npix * 4);
-            for (int64_t p = 0; p < q.npix; p++)
-                for (int c = 0; c < 4; c++) stage[(size_t)c * q.npix + p] = (float)r.image_in[p * 4 + c];
-            HIPCHK(hipMemcpy(A + o.init, stage.data(), sizeof(float) * 4 * q.npix, hipMemcpyHostToDevice));
-        }
-        if (r.kind == YTRC_VOLUME_RENDER) {
-            tfs[i].table = (const float4 *)(A + tfall_off) + slot[i];
-            d.tf_slot = slot[i];
-            HIPCHK(hipMemcpy(A + o.tf, &tfs[i], sizeof(DTF), hipMemcpyHostToDevice));
-        }
+        const int rc = stage_request(ctx, P[i], A + base[i], (const float4 *)(A + tfall_off), slot[i],
+                                     B->h_imgs[i], stage);
+        if (rc != YTRC_OK) return bad(rc, g_err);
     }
     B->tf_all = (const float4 *)(A + tfall_off);
     B->tf_bins = (int)tf_all.size();
-    if (!tf_all.empty())
-        HIPCHK(hipMemcpy(A + tfall_off, tf_all.data(), sizeof(float4) * tf_all.size(), hipMemcpyHostToDevice));
+    if (!tf_all.empty()) {
+        e = hipMemcpy(A + tfall_off, tf_all.data(), sizeof(float4) * tf_all.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return bad(YTRC_EDEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    }
     for (size_t g = 0; g < B->groups.size(); g++) {
         B->groups[g].items = (const int2 *)(A + gitems_off[g]);
-        if (!gitems[g].empty())
-            HIPCHK(hipMemcpy(A + gitems_off[g], gitems[g].data(), sizeof(int2) * gitems[g].size(),
-                             hipMemcpyHostToDevice));
+        if (!gitems[g].empty()) {
+            e = hipMemcpy(A + gitems_off[g], gitems[g].data(), sizeof(int2) * gitems[g].size(),
+                          hipMemcpyHostToDevice);
+            if (e != hipSuccess) return bad(YTRC_EDEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+        }
     }
     B->counters = (unsigned int *)(A + counters_off);
     B->d_imgs = (DImg *)(A + imgs_off);
-    HIPCHK(hipMemcpy(B->d_imgs, B->h_imgs.data(), sizeof(DImg) * n_req, hipMemcpyHostToDevice));
+    e = hipMemcpy(B->d_imgs, B->h_imgs.data(), sizeof(DImg) * n_req, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bad(YTRC_EDEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
     HIPCHK(hipEventCreate(&B->ev0));
     HIPCHK(hipEventCreate(&B->ev1));
     if (!(batch_flags & YTRC_BATCH_PERSISTENT)) {
         B->ev_req.resize(2 * n_req);
-        for (auto &e : B->ev_req) HIPCHK(hipEventCreate(&e));
+        for (auto &ev : B->ev_req) HIPCHK(hipEventCreate(&ev));
     }
     *out = B;
     return YTRC_OK;
@@ -1569,21 +1631,7 @@ int ytrc_batch_read_image(ytrc_batch *b, int32_t i, double *image_out)
     if (i < 0 || i >= (int)b->reqs.size()) return fail(YTRC_EINVAL, "ytrc_batch_read_image: bad index");
     HIPCHK(hipSetDevice(b->ctx->device));
     HIPCHK(hipStreamSynchronize(b->ctx->stream));
-    const ReqHost &q = b->reqs[i];
-    const DImg &d = b->h_imgs[i];
-    std::vector<float> h((size_t)q.npix * q.nch_out);
-    HIPCHK(hipMemcpy(h.data(), d.out, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
-    for (int64_t p = 0; p < q.npix; p++) {
-        for (int c = 0; c < 4; c++) {
-            double v;
-            if (c < q.nch_out)
-                v = (double)h[(size_t)c * q.npix + p];
-            else
-                v = q.image_in.empty() ? 0.0 : q.image_in[(size_t)p * 4 + c];
-            image_out[(size_t)p * 4 + c] = v;
-        }
-    }
-    return YTRC_OK;
+    return read_request_image(b->reqs[i], b->h_imgs[i], image_out);
 }
 
 int ytrc_batch_read_record(ytrc_batch *b, int32_t i, int64_t *visits, uint64_t *hashes,
@@ -1660,6 +1708,537 @@ void ytrc_batch_destroy(ytrc_batch *b)
     if (b->gexec) hipGraphExecDestroy(b->gexec);
     if (b->graph) hipGraphDestroy(b->graph);
     delete b;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Serving: continuous batching of requests against resident datasets.
+//
+// Admitted requests get a descriptor slot and a region of a pre-allocated
+// device arena (the HBM budget), and their 8x8 ray tiles join a FIFO.  The
+// host keeps up to two chunks in flight; a chunk takes the next tiles of the
+// FIFO -- from every admitted request, in arrival order -- and runs them as
+// persistent tile-queue launches (one per kernel variant).  A request that
+// arrives while others are on the GPU joins at the next chunk boundary; it
+// is finished when the chunk holding its last tile completes.  Every kernel
+// terminates; nothing spins on host-written flags.
+
+namespace {
+
+struct ArenaAlloc {
+    // First-fit allocator over one device allocation (no hipMalloc/hipFree
+    // while serving).
+    char *base = nullptr;
+    size_t size = 0;
+    std::map<size_t, size_t> free_;  // offset -> length
+    void init(char *b, size_t n)
+    {
+        base = b;
+        size = n;
+        free_.clear();
+        free_[0] = n;
+    }
+    bool alloc(size_t n, size_t &off)
+    {
+        n = align_up(n, 256);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= n) {
+                off = it->first;
+                const size_t rest = it->second - n;
+                free_.erase(it);
+                if (rest) free_[off + n] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t n)
+    {
+        n = align_up(n, 256);
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+struct SrvReq {
+    int64_t id = 0;
+    PrepReq P;
+    DImg d;
+    size_t off = 0;
+    int slot = -1;
+    int variant = -1;
+    int next_tile = 0;
+    int64_t last_chunk = -1;
+    int state = 0;  // 0 tiles pending, 1 all tiles launched, 2 done, 3 cancelled
+    int tf_entry = -1;
+};
+
+struct TFEntry {
+    std::string key;
+    int base = 0;
+    int len = 0;
+    int refs = 0;
+};
+
+struct SrvChunk {
+    int64_t seq = -1;
+    hipEvent_t done = nullptr;
+    int2 *h_items = nullptr;  // pinned
+    int2 *d_items = nullptr;
+    unsigned int *d_counters = nullptr;
+    std::vector<int64_t> finishing;
+    bool busy = false;
+};
+
+constexpr int SRV_CHUNKS = 3;
+constexpr int SRV_TF_CAP = 3072;  // float4 entries of the server's table (48 KiB)
+
+}  // namespace
+
+struct ytrc_server {
+    ytrc_ctx *ctx = nullptr;
+    int max_slots = 0;
+    int chunk_tiles = 0;
+    int max_inflight = 2;
+    uint64_t budget = 0;
+    char *arena = nullptr;
+    ArenaAlloc alloc;
+    DImg *d_slots = nullptr;
+    std::vector<int> free_slots;
+    float4 *d_tf = nullptr;
+    std::vector<TFEntry> tfs;
+    int tf_hi = 0;
+    std::map<int64_t, SrvReq> reqs;
+    std::deque<int64_t> fifo;
+    std::vector<std::tuple<int, int, int, int>> vkeys;
+    std::vector<pfn_t> vfns;
+    SrvChunk chunks[SRV_CHUNKS];
+    std::deque<int> inflight;
+    int64_t next_seq = 0;
+    int64_t next_id = 1;
+    std::deque<int64_t> finished;
+    uint64_t used = 0;
+    int64_t tiles_launched = 0;
+    int64_t chunks_launched = 0;
+    std::mutex mu;
+};
+
+namespace {
+
+int srv_variant(ytrc_server *S, const PrepReq &P)
+{
+    const bool rec = (P.q.r.flags & YTRC_FLAG_RECORD) != 0;
+    auto key = std::make_tuple(P.q.r.kind, P.q.nfp, rec ? 1 : 0, P.q.mode);
+    for (size_t i = 0; i < S->vkeys.size(); i++)
+        if (S->vkeys[i] == key) return (int)i;
+    S->vkeys.push_back(key);
+    S->vfns.push_back(pick_persist(P.q.r.kind, P.q.nfp, rec, P.q.mode));
+    return (int)S->vkeys.size() - 1;
+}
+
+// Finds or places a transfer-function table in the server's device table.
+int srv_tf_acquire(ytrc_server *S, const std::vector<float4> &t, int &entry)
+{
+    std::string key((const char *)t.data(), t.size() * sizeof(float4));
+    for (size_t i = 0; i < S->tfs.size(); i++)
+        if (S->tfs[i].len > 0 && S->tfs[i].key == key) {
+            S->tfs[i].refs++;
+            entry = (int)i;
+            return YTRC_OK;
+        }
+    // first fit among gaps of the entries in use
+    std::vector<std::pair<int, int>> used;
+    for (auto &e : S->tfs)
+        if (e.refs > 0) used.push_back({e.base, e.len});
+    std::sort(used.begin(), used.end());
+    int pos = 0;
+    for (auto &u : used) {
+        if (u.first - pos >= (int)t.size()) break;
+        pos = std::max(pos, u.first + u.second);
+    }
+    if (pos + (int)t.size() > SRV_TF_CAP)
+        return fail(YTRC_EBUDGET, "server: no room for another transfer function in LDS");
+    HIPCHK(hipMemcpy(S->d_tf + pos, t.data(), t.size() * sizeof(float4), hipMemcpyHostToDevice));
+    int slot = -1;
+    for (size_t i = 0; i < S->tfs.size(); i++)
+        if (S->tfs[i].refs == 0) slot = (int)i;
+    if (slot < 0) {
+        S->tfs.push_back(TFEntry());
+        slot = (int)S->tfs.size() - 1;
+    }
+    S->tfs[slot].key = key;
+    S->tfs[slot].base = pos;
+    S->tfs[slot].len = (int)t.size();
+    S->tfs[slot].refs = 1;
+    S->tf_hi = 0;
+    for (auto &e : S->tfs)
+        if (e.refs > 0) S->tf_hi = std::max(S->tf_hi, e.base + e.len);
+    entry = slot;
+    return YTRC_OK;
+}
+
+void srv_release(ytrc_server *S, SrvReq &R)
+{
+    if (R.off != (size_t)-1 && R.P.bytes) {
+        S->alloc.release(R.off, R.P.bytes);
+        S->used -= align_up(R.P.bytes, 256);
+        R.off = (size_t)-1;
+    }
+    if (R.slot >= 0) {
+        S->free_slots.push_back(R.slot);
+        R.slot = -1;
+    }
+    if (R.tf_entry >= 0) {
+        S->tfs[R.tf_entry].refs--;
+        R.tf_entry = -1;
+    }
+}
+
+// Launches chunks while fewer than max_inflight are on the GPU.
+int srv_pump(ytrc_server *S)
+{
+    hipStream_t s = S->ctx->stream;
+    while ((int)S->inflight.size() < S->max_inflight && !S->fifo.empty()) {
+        int ci = -1;
+        for (int i = 0; i < SRV_CHUNKS; i++)
+            if (!S->chunks[i].busy) {
+                ci = i;
+                break;
+            }
+        if (ci < 0) break;
+        SrvChunk &C = S->chunks[ci];
+        C.finishing.clear();
+        const int nv = (int)S->vkeys.size();
+        std::vector<std::vector<int2>> per(nv);
+        int filled = 0;
+        while (!S->fifo.empty() && filled < S->chunk_tiles) {
+            SrvReq &R = S->reqs[S->fifo.front()];
+            const int take = std::min(R.P.q.ntiles - R.next_tile, S->chunk_tiles - filled);
+            for (int t = 0; t < take; t++) per[R.variant].push_back(make_int2(R.slot, R.next_tile + t));
+            R.next_tile += take;
+            filled += take;
+            R.last_chunk = S->next_seq;
+            if (R.next_tile >= R.P.q.ntiles) {
+                R.state = 1;
+                C.finishing.push_back(R.id);
+                S->fifo.pop_front();
+            }
+        }
+        std::vector<int> voff(nv, 0), vcnt(nv, 0);
+        int n = 0;
+        for (int v = 0; v < nv; v++) {
+            voff[v] = n;
+            vcnt[v] = (int)per[v].size();
+            for (auto &it : per[v]) C.h_items[n++] = it;
+        }
+        HIPCHK(hipMemcpyAsync(C.d_items, C.h_items, sizeof(int2) * n, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, C.d_counters, 64);
+        HIPCHK(hipGetLastError());
+        for (int v = 0; v < nv; v++) {
+            if (vcnt[v] == 0) continue;
+            const bool vr = std::get<0>(S->vkeys[v]) == YTRC_VOLUME_RENDER;
+            const size_t lds = vr ? sizeof(float4) * S->tf_hi : 0;
+            int nb = 0;
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, S->vfns[v], 256, lds));
+            const int grid = std::max(1, std::min(std::max(1, nb) * S->ctx->n_cu, (vcnt[v] + 3) / 4));
+            hipLaunchKernelGGL(S->vfns[v], dim3(grid), dim3(256), lds, s, (const DImg *)S->d_slots,
+                               (const int2 *)(C.d_items + voff[v]), vcnt[v], C.d_counters + v,
+                               (const float4 *)S->d_tf, vr ? S->tf_hi : 0);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipEventRecord(C.done, s));
+        C.seq = S->next_seq++;
+        C.busy = true;
+        S->inflight.push_back(ci);
+        S->tiles_launched += n;
+        S->chunks_launched += 1;
+    }
+    return YTRC_OK;
+}
+
+// Retires completed chunks; requests whose last tile ran in them finish.
+int srv_poll(ytrc_server *S, bool block)
+{
+    while (!S->inflight.empty()) {
+        SrvChunk &C = S->chunks[S->inflight.front()];
+        hipError_t e = block ? hipEventSynchronize(C.done) : hipEventQuery(C.done);
+        if (e == hipErrorNotReady) break;
+        if (e != hipSuccess) return fail(YTRC_EDEVICE, std::string("server chunk: ") + hipGetErrorString(e));
+        for (int64_t id : C.finishing) {
+            auto it = S->reqs.find(id);
+            if (it == S->reqs.end()) continue;
+            if (it->second.state == 3) {
+                srv_release(S, it->second);
+                S->reqs.erase(it);
+            } else {
+                it->second.state = 2;
+                S->finished.push_back(id);
+            }
+        }
+        C.busy = false;
+        S->inflight.pop_front();
+        if (block) break;
+    }
+    return YTRC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ytrc_server_create(ytrc_ctx *ctx, int32_t max_requests, uint64_t budget_bytes,
+                       int32_t chunk_tiles, ytrc_server **out)
+{
+    if (ctx == nullptr || out == nullptr) return fail(YTRC_EINVAL, "ytrc_server_create: NULL argument");
+    if (max_requests <= 0 || max_requests > 4096)
+        return fail(YTRC_EINVAL, "ytrc_server_create: max_requests must be in 1..4096");
+    if (chunk_tiles < 64 || chunk_tiles > (1 << 22))
+        return fail(YTRC_EINVAL, "ytrc_server_create: chunk_tiles must be in 64..4194304");
+    if (budget_bytes < (1u << 20)) return fail(YTRC_EINVAL, "ytrc_server_create: budget below 1 MiB");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    if (ctx->budget != 0 && ctx->used + budget_bytes > ctx->budget)
+        return fail(YTRC_EBUDGET, "ytrc_server_create: server budget exceeds the context budget");
+    ytrc_server *S = new ytrc_server();
+    S->ctx = ctx;
+    S->max_slots = max_requests;
+    S->chunk_tiles = chunk_tiles;
+    S->budget = budget_bytes;
+    auto bad = [&](hipError_t e, const char *what) {
+        if (S->arena) hipFree(S->arena);
+        if (S->d_slots) hipFree(S->d_slots);
+        if (S->d_tf) hipFree(S->d_tf);
+        for (auto &c : S->chunks) {
+            if (c.h_items) hipHostFree(c.h_items);
+            if (c.d_items) hipFree(c.d_items);
+            if (c.d_counters) hipFree(c.d_counters);
+            if (c.done) hipEventDestroy(c.done);
+        }
+        delete S;
+        return fail(YTRC_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    hipError_t e = hipMalloc(&S->arena, budget_bytes);
+    if (e != hipSuccess) return bad(e, "hipMalloc(arena)");
+    S->alloc.init(S->arena, budget_bytes);
+    e = hipMalloc(&S->d_slots, sizeof(DImg) * max_requests);
+    if (e != hipSuccess) return bad(e, "hipMalloc(slots)");
+    e = hipMalloc(&S->d_tf, sizeof(float4) * SRV_TF_CAP);
+    if (e != hipSuccess) return bad(e, "hipMalloc(tf)");
+    for (auto &c : S->chunks) {
+        e = hipHostMalloc(&c.h_items, sizeof(int2) * chunk_tiles, hipHostMallocDefault);
+        if (e != hipSuccess) return bad(e, "hipHostMalloc(items)");
+        e = hipMalloc(&c.d_items, sizeof(int2) * chunk_tiles);
+        if (e != hipSuccess) return bad(e, "hipMalloc(items)");
+        e = hipMalloc(&c.d_counters, sizeof(unsigned int) * 64);
+        if (e != hipSuccess) return bad(e, "hipMalloc(counters)");
+        e = hipEventCreateWithFlags(&c.done, hipEventDisableTiming);
+        if (e != hipSuccess) return bad(e, "hipEventCreate");
+    }
+    for (int i = max_requests - 1; i >= 0; i--) S->free_slots.push_back(i);
+    ctx->used += budget_bytes;
+    *out = S;
+    return YTRC_OK;
+}
+
+int ytrc_server_submit(ytrc_server *S, const ytrc_request *r, int64_t *id_out)
+{
+    if (S == nullptr || r == nullptr || id_out == nullptr) return fail(YTRC_EINVAL, "ytrc_server_submit: NULL");
+    if (r->flags & YTRC_FLAG_RECORD)
+        return fail(YTRC_EINVAL, "ytrc_server_submit: record mode is for batches");
+    std::lock_guard<std::mutex> lk(S->mu);
+    HIPCHK(hipSetDevice(S->ctx->device));
+    SrvReq R;
+    {
+        std::lock_guard<std::mutex> lc(S->ctx->mu);
+        const int rc = prep_request(S->ctx, *r, "request: ", R.P);
+        if (rc != YTRC_OK) return rc;
+    }
+    // Admission: a descriptor slot and arena room for the request's region.
+    if (S->free_slots.empty()) return fail(YTRC_EBUDGET, "server: all request slots are in use");
+    size_t off = 0;
+    if (!S->alloc.alloc(R.P.bytes, off))
+        return fail(YTRC_EBUDGET, "server: request needs " + std::to_string(R.P.bytes) +
+                                      " bytes, more than the free HBM budget");
+    R.off = off;
+    S->used += align_up(R.P.bytes, 256);
+    int tf_base = 0;
+    if (r->kind == YTRC_VOLUME_RENDER) {
+        const int rc = srv_tf_acquire(S, R.P.tft, R.tf_entry);
+        if (rc != YTRC_OK) {
+            srv_release(S, R);
+            return rc;
+        }
+        tf_base = S->tfs[R.tf_entry].base;
+    }
+    R.slot = S->free_slots.back();
+    S->free_slots.pop_back();
+    std::vector<float> stage;
+    int rc = stage_request(S->ctx, R.P, S->arena + off, S->d_tf, tf_base, R.d, stage);
+    if (rc == YTRC_OK) {
+        hipError_t e = hipMemcpy(S->d_slots + R.slot, &R.d, sizeof(DImg), hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = fail(YTRC_EDEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    }
+    if (rc != YTRC_OK) {
+        srv_release(S, R);
+        return rc;
+    }
+    R.variant = srv_variant(S, R.P);
+    R.id = S->next_id++;
+    R.P.q.r.order = nullptr;  // caller-owned pointers are not kept
+    R.P.q.r.zbuffer = nullptr;
+    R.P.q.r.image_in = nullptr;
+    R.P.q.r.tf = nullptr;
+    *id_out = R.id;
+    const int64_t id = R.id;
+    S->reqs.emplace(id, std::move(R));
+    S->fifo.push_back(id);
+    return srv_pump(S);
+}
+
+int ytrc_server_progress(ytrc_server *S, int64_t *done_ids, int32_t cap, int32_t *n_done)
+{
+    if (S == nullptr || n_done == nullptr) return fail(YTRC_EINVAL, "ytrc_server_progress: NULL");
+    std::lock_guard<std::mutex> lk(S->mu);
+    HIPCHK(hipSetDevice(S->ctx->device));
+    int rc = srv_poll(S, false);
+    if (rc != YTRC_OK) return rc;
+    rc = srv_pump(S);
+    if (rc != YTRC_OK) return rc;
+    int n = 0;
+    while (!S->finished.empty() && n < cap) {
+        if (done_ids) done_ids[n] = S->finished.front();
+        S->finished.pop_front();
+        n++;
+    }
+    *n_done = n;
+    return YTRC_OK;
+}
+
+int ytrc_server_wait(ytrc_server *S, int64_t *done_ids, int32_t cap, int32_t *n_done)
+{
+    if (S == nullptr || n_done == nullptr) return fail(YTRC_EINVAL, "ytrc_server_wait: NULL");
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        HIPCHK(hipSetDevice(S->ctx->device));
+        if (S->finished.empty() && !S->inflight.empty()) {
+            const int rc = srv_poll(S, true);
+            if (rc != YTRC_OK) return rc;
+        }
+    }
+    return ytrc_server_progress(S, done_ids, cap, n_done);
+}
+
+int ytrc_server_read(ytrc_server *S, int64_t id, double *image_out)
+{
+    if (S == nullptr || image_out == nullptr) return fail(YTRC_EINVAL, "ytrc_server_read: NULL");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return fail(YTRC_EINVAL, "ytrc_server_read: unknown request");
+    if (it->second.state != 2) return fail(YTRC_EINVAL, "ytrc_server_read: request is not finished");
+    HIPCHK(hipSetDevice(S->ctx->device));
+    const int rc = read_request_image(it->second.P.q, it->second.d, image_out);
+    srv_release(S, it->second);
+    S->reqs.erase(it);
+    return rc;
+}
+
+int ytrc_server_read_planar(ytrc_server *S, int64_t id, float *planes, int32_t cap_ch, int32_t *nch)
+{
+    if (S == nullptr || planes == nullptr || nch == nullptr)
+        return fail(YTRC_EINVAL, "ytrc_server_read_planar: NULL");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return fail(YTRC_EINVAL, "ytrc_server_read_planar: unknown request");
+    if (it->second.state != 2) return fail(YTRC_EINVAL, "ytrc_server_read_planar: request is not finished");
+    const ReqHost &q = it->second.P.q;
+    if (cap_ch < q.nch_out) return fail(YTRC_EINVAL, "ytrc_server_read_planar: output has too few channels");
+    HIPCHK(hipSetDevice(S->ctx->device));
+    HIPCHK(hipMemcpy(planes, it->second.d.out, sizeof(float) * (size_t)q.npix * q.nch_out,
+                     hipMemcpyDeviceToHost));
+    *nch = q.nch_out;
+    srv_release(S, it->second);
+    S->reqs.erase(it);
+    return YTRC_OK;
+}
+
+int ytrc_server_cancel(ytrc_server *S, int64_t id)
+{
+    if (S == nullptr) return fail(YTRC_EINVAL, "ytrc_server_cancel: NULL");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return fail(YTRC_EINVAL, "ytrc_server_cancel: unknown request");
+    SrvReq &R = it->second;
+    if (R.state == 0) {
+        S->fifo.erase(std::remove(S->fifo.begin(), S->fifo.end(), id), S->fifo.end());
+        // Tiles already launched may still be running: release with the
+        // chunk that holds its last launched tile, or now if none ran.
+        bool running = false;
+        for (int ci : S->inflight)
+            if (S->chunks[ci].seq == R.last_chunk) {
+                S->chunks[ci].finishing.push_back(id);
+                running = true;
+            }
+        if (!running) {
+            srv_release(S, R);
+            S->reqs.erase(it);
+            return YTRC_OK;
+        }
+        R.state = 3;
+    } else if (R.state == 1) {
+        R.state = 3;  // released when its last chunk retires
+    } else if (R.state == 2) {
+        S->finished.erase(std::remove(S->finished.begin(), S->finished.end(), id), S->finished.end());
+        srv_release(S, R);
+        S->reqs.erase(it);
+    }
+    return YTRC_OK;
+}
+
+int ytrc_server_stats(const ytrc_server *S, int64_t *stats)
+{
+    if (S == nullptr || stats == nullptr) return fail(YTRC_EINVAL, "ytrc_server_stats: NULL");
+    stats[0] = (int64_t)S->reqs.size();
+    stats[1] = (int64_t)S->fifo.size();
+    stats[2] = (int64_t)S->inflight.size();
+    stats[3] = (int64_t)S->used;
+    stats[4] = (int64_t)S->budget;
+    stats[5] = S->tiles_launched;
+    stats[6] = S->chunks_launched;
+    stats[7] = (int64_t)S->free_slots.size();
+    return YTRC_OK;
+}
+
+void ytrc_server_destroy(ytrc_server *S)
+{
+    if (S == nullptr) return;
+    hipSetDevice(S->ctx->device);
+    hipStreamSynchronize(S->ctx->stream);
+    hipFree(S->arena);
+    hipFree(S->d_slots);
+    hipFree(S->d_tf);
+    for (auto &c : S->chunks) {
+        hipHostFree(c.h_items);
+        hipFree(c.d_items);
+        hipFree(c.d_counters);
+        hipEventDestroy(c.done);
+    }
+    {
+        std::lock_guard<std::mutex> lk(S->ctx->mu);
+        S->ctx->used -= S->budget;
+    }
+    delete S;
 }
 
 }  // extern "C"

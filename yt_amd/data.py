@@ -151,14 +151,18 @@ class GridDataset:
             g.child_mask[...] = True
             g.Children = []
         for lvl, gs in by_level.items():
-            for c in by_level.get(lvl + 1, []):
-                for p in gs:
-                    lo = np.maximum(p.LeftEdge, c.LeftEdge)
-                    hi = np.minimum(p.RightEdge, c.RightEdge)
-                    if np.any(hi <= lo):
-                        continue
-                    i0 = np.rint((lo - p.LeftEdge) / p.dds).astype(np.int64)
-                    i1 = np.rint((hi - p.LeftEdge) / p.dds).astype(np.int64)
+            kids = by_level.get(lvl + 1, [])
+            if not kids:
+                continue
+            ple = np.array([p.LeftEdge for p in gs])
+            pre = np.array([p.RightEdge for p in gs])
+            for c in kids:
+                lo = np.maximum(ple, c.LeftEdge)
+                hi = np.minimum(pre, c.RightEdge)
+                for pi in np.nonzero(np.all(hi > lo, axis=1))[0]:
+                    p = gs[pi]
+                    i0 = np.rint((lo[pi] - p.LeftEdge) / p.dds).astype(np.int64)
+                    i1 = np.rint((hi[pi] - p.LeftEdge) / p.dds).astype(np.int64)
                     p.child_mask[i0[0]:i1[0], i0[1]:i1[1], i0[2]:i1[2]] = False
                     p.Children.append(c)
                     c.Parent = p
@@ -305,51 +309,65 @@ def _trilinear_prolong(parent_block, r=2):
 
 
 def make_amr_hierarchy(root_n=128, levels=3, seed_density=1001, seed_flag=1003,
-                       percentile=90.0, block=16, max_block=32, sigma_level=0.1):
-    """The seeded AMR workload: root_n^3 lognormal root grid; at each level,
-    cells above the given percentile of the parent level are flagged and
-    covered by block^3-cell patches (merged along x up to max_block), refined
-    by 2 with trilinear prolongation times a seeded lognormal perturbation."""
+                       percentile=90.0, block=16, max_block=32, sigma_level=0.1,
+                       blocks_per_level=(330, 280, 230)):
+    """The seeded AMR workload: a root_n^3 lognormal root grid (seed
+    seed_density) and `levels` levels of refinement by 2.  At each level the
+    parent cells above the given percentile are flagged; the (block/2)^3
+    parent-cell blocks with the most flagged cells (up to blocks_per_level[l]
+    of them, ties broken by a seeded shuffle) are refined into block^3 child
+    grids, and adjacent selected blocks along the last axis are merged into
+    grids up to max_block cells long.  Child data is the trilinear
+    prolongation of the parent times a seeded lognormal perturbation
+    (sigma_level).  With the defaults this makes about 600 grids."""
     rng = np.random.default_rng(seed_flag)
     rho = lognormal_density(root_n, seed=seed_density)
     grid_data = [dict(left_edge=np.zeros(3), right_edge=np.ones(3), level=0,
                       dimensions=np.array([root_n] * 3), density=rho)]
     parents = [grid_data[0]]
-    thresh = np.percentile(rho, percentile)
+    pb = block // 2  # parent cells per child block edge
     for lvl in range(1, levels + 1):
-        children = []
-        for p in parents:
-            pd = p["density"]
-            dims = np.asarray(p["dimensions"])
-            pdds = (np.asarray(p["right_edge"]) - np.asarray(p["left_edge"])) / dims
-            pb = block // 2  # parent cells per child block edge
-            nb = dims // pb
-            flags = (pd > thresh).reshape(nb[0], pb, nb[1], pb, nb[2], pb).any(axis=(1, 3, 5))
-            for bi in range(nb[0]):
-                bj_list = np.nonzero(flags[bi].any(axis=1))[0]
-                for bj in bj_list:
-                    ks = np.nonzero(flags[bi, bj])[0]
-                    # merge runs along the last axis up to max_block child cells
-                    runs = []
-                    for k in ks:
-                        if runs and k == runs[-1][1] and (runs[-1][1] - runs[-1][0]) * block < max_block:
-                            runs[-1][1] = k + 1
-                        else:
-                            runs.append([k, k + 1])
-                    for k0, k1 in runs:
-                        lo = np.array([bi * pb, bj * pb, k0 * pb])
-                        hi = np.array([(bi + 1) * pb, (bj + 1) * pb, k1 * pb])
-                        sub = pd[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
-                        fine = _trilinear_prolong(sub, 2)
-                        fine = fine * np.exp(sigma_level * rng.standard_normal(fine.shape))
-                        le = np.asarray(p["left_edge"]) + lo * pdds
-                        re = np.asarray(p["left_edge"]) + hi * pdds
-                        children.append(dict(left_edge=le, right_edge=re, level=lvl,
-                                             dimensions=np.array(fine.shape), density=fine))
-        if not children:
-            break
-        allv = np.concatenate([c["density"].ravel() for c in children])
+        allv = np.concatenate([p["density"].ravel() for p in parents])
         thresh = np.percentile(allv, percentile)
+        cands = []  # (count, tiebreak, parent index, bi, bj, bk)
+        for pi, p in enumerate(parents):
+            dims = np.asarray(p["dimensions"])
+            nb = dims // pb
+            if np.any(nb == 0):
+                continue
+            sub = p["density"][:nb[0] * pb, :nb[1] * pb, :nb[2] * pb]
+            cnt = (sub > thresh).reshape(nb[0], pb, nb[1], pb, nb[2], pb).sum(axis=(1, 3, 5))
+            for (bi, bj, bk), c in np.ndenumerate(cnt):
+                if c > 0:
+                    cands.append((int(c), rng.random(), pi, bi, bj, bk))
+        if not cands:
+            break
+        cands.sort(key=lambda x: (-x[0], x[1]))
+        nsel = blocks_per_level[min(lvl - 1, len(blocks_per_level) - 1)]
+        chosen = {}
+        for c in cands[:nsel]:
+            chosen.setdefault((c[2], c[3], c[4]), []).append(c[5])
+        children = []
+        for (pi, bi, bj), ks in sorted(chosen.items()):
+            p = parents[pi]
+            pd = p["density"]
+            pdds = (np.asarray(p["right_edge"]) - np.asarray(p["left_edge"])) / np.asarray(p["dimensions"])
+            runs = []
+            for k in sorted(ks):
+                if runs and k == runs[-1][1] and (runs[-1][1] + 1 - runs[-1][0]) * block <= max_block:
+                    runs[-1][1] = k + 1
+                else:
+                    runs.append([k, k + 1])
+            for k0, k1 in runs:
+                lo = np.array([bi * pb, bj * pb, k0 * pb])
+                hi = np.array([(bi + 1) * pb, (bj + 1) * pb, k1 * pb])
+                sub = pd[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+                fine = _trilinear_prolong(sub, 2)
+                fine = fine * np.exp(sigma_level * rng.standard_normal(fine.shape))
+                le = np.asarray(p["left_edge"]) + lo * pdds
+                re = np.asarray(p["left_edge"]) + hi * pdds
+                children.append(dict(left_edge=le, right_edge=re, level=lvl,
+                                     dimensions=np.array(fine.shape), density=fine))
         grid_data.extend(children)
         parents = children
     return grid_data

@@ -1,0 +1,95 @@
+"""Serving loop: Poisson request arrivals against resident volumes.
+
+Requests (a camera, an image size, projection or volume render) arrive at
+seeded Poisson times.  Each is offered to the continuous-batching server
+(ytrc_server_*) when it arrives; a request the server cannot admit yet (no
+slot or HBM budget) waits in arrival order and is offered again as finished
+requests are read back and release their memory.  Latency is measured from
+arrival to the image being in host memory.
+"""
+
+import time
+
+import numpy as np
+
+from . import _lib
+from .render import camera_params
+
+
+def make_requests(volumes, n, seed=1006, side_range=(256, 2048), vr_fraction=0.5,
+                  tfs=None, rng_cam_seed=1004):
+    """n request specs spread over the resident volumes: random view
+    direction and north, side uniform in side_range, projection or VR."""
+    rng = np.random.default_rng(rng_cam_seed)
+    rs = np.random.default_rng(seed)
+    specs = []
+    for i in range(n):
+        vol = volumes[i % len(volumes)]
+        nrm = rng.normal(size=3)
+        nrm /= np.linalg.norm(nrm)
+        side = int(rs.integers(side_range[0], side_range[1] + 1))
+        p = camera_params(vol.ds.domain_center, nrm, float(vol.ds.domain_width.max()), side,
+                          rng.normal(size=3))
+        vr = (vol.vr_ds_id is not None) and (vol.proj_ds_id is None or rs.random() < vr_fraction)
+        if vr:
+            specs.append(vol.vr_request(p, tfs[id(vol)] if tfs else None))
+        else:
+            specs.append(vol.projection_request(p))
+    return specs
+
+
+def run_poisson(server, specs, rate, seed=1005, readback="planar", idle_sleep=2e-4):
+    """Offers specs[i] at Poisson arrival times of the given rate (req/s).
+    Returns latencies (s), wall time and the number of admission waits."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(specs)))
+    lat = np.full(len(specs), np.nan)
+    id_to_idx = {}
+    waiting = []
+    nxt = 0
+    done = 0
+    waits = 0
+    t0 = time.perf_counter()
+    while done < len(specs):
+        now = time.perf_counter() - t0
+        while nxt < len(specs) and arrivals[nxt] <= now:
+            waiting.append(nxt)
+            nxt += 1
+        while waiting:
+            i = waiting[0]
+            rid = server.submit(specs[i])
+            if rid is None:
+                waits += 1
+                break
+            id_to_idx[rid] = i
+            waiting.pop(0)
+        fin = server.progress()
+        for rid in fin:
+            i = id_to_idx.pop(rid)
+            if readback == "planar":
+                server.read_planar(rid)
+            else:
+                server.read(rid)
+            lat[i] = time.perf_counter() - t0 - arrivals[i]
+            done += 1
+        if not fin and (not waiting or nxt < len(specs)):
+            if server.stats()["chunks_inflight"] == 0 and not waiting:
+                time.sleep(min(idle_sleep, max(0.0, arrivals[nxt] - now) if nxt < len(specs) else idle_sleep))
+            else:
+                time.sleep(idle_sleep / 4)
+    wall = time.perf_counter() - t0
+    return dict(latency_s=lat, wall_s=wall, admission_waits=waits,
+                first_arrival=float(arrivals[0]), last_arrival=float(arrivals[-1]))
+
+
+def summarize(res, specs):
+    lat = res["latency_s"] * 1e3
+    span = res["wall_s"] - res["first_arrival"]
+    return dict(
+        n=len(specs),
+        p50_ms=float(np.percentile(lat, 50)),
+        p99_ms=float(np.percentile(lat, 99)),
+        mean_ms=float(np.mean(lat)),
+        images_per_s=len(specs) / span,
+        admission_waits=int(res["admission_waits"]),
+    )
