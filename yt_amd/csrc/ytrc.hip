@@ -380,9 +380,11 @@ struct VRSampler {
     const float4 *lt;
     int nb;
     int tform;
+    static constexpr bool ert = (MODE & 4) != 0;  // also track the brick's transmittance tb
     double inv_ns;
     int n_samples;
     double st[4];
+    double tb[4];
     Rec *rec;
 
     __device__ __forceinline__ void eval(float v0, const float (&dvs)[NFP], float e[4]) const
@@ -431,6 +433,7 @@ struct VRSampler {
                 const double ta = y_fmax(1.0 - dt * (double)e[3], 0.0);
 #pragma unroll
                 for (int ch = 0; ch < 4; ch++) st[ch] = __builtin_fma(ta, st[ch], dt * (double)e[ch]);
+                if constexpr (ert) tb[0] *= ta;
             });
         } else {
             each_sample<NS>(ns, [&](int s) {
@@ -442,6 +445,7 @@ struct VRSampler {
                     const double a = dt * (double)e[ch];
                     const double ta = y_fmax(1.0 - a, 0.0);
                     st[ch] = __builtin_fma(ta, st[ch], a);
+                    if constexpr (ert) tb[ch] *= ta;
                 }
             });
         }
@@ -562,55 +566,104 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
         for (int c = 0; c < 3; c++) st[c] = 1.0 - st[c];
     }
 
-    for (int k = 0; k < im.n_order; k++) {
+    // Early ray termination (YTRC_FLAG_ERT, VR only): the reference composites
+    // each new sample on top as t grows (field_interpolation_tables.pxd:107-110),
+    // so its eye is at t -> +inf.  Front to back is therefore the reverse brick
+    // order; each brick's cells are still walked forward (the same DDA) to form
+    // its per-channel affine map rgba -> C_b + T_b rgba, maps are composed as
+    // C += T C_b, T *= T_b, and a wave stops when every lane has T < ERT_EPS.
+    constexpr double ERT_EPS = 1e-4;
+    constexpr bool ert = (KIND == 1) && ((MODE & 4) != 0);
+    double rgba0[4], Cacc[4], Tacc[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        rgba0[c] = st[c];
+        Cacc[c] = 0.0;
+        Tacc[c] = 1.0;
+    }
+    if (ert && tform) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) rgba0[c] = 1.0 - st[c];
+    }
+
+    for (int kk = 0; kk < im.n_order; kk++) {
+        const int k = ert ? im.n_order - 1 - kk : kk;
         const int4 e = im.ext[k];
         if (e.y <= ti0 || e.x >= ti0 + TILE || e.w <= tj0 || e.z >= tj0 + TILE) continue;
-        if (!(active && vi >= e.x && vi < e.y && vj >= e.z && vj < e.w)) continue;
+        bool go = active && vi >= e.x && vi < e.y && vj >= e.z && vj < e.w;
         const int bid = im.order[k];
         const DBrick &b = im.bricks[bid];
         Walk w;
-        if (!walk_init(b, pos, dir, iv, max_t, w)) continue;
-        rec.brick = bid;
-        if (KIND == 0) {
-            ProjSampler<NFP, REC> s;
-            s.data = im.pool + b.data_off;
-            s.d1 = b.dims[1];
-            s.d2 = b.dims[2];
-            s.dims0 = b.dims[0];
+        if (go) go = walk_init(b, pos, dir, iv, max_t, w);
+        if (go) {
+            rec.brick = bid;
+            if (KIND == 0) {
+                ProjSampler<NFP, REC> s;
+                s.data = im.pool + b.data_off;
+                s.d1 = b.dims[1];
+                s.d2 = b.dims[2];
+                s.dims0 = b.dims[0];
 #pragma unroll
-            for (int f = 0; f < NFP; f++) {
-                s.s[f] = ps[f];
-                s.c[f] = pc[f];
+                for (int f = 0; f < NFP; f++) {
+                    s.s[f] = ps[f];
+                    s.c[f] = pc[f];
+                }
+                s.rec = &rec;
+                walk_projection(b, w, max_t, s);
+#pragma unroll
+                for (int f = 0; f < NFP; f++) {
+                    ps[f] = s.s[f];
+                    pc[f] = s.c[f];
+                }
+            } else {
+                VRSampler<NFP, REC, MODE> s;
+                s.data = im.pool + b.data_off;
+                s.mask = (b.mask_off >= 0) ? im.masks + b.mask_off : nullptr;
+                s.b = brick_reg(b);
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    s.pa[i] = uni_d(dir[i] * b.idds[i]);
+                    s.pb[i] = (pos[i] - b.le[i]) * b.idds[i];
+                }
+                s.tf = &tfc;
+                s.lt = lt;
+                s.nb = nb;
+                s.tform = tform;
+                s.inv_ns = im.inv_ns;
+                s.n_samples = im.n_samples;
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    s.st[c] = ert ? (tform ? 1.0 : 0.0) : st[c];
+                    s.tb[c] = 1.0;
+                }
+                s.rec = &rec;
+                walk_loop(b, w, max_t, s);
+                if constexpr (ert) {
+                    double cb[4], tb[4];
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        if (tform) {
+                            cb[c] = (c < 3) ? 1.0 - s.st[c] : 0.0;
+                            tb[c] = (c < 3) ? s.st[c] : 1.0;
+                        } else if (tfc.grey) {
+                            cb[c] = s.st[c];
+                            tb[c] = s.tb[0];
+                        } else {
+                            cb[c] = (c < 3) ? s.st[c] : 0.0;
+                            tb[c] = (c < 3) ? s.tb[c] : 1.0;
+                        }
+                        Cacc[c] = __builtin_fma(Tacc[c], cb[c], Cacc[c]);
+                        Tacc[c] *= tb[c];
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 4; c++) st[c] = s.st[c];
+                }
             }
-            s.rec = &rec;
-            walk_projection(b, w, max_t, s);
-#pragma unroll
-            for (int f = 0; f < NFP; f++) {
-                ps[f] = s.s[f];
-                pc[f] = s.c[f];
-            }
-        } else {
-            VRSampler<NFP, REC, MODE> s;
-            s.data = im.pool + b.data_off;
-            s.mask = (b.mask_off >= 0) ? im.masks + b.mask_off : nullptr;
-            s.b = brick_reg(b);
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                s.pa[i] = uni_d(dir[i] * b.idds[i]);
-                s.pb[i] = (pos[i] - b.le[i]) * b.idds[i];
-            }
-            s.tf = &tfc;
-            s.lt = lt;
-            s.nb = nb;
-            s.tform = tform;
-            s.inv_ns = im.inv_ns;
-            s.n_samples = im.n_samples;
-#pragma unroll
-            for (int c = 0; c < 4; c++) s.st[c] = st[c];
-            s.rec = &rec;
-            walk_loop(b, w, max_t, s);
-#pragma unroll
-            for (int c = 0; c < 4; c++) st[c] = s.st[c];
+        }
+        if constexpr (ert) {
+            const double tm = y_fmax(y_fmax(Tacc[0], Tacc[1]), y_fmax(Tacc[2], tfc.grey ? Tacc[3] : 0.0));
+            if (__all(!active || tm < ERT_EPS)) break;
         }
     }
     if (!active) return;
@@ -618,6 +671,10 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
 #pragma unroll
         for (int f = 0; f < NFP; f++)
             if (f < im.nch_out) im.out[f * npix + pix] = ps[f] - pc[f];
+    } else if constexpr (ert) {
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            if (c < im.nch_out) im.out[c * npix + pix] = (float)__builtin_fma(Tacc[c], rgba0[c], Cacc[c]);
     } else {
         if (tform) {
 #pragma unroll
@@ -656,8 +713,17 @@ __global__ __launch_bounds__(256) void k_render(const DImg *__restrict__ imgs, i
 // tables of the batch sit in LDS; each image reads its own slot.  The item
 // index is made wave-uniform so that image, brick and table descriptors are
 // read with scalar loads.
+// Minimum waves per SIMD requested from the register allocator: the
+// production volume-render variants (one field, ColorTransferFunction path,
+// no record/ERT) fit 128 VGPRs, i.e. 4 waves per SIMD, without spilling.
 template <int KIND, int NFP, bool REC, int MODE>
-__global__ __launch_bounds__(256) void k_persist(const DImg *__restrict__ imgs,
+struct MinWaves {
+    static constexpr int value = (KIND == 1 && NFP == 1 && !REC && (MODE == 1 || MODE == 3)) ? 4 : 1;
+};
+
+template <int KIND, int NFP, bool REC, int MODE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves<KIND, NFP, REC, MODE>::value)))
+void k_persist(const DImg *__restrict__ imgs,
                                                  const int2 *__restrict__ items, int n_items,
                                                  unsigned int *__restrict__ counter,
                                                  const float4 *__restrict__ tf_all, int tf_bins)
@@ -705,8 +771,8 @@ pfn_t pick_persist_nfp(int nfp)
     switch (nfp) {
     case 1: return k_persist<KIND, 1, REC, MODE>;
     case 2: return k_persist<KIND, 2, REC, MODE>;
-    case 4: return k_persist<KIND, 4, REC, (MODE & 1)>;
-    default: return KIND == 1 ? k_persist<KIND, 8, REC, (MODE & 1)> : nullptr;
+    case 4: return (MODE & 4) ? nullptr : k_persist<KIND, 4, REC, (MODE & 1)>;
+    default: return (KIND == 1 && !(MODE & 4)) ? k_persist<KIND, 8, REC, (MODE & 1)> : nullptr;
     }
 }
 
@@ -716,32 +782,53 @@ kfn_t pick_kernel_nfp(int nfp)
     switch (nfp) {
     case 1: return k_render<KIND, 1, REC, MODE>;
     case 2: return k_render<KIND, 2, REC, MODE>;
-    case 4: return k_render<KIND, 4, REC, (MODE & 1)>;
-    default: return KIND == 1 ? k_render<KIND, 8, REC, (MODE & 1)> : nullptr;
+    case 4: return (MODE & 4) ? nullptr : k_render<KIND, 4, REC, (MODE & 1)>;
+    default: return (KIND == 1 && !(MODE & 4)) ? k_render<KIND, 8, REC, (MODE & 1)> : nullptr;
     }
 }
 
-// mode: bit 0 ColorTransferFunction path, bit 1 n_samples == 10 (VR only).
+// mode (VR only): bit 0 ColorTransferFunction path, bit 1 n_samples == 10,
+// bit 2 front-to-back early ray termination.
+template <bool REC>
+pfn_t pick_persist_vr(int nfp, int mode)
+{
+    switch (mode & 7) {
+    case 0: return pick_persist_nfp<1, REC, 0>(nfp);
+    case 1: return pick_persist_nfp<1, REC, 1>(nfp);
+    case 2: return pick_persist_nfp<1, REC, 2>(nfp);
+    case 3: return pick_persist_nfp<1, REC, 3>(nfp);
+    case 4: return pick_persist_nfp<1, REC, 4>(nfp);
+    case 5: return pick_persist_nfp<1, REC, 5>(nfp);
+    case 6: return pick_persist_nfp<1, REC, 6>(nfp);
+    default: return pick_persist_nfp<1, REC, 7>(nfp);
+    }
+}
+
+template <bool REC>
+kfn_t pick_kernel_vr(int nfp, int mode)
+{
+    switch (mode & 7) {
+    case 0: return pick_kernel_nfp<1, REC, 0>(nfp);
+    case 1: return pick_kernel_nfp<1, REC, 1>(nfp);
+    case 2: return pick_kernel_nfp<1, REC, 2>(nfp);
+    case 3: return pick_kernel_nfp<1, REC, 3>(nfp);
+    case 4: return pick_kernel_nfp<1, REC, 4>(nfp);
+    case 5: return pick_kernel_nfp<1, REC, 5>(nfp);
+    case 6: return pick_kernel_nfp<1, REC, 6>(nfp);
+    default: return pick_kernel_nfp<1, REC, 7>(nfp);
+    }
+}
+
 pfn_t pick_persist(int kind, int nfp, bool rec, int mode)
 {
     if (kind == 0) return rec ? pick_persist_nfp<0, true, 0>(nfp) : pick_persist_nfp<0, false, 0>(nfp);
-    switch (mode & 3) {
-    case 0: return rec ? pick_persist_nfp<1, true, 0>(nfp) : pick_persist_nfp<1, false, 0>(nfp);
-    case 1: return rec ? pick_persist_nfp<1, true, 1>(nfp) : pick_persist_nfp<1, false, 1>(nfp);
-    case 2: return rec ? pick_persist_nfp<1, true, 2>(nfp) : pick_persist_nfp<1, false, 2>(nfp);
-    default: return rec ? pick_persist_nfp<1, true, 3>(nfp) : pick_persist_nfp<1, false, 3>(nfp);
-    }
+    return rec ? pick_persist_vr<true>(nfp, mode) : pick_persist_vr<false>(nfp, mode);
 }
 
 kfn_t pick_kernel(int kind, int nfp, bool rec, int mode)
 {
     if (kind == 0) return rec ? pick_kernel_nfp<0, true, 0>(nfp) : pick_kernel_nfp<0, false, 0>(nfp);
-    switch (mode & 3) {
-    case 0: return rec ? pick_kernel_nfp<1, true, 0>(nfp) : pick_kernel_nfp<1, false, 0>(nfp);
-    case 1: return rec ? pick_kernel_nfp<1, true, 1>(nfp) : pick_kernel_nfp<1, false, 1>(nfp);
-    case 2: return rec ? pick_kernel_nfp<1, true, 2>(nfp) : pick_kernel_nfp<1, false, 2>(nfp);
-    default: return rec ? pick_kernel_nfp<1, true, 3>(nfp) : pick_kernel_nfp<1, false, 3>(nfp);
-    }
+    return rec ? pick_kernel_vr<true>(nfp, mode) : pick_kernel_vr<false>(nfp, mode);
 }
 
 }  // namespace
@@ -1222,7 +1309,7 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
             tot = (int)P.tft.size();
             T.mode = 1;
         }
-        q.mode = T.mode | (r.n_samples == 10 ? 2 : 0);
+        q.mode = T.mode | (r.n_samples == 10 ? 2 : 0) | ((r.flags & YTRC_FLAG_ERT) ? 4 : 0);
         // Transmittance-form compositing is exact when every sub-sample has
         // a = dt * e <= 1: bound e by the tables' largest magnitudes and dt
         // by the longest in-cell t-interval / n_samples.
