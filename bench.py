@@ -76,6 +76,22 @@ def alg_bytes(kind, samples, npix, nf=1, n_samples=10):
     return samples * (8.0 * nf * 4.0 + 1.0) / n_samples + npix * 3 * 4.0
 
 
+def measured_traffic(path, kind, args):
+    """HBM bytes per dispatch of the dominant kernel from a committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE in KiB per dispatch,
+    each multiplied by its copy-kernel calibration factor), when that summary
+    was taken on this same workload; None otherwise."""
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    want = {"n": args.n, "psize": args.psize, "cams": args.cams, "kind": kind}
+    if any(t.get("workload", {}).get(k) != v for k, v in want.items()):
+        return None
+    return int(round(1024 * (t["fetch_kib"] * t["fetch_factor"] + t["write_kib"] * t["write_factor"])))
+
+
 def cpu_baseline(ds, vol, tf, cams, samples, budget_s):
     """FP64 oracle (OpenMP, all host threads of this process, up to 16) on a
     bounded sample: requests taken smallest first, alternating kinds, until
@@ -176,6 +192,10 @@ def main():
     ap.add_argument("--serve-rates", default="10,50,200",
                     help="Poisson arrival rates (req/s) of the serving leg")
     ap.add_argument("--serve-n", type=int, default=48, help="requests per rate")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the launch-shape autotuner (8x8 tiles, 4 waves/group, occupancy grid)")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_k_persist.json"),
+                    help="measured HBM bytes per dispatch (rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run the HBM copy probe on this many GiB and exit")
     args = ap.parse_args()
@@ -184,7 +204,7 @@ def main():
               "(every GPU box here has one GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    from yt_amd import _lib, render
+    from yt_amd import _lib, autotune, render
 
     if args.copy_probe > 0:
         ctx = _lib.Context(0)
@@ -219,7 +239,16 @@ def main():
 
     flags = _lib.BATCH_PERSISTENT | (0 if args.no_graph else _lib.BATCH_GRAPH)
     specs = make_specs(vol, tf, cams)
-    batch = _lib.Batch(ctx, specs, flags)
+    cfg = dict(autotune.DEFAULT)
+    if not args.no_tune:
+        t = time.time()
+        cfg, table = autotune.tune(ctx, specs, flags, use_cache=False)
+        ranked = sorted(table, key=lambda x: x[1])
+        log(f"autotune: {len(table)} launch shapes in {time.time() - t:.1f}s; best "
+            + ", ".join(f"{c['tile'][0]}x{c['tile'][1]}/w{c['waves_per_group']}/g{c['groups_per_cu']}"
+                        f"={ms:.2f}ms" for c, ms in ranked[:4])
+            + f"; default {dict((str(c), ms) for c, ms in table).get(str(autotune.DEFAULT), float('nan')):.2f}ms")
+    batch = autotune.make_batch(ctx, specs, cfg, flags)
     for _ in range(args.warmup):
         batch.run()
     t0 = time.perf_counter()
@@ -237,7 +266,7 @@ def main():
     kinds = {}
     for kind in (0, 1):
         idx = [i for i in range(len(cams)) if cams[i][0] == kind]
-        b = _lib.Batch(ctx, [specs[i] for i in idx], _lib.BATCH_PERSISTENT)
+        b = autotune.make_batch(ctx, [specs[i] for i in idx], cfg, _lib.BATCH_PERSISTENT)
         b.run()
         tms = []
         for _ in range(max(3, args.steps)):
@@ -250,6 +279,7 @@ def main():
             f"{byt / ms / 1e6:.1f} GB/s algorithmic")
     dom = max(kinds, key=lambda k: kinds[k]["ms"])
     ach = kinds[dom]["bytes"] / kinds[dom]["ms"] / 1e6
+    traffic = measured_traffic(args.traffic_file, dom, args)
 
     # Plain: one launch + sync per image.
     plain = None
@@ -320,6 +350,7 @@ def main():
             "samples_per_step": int(step_samples),
             "visits_per_step": int(visits),
             "dda": "f64",
+            "launch": cfg,
             "parallelism": "1 GPU",
         },
         "images_per_s": round(imgs_s, 2),
@@ -333,7 +364,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "kernel_ms": round(kinds[dom]["ms"], 4),
             "copy_probe_gbps": round(copy_gbps, 1),
         },

@@ -51,6 +51,10 @@ extern "C" {
 #define YTRC_BATCH_PLAIN 0u      /* one kernel launch per request                    */
 #define YTRC_BATCH_PERSISTENT 1u /* one persistent launch draining a global tile queue */
 #define YTRC_BATCH_GRAPH 2u      /* capture the launch sequence into a hipGraph       */
+/* Rays per 64-lane tile (rows along image axis 0 x columns along axis 1). */
+#define YTRC_BATCH_TILE_8x8 0u
+#define YTRC_BATCH_TILE_4x16 16u
+#define YTRC_BATCH_TILE_16x4 32u
 
 typedef struct ytrc_ctx ytrc_ctx;
 typedef struct ytrc_batch ytrc_batch;
@@ -142,7 +146,7 @@ int ytrc_batch_read_image(ytrc_batch *b, int32_t i, double *image_out);
 /* Record mode only: per-pixel visit count, visit hash, sample count. */
 int ytrc_batch_read_record(ytrc_batch *b, int32_t i, int64_t *visits, uint64_t *hashes,
                            int64_t *samples);
-/* Launch shape: waves (8x8 ray tiles) per workgroup (1, 2 or 4) and, for
+/* Launch shape: waves (64-ray tiles) per workgroup (1, 2 or 4) and, for
  * persistent batches, workgroups per CU (0 = the occupancy limit). */
 int ytrc_batch_set_launch(ytrc_batch *b, int32_t waves_per_group, int32_t groups_per_cu);
 /* Plain, uncaptured batches: GPU time of request i's kernel in the last

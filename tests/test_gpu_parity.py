@@ -266,3 +266,59 @@ def test_early_ray_termination(ctx):
         assert rel_l2(img_f[..., :3], oimg[..., :3]) <= VR_TOL
     finally:
         vol.free()
+
+
+@pytest.mark.parametrize("tile", [(4, 16), (16, 4)])
+def test_tile_shapes(ctx, tile):
+    """Every tile shape casts the same rays with the same per-ray code: images
+    and recorded visit sequences equal the 8x8 tiling's exactly, for an image
+    size that leaves ragged tiles at both edges."""
+    ds = uniform_lognormal(32, seed=61, nprocs=4)
+    tf = vr_tf_for(ds, nbins=64)
+    vol = render.ResidentVolume(ctx, ds, proj_fields="density", vr_field="density")
+    try:
+        specs = [vol.projection_request(c0_params(70, normal=(0.2, -0.7, 0.6)), record=True),
+                 vol.vr_request(c0_params(37, normal=(0.6, 0.3, -0.74)), tf, record=True)]
+        base = _run(ctx, specs)
+        for flags in (_lib.BATCH_PLAIN, _lib.BATCH_PERSISTENT):
+            b = _lib.Batch(ctx, specs, flags, tile=tile)
+            try:
+                b.run()
+                for i in range(len(specs)):
+                    assert np.array_equal(b.read_image(i), base[i][0]), (flags, i)
+                    for got, want in zip(b.read_record(i), base[i][1]):
+                        assert np.array_equal(got, want), (flags, i)
+            finally:
+                b.close()
+    finally:
+        vol.free()
+
+
+def test_autotune_picks_a_working_shape(ctx, tmp_path):
+    """The autotuner measures each candidate, caches the fastest, returns the
+    cached entry unmeasured the second time, and the chosen launch shape
+    renders the same images as the default one."""
+    from yt_amd import autotune
+
+    ds = uniform_lognormal(32, seed=71)
+    tf = vr_tf_for(ds, nbins=64)
+    vol = render.ResidentVolume(ctx, ds, proj_fields="density", vr_field="density")
+    try:
+        specs = [vol.projection_request(c0_params(96, normal=(0.3, 0.5, 0.81))),
+                 vol.vr_request(c0_params(80, normal=(-0.4, 0.1, 0.9)), tf)]
+        path = str(tmp_path / "tune.json")
+        best, table = autotune.tune(ctx, specs, waves_per_group=(1, 4), groups_per_cu=(0, 2),
+                                    reps=1, path=path)
+        assert len(table) == 3 * 2 * 2 and all(ms > 0 for _, ms in table)
+        again, table2 = autotune.tune(ctx, specs, path=path)
+        assert again == best and table2 == []
+        base = _run(ctx, specs)
+        b = autotune.make_batch(ctx, specs, best)
+        try:
+            b.run()
+            for i in range(len(specs)):
+                assert np.array_equal(b.read_image(i), base[i][0]), i
+        finally:
+            b.close()
+    finally:
+        vol.free()

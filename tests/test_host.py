@@ -154,3 +154,22 @@ def test_library_missing_fails_loudly(monkeypatch):
     monkeypatch.setattr(_lib, "LIB_PATH", "/nonexistent/libytrc.so")
     with pytest.raises(_lib.YtrcLibraryError):
         _lib.lib()
+
+
+def test_autotune_signature_and_cache(tmp_path):
+    """The autotune cache key separates kernel variants and image-size
+    classes, and the JSON cache round-trips."""
+    from yt_amd import _lib, autotune
+
+    p = {"nv": (64, 64)}
+    a = _lib.RequestSpec(_lib.PROJECTION, p, 0, [0])
+    b = _lib.RequestSpec(_lib.VOLUME_RENDER, p, 0, [0])
+    big = _lib.RequestSpec(_lib.PROJECTION, {"nv": (1024, 1024)}, 0, [0])
+    s1 = autotune.signature([a, a], "dev")
+    assert s1 != autotune.signature([a, b], "dev")
+    assert s1 != autotune.signature([big, big], "dev")
+    assert s1 == autotune.signature([a, a], "dev")
+    path = str(tmp_path / "sub" / "t.json")
+    autotune._store(path, {"k": {"cfg": autotune.DEFAULT}})
+    assert autotune._load(path)["k"]["cfg"] == autotune.DEFAULT
+    assert autotune._load(str(tmp_path / "missing.json")) == {}

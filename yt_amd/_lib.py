@@ -31,6 +31,8 @@ FLAG_ERT = 2
 BATCH_PLAIN = 0
 BATCH_PERSISTENT = 1
 BATCH_GRAPH = 2
+# rays per 64-lane tile, rows x columns
+BATCH_TILE = {(8, 8): 0, (4, 16): 16, (16, 4): 32}
 
 EXPORTED = [
     "ytrc_ctx_create",
@@ -381,9 +383,13 @@ class RequestSpec:
 class Batch:
     """A set of requests staged on the device; launch() runs them."""
 
-    def __init__(self, ctx, specs, flags=BATCH_PLAIN):
+    def __init__(self, ctx, specs, flags=BATCH_PLAIN, tile=(8, 8)):
         self.ctx = ctx
         self.specs = list(specs)
+        if tuple(tile) not in BATCH_TILE:
+            raise ValueError(f"tile must be one of {sorted(BATCH_TILE)}")
+        self.tile = tuple(tile)
+        flags = int(flags) | BATCH_TILE[self.tile]
         n = len(self.specs)
         arr = (Request * n)()
         for i, s in enumerate(self.specs):

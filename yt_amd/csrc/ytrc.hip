@@ -512,11 +512,13 @@ template <int KIND, int NFP, bool REC, int MODE>
 __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
                                             const float4 *lt)
 {
+    const int tsh = im.tsh;
+    const int tw = 1 << tsh, th = 64 >> tsh;
     const int tx = tile / im.tiles1;
     const int ty = tile - tx * im.tiles1;
-    const int ti0 = tx * TILE, tj0 = ty * TILE;
-    const int vi = ti0 + (lane >> 3);
-    const int vj = tj0 + (lane & 7);
+    const int ti0 = tx * th, tj0 = ty * tw;
+    const int vi = ti0 + (lane >> tsh);
+    const int vj = tj0 + (lane & (tw - 1));
     const bool active = (vi < im.nv0) && (vj < im.nv1);
     const int64_t npix = (int64_t)im.nv0 * im.nv1;
     const int64_t pix = active ? (int64_t)vi * im.nv1 + vj : 0;
@@ -589,7 +591,7 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
     for (int kk = 0; kk < im.n_order; kk++) {
         const int k = ert ? im.n_order - 1 - kk : kk;
         const int4 e = im.ext[k];
-        if (e.y <= ti0 || e.x >= ti0 + TILE || e.w <= tj0 || e.z >= tj0 + TILE) continue;
+        if (e.y <= ti0 || e.x >= ti0 + th || e.w <= tj0 || e.z >= tj0 + tw) continue;
         bool go = active && vi >= e.x && vi < e.y && vj >= e.z && vj < e.w;
         const int bid = im.order[k];
         const DBrick &b = im.bricks[bid];
@@ -1178,10 +1180,13 @@ struct PrepReq {
     size_t o_order = 0, o_ext = 0, o_zbuf = 0, o_init = 0, o_out = 0;
     size_t o_rv = 0, o_rh = 0, o_rs = 0, o_tf = 0;
     size_t bytes = 0;
+    int tsh = TILE_SHIFT_DEFAULT;
 };
 
-int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, PrepReq &P)
+int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, PrepReq &P,
+                 int tsh = TILE_SHIFT_DEFAULT)
 {
+    P.tsh = tsh;
     ReqHost &q = P.q;
     q.r = r;
     if (r.kind != YTRC_PROJECTION && r.kind != YTRC_VOLUME_RENDER)
@@ -1204,7 +1209,8 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
             return fail(YTRC_EINVAL, tag + "VolumeRenderSampler needs vertex-centred bricks");
     }
     q.npix = (int64_t)r.nv[0] * r.nv[1];
-    const int t0 = (r.nv[0] + TILE - 1) / TILE, t1 = (r.nv[1] + TILE - 1) / TILE;
+    const int th = 64 >> tsh, tw = 1 << tsh;
+    const int t0 = (r.nv[0] + th - 1) / th, t1 = (r.nv[1] + tw - 1) / tw;
     q.ntiles = t0 * t1;
     q.nfp = ds.nfp;
     const bool rec = (r.flags & YTRC_FLAG_RECORD) != 0;
@@ -1412,8 +1418,9 @@ int stage_request(ytrc_ctx *ctx, PrepReq &P, char *A, const float4 *tf_table, in
     d.n_order = r.n_order;
     d.nf = ds.nf;
     d.nfp = ds.nfp;
-    d.tiles0 = (r.nv[0] + TILE - 1) / TILE;
-    d.tiles1 = (r.nv[1] + TILE - 1) / TILE;
+    d.tsh = P.tsh;
+    d.tiles0 = (r.nv[0] + (64 >> P.tsh) - 1) / (64 >> P.tsh);
+    d.tiles1 = (r.nv[1] + (1 << P.tsh) - 1) / (1 << P.tsh);
     d.flags = (int32_t)r.flags;
     d.nch_out = q.nch_out;
     d.inv_ns = r.n_samples > 0 ? 1.0 / (double)r.n_samples : 0.0;
@@ -1473,8 +1480,11 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
     std::vector<PrepReq> P(n_req);
     size_t off = 0;
     std::vector<size_t> base(n_req);
+    const int tcode = (int)((batch_flags >> 4) & 3);
+    if (tcode == 3) return fail(YTRC_EINVAL, "ytrc_batch_create: unknown tile shape");
+    const int tsh = tcode == 1 ? 4 : (tcode == 2 ? 2 : TILE_SHIFT_DEFAULT);
     for (int i = 0; i < n_req; i++) {
-        const int rc = prep_request(ctx, reqs[i], "request " + std::to_string(i) + ": ", P[i]);
+        const int rc = prep_request(ctx, reqs[i], "request " + std::to_string(i) + ": ", P[i], tsh);
         if (rc != YTRC_OK) return rc;
         base[i] = off;
         off = align_up(off + P[i].bytes, 256);

@@ -68,11 +68,14 @@ struct alignas(16) DImg {
     int32_t nfp, tiles0;
     int32_t tiles1, flags;
     int32_t nch_out, tf_slot;
-    int32_t tform, pad1;    // 1: VR compositing state kept as transmittance
+    int32_t tform;          // 1: VR compositing state kept as transmittance
+    int32_t tsh;            // tile = (64 >> tsh) rows x (1 << tsh) columns of rays
     double inv_ns;
 };
 
-constexpr int TILE = 8;  // 8x8 rays per 64-lane wavefront
+// One 64-lane wavefront casts one tile of 64 rays: 8x8 by default, 4x16 or
+// 16x4 when a batch is built with another shape.
+constexpr int TILE_SHIFT_DEFAULT = 3;
 
 __device__ __forceinline__ double y_fmin(double a, double b) { return (a < b) ? a : b; }
 
