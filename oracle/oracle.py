@@ -12,7 +12,8 @@ Parity pinning of each oracle function (see tests/test_oracle_*.py):
   ProjectionSampler      closed forms (sqrt(3) chord, constant field), linearity
   VolumeRenderSampler    closed form 1 - prod(1 - dt*e) for constant fields
   FIT_get_value          exact at nodes, linear between, 0 outside/NaN/0
-  offset_interpolate     exact on linear fields, corner values
+  offset_interpolate     exact on linear fields, corner values, and bit-equal to
+                         yt's own fixed_interpolator.cpp compiled into _ref/
   lens extent            conservativeness against the ray lattice
 """
 
@@ -24,6 +25,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libytrc_oracle.so")
+REF_LIB_PATH = os.path.join(HERE, "_ref", "libyt_fixed_interpolator.so")
 
 _lib = None
 
@@ -226,3 +228,38 @@ def trilinear(dims, vdata, cells, dps):
 
 def max_threads():
     return lib().orc_max_threads()
+
+
+_ref = None
+
+
+def ref_lib():
+    """yt's fixed_interpolator.cpp built from the reference tree (oracle/Makefile
+    target ``ref``), or None where it was not built.  A checker only."""
+    global _ref
+    if _ref is None and os.path.exists(REF_LIB_PATH):
+        L = C.CDLL(REF_LIB_PATH)
+        # C++ symbols of fixed_interpolator.hpp:17-24 (g++ name mangling)
+        f = L._Z18offset_interpolatePiPdS0_
+        f.restype = C.c_double
+        f.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        g = L._Z13eval_gradientPiPdS0_S0_
+        g.restype = None
+        g.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                      C.POINTER(C.c_double)]
+        _ref = L
+    return _ref
+
+
+def ref_offset_interpolate(dims, vdata, cells, dps):
+    """yt's offset_interpolate (fixed_interpolator.cpp:34-52) at each cell's
+    offset (i*(d1+1)+j)*(d2+1)+k, as image_samplers.pyx:465-466 computes it."""
+    L = ref_lib()
+    ds = (C.c_int * 3)(*[int(x) for x in dims])
+    vd = np.ascontiguousarray(vdata, dtype=np.float64).ravel()
+    out = np.empty(len(cells))
+    for n, (c, dp) in enumerate(zip(cells, dps)):
+        off = (int(c[0]) * (dims[1] + 1) + int(c[1])) * (dims[2] + 1) + int(c[2])
+        p = (C.c_double * 3)(*[float(x) for x in dp])
+        out[n] = L._Z18offset_interpolatePiPdS0_(ds, p, _d(vd[off:]))
+    return out
