@@ -202,6 +202,7 @@ struct TFCache {
     int grey;
     int fid;       // mode 1: the one field
     float b0, b1, idbin, nb0idbin;  // nb0idbin = -b0 * idbin
+    float qc1, qzb;                 // transmittance-form CTF entry selection
     int nbm2;      // mode 1: nbins - 2
     int nbins[6];
     int off[6];
@@ -227,6 +228,8 @@ __device__ __forceinline__ void load_tf(const DTF &T, TFCache &c)
     c.b1 = unif(T.b1[0]);
     c.idbin = unif(T.idbin[0]);
     c.nb0idbin = -c.b0 * c.idbin;
+    c.qc1 = unif(T.qc1);
+    c.qzb = unif(T.qzb);
     c.nbm2 = uni(T.nbins[0] - 2);
 #pragma unroll
     for (int t = 0; t < 6; t++) {
@@ -288,25 +291,23 @@ __device__ __forceinline__ void fit_eval_general(const TFCache &tf, const float4
 // four tables, e_c = table_c(v) * table_alpha(v) for c < 3, e_3 = alpha.
 // An out-of-range or non-normal v reads bin 0 and gets alpha 0, so every
 // channel is 0 without a select per channel.
-__device__ __forceinline__ void fit_eval_ctf(const TFCache &tf, const float4 *__restrict__ lt,
-                                             int nb, float v, float e[4])
+// ColorTransferFunction emissions dt-free, transmittance-form layout: entry
+// floor(u) of the per-bin quadratics in the in-bin coordinate fract(u).
+__device__ __forceinline__ void fit_eval_quad(const TFCache &tf, const float4 *__restrict__ lt,
+                                              float v, float e[3])
 {
     const bool ok = (v < tf.b1) && (v > tf.b0) && __builtin_isnormal(v);
-    const float vs = ok ? v : tf.b0;
-    int bin = (int)__builtin_fmaf(vs, tf.idbin, tf.nb0idbin);
-    bin = bin > tf.nbm2 ? tf.nbm2 : bin;
-    // Bins are interleaved {y_rgba}, {dy_rgba}, {d0, -, -, -}: one address.
+    float u = __builtin_fmaf(v, tf.idbin, tf.qc1);
+    u = ok ? u : tf.qzb;
+    const int bin = (int)u;
+    const float f = __builtin_amdgcn_fractf(u);
     const float4 *q = lt + 3 * bin;
-    const float4 y = q[0];
-    const float4 dy = q[1];
-    const float d0 = q[2].x;
-    const float dd = vs - d0;
-    float a = __builtin_fmaf(dd, dy.w, y.w);
-    a = ok ? a : 0.0f;
-    e[0] = __builtin_fmaf(dd, dy.x, y.x) * a;
-    e[1] = __builtin_fmaf(dd, dy.y, y.y) * a;
-    e[2] = __builtin_fmaf(dd, dy.z, y.z) * a;
-    e[3] = a;
+    const float4 q0 = q[0];
+    const float4 q1 = q[1];
+    const float c2 = q[2].x;
+    e[0] = __builtin_fmaf(__builtin_fmaf(q1.z, f, q0.w), f, q0.x);
+    e[1] = __builtin_fmaf(__builtin_fmaf(q1.w, f, q1.x), f, q0.y);
+    e[2] = __builtin_fmaf(__builtin_fmaf(c2, f, q1.y), f, q0.z);
 }
 
 // Brick fields the samplers read, copied by value (uniform across a wave).
@@ -364,7 +365,8 @@ __device__ __forceinline__ void trilinear_cubic(const float (&c)[8], const float
 }
 
 // VolumeRenderSampler.sample (image_samplers.pyx:453-487).
-// MODE bit 0: ColorTransferFunction path; bit 1: n_samples == 10 unrolled.
+// MODE bit 0: ColorTransferFunction in transmittance form; bit 1: n_samples == 10 unrolled;
+// bit 2: early ray termination.
 // st[] is the compositing state: rgba, or, when `tform` is set (non-grey
 // and the host has shown dt * e <= 1 for every sub-sample), the per-channel
 // transmittance T = 1 - rgba, for which rgba' = a + max(1 - a, 0) rgba
@@ -387,12 +389,9 @@ struct VRSampler {
     double tb[4];
     Rec *rec;
 
-    __device__ __forceinline__ void eval(float v0, const float (&dvs)[NFP], float e[4]) const
+    __device__ __forceinline__ void eval(const float (&dvs)[NFP], float e[4]) const
     {
-        if (MODE & 1)
-            fit_eval_ctf(*tf, lt, nb, v0, e);
-        else
-            fit_eval_general<NFP>(*tf, lt, dvs, e);
+        fit_eval_general<NFP>(*tf, lt, dvs, e);
     }
 
     template <int NS, class Body>
@@ -417,11 +416,33 @@ struct VRSampler {
                 dvs[f] = __builtin_fmaf(
                     __builtin_fmaf(__builtin_fmaf(k[f][3], sf, k[f][2]), sf, k[f][1]), sf, k[f][0]);
         };
-        if (tform) {
+        if constexpr ((MODE & 1) != 0) {
+            // The visit's opacity Q_c = 1 - prod_s (1 - dt e_s) by the
+            // recurrence Q' = Q + e G, G = dt (1 - Q): Q stays small where
+            // the medium is thin, so FP32 keeps its relative precision;
+            // the FP64 transmittance then takes T (1 - Q) once per visit.
+            const float dtf = (float)dt;
+            float Q[3] = {0.0f, 0.0f, 0.0f};
+            float G[3] = {dtf, dtf, dtf};
+            const int nsq = (NS > 0) ? NS : ns;
+#pragma unroll 2
+            for (int s = 0; s < nsq; s++) {
+                float dvs[NFP], e[3];
+                value(s, dvs);
+                fit_eval_quad(T, lt, (NFP == 1) ? dvs[0] : pick(dvs, T.fid), e);
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++) {
+                    Q[ch] = __builtin_fmaf(e[ch], G[ch], Q[ch]);
+                    G[ch] = __builtin_fmaf(-dtf, Q[ch], dtf);
+                }
+            }
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++) st[ch] = __builtin_fma(-st[ch], (double)Q[ch], st[ch]);
+        } else if (tform) {
             each_sample<NS>(ns, [&](int s) {
                 float dvs[NFP], e[4];
                 value(s, dvs);
-                eval((NFP == 1) ? dvs[0] : pick(dvs, T.fid), dvs, e);
+                eval(dvs, e);
 #pragma unroll
                 for (int ch = 0; ch < 3; ch++) st[ch] *= __builtin_fma(-dt, (double)e[ch], 1.0);
             });
@@ -429,7 +450,7 @@ struct VRSampler {
             each_sample<NS>(ns, [&](int s) {
                 float dvs[NFP], e[4];
                 value(s, dvs);
-                eval((NFP == 1) ? dvs[0] : pick(dvs, T.fid), dvs, e);
+                eval(dvs, e);
                 const double ta = y_fmax(1.0 - dt * (double)e[3], 0.0);
 #pragma unroll
                 for (int ch = 0; ch < 4; ch++) st[ch] = __builtin_fma(ta, st[ch], dt * (double)e[ch]);
@@ -439,7 +460,7 @@ struct VRSampler {
             each_sample<NS>(ns, [&](int s) {
                 float dvs[NFP], e[4];
                 value(s, dvs);
-                eval((NFP == 1) ? dvs[0] : pick(dvs, T.fid), dvs, e);
+                eval(dvs, e);
 #pragma unroll
                 for (int ch = 0; ch < 3; ch++) {
                     const double a = dt * (double)e[ch];
@@ -1293,7 +1314,7 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
             T.chexp[c] = ce;
         }
         // ColorTransferFunction pattern (transfer_functions.py:385-408):
-        // one shared bin per sample, tables interleaved per bin.
+        // four tables on one field sharing bins, rgb weighted by alpha.
         bool ctf = tf.n_fits == 4;
         for (int t = 0; ctf && t < 4; t++)
             ctf = tf.field_ids[t] == tf.field_ids[0] && tf.weight_field_ids[t] == -1 &&
@@ -1301,21 +1322,6 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
                   tf.bounds[2 * t + 1] == tf.bounds[1];
         ctf = ctf && T.chexp[0] == (1 | (1 << 6)) && T.chexp[1] == ((1 << 2) | (1 << 6)) &&
               T.chexp[2] == ((1 << 4) | (1 << 6)) && T.chexp[3] == (1 << 6);
-        if (ctf) {
-            const int nb = tf.nbins[0];
-            std::vector<float4> lay((size_t)3 * nb);
-            for (int k = 0; k < nb; k++) {
-                const float4 a0 = P.tft[k], a1 = P.tft[nb + k], a2 = P.tft[2 * nb + k],
-                             a3 = P.tft[3 * nb + k];
-                lay[3 * k] = make_float4(a0.x, a1.x, a2.x, a3.x);
-                lay[3 * k + 1] = make_float4(a0.y, a1.y, a2.y, a3.y);
-                lay[3 * k + 2] = make_float4(a0.z, 0.0f, 0.0f, 0.0f);
-            }
-            P.tft = std::move(lay);
-            tot = (int)P.tft.size();
-            T.mode = 1;
-        }
-        q.mode = T.mode | (r.n_samples == 10 ? 2 : 0) | ((r.flags & YTRC_FLAG_ERT) ? 4 : 0);
         // Transmittance-form compositing is exact when every sub-sample has
         // a = dt * e <= 1: bound e by the tables' largest magnitudes and dt
         // by the longest in-cell t-interval / n_samples.
@@ -1344,6 +1350,52 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
             }
             q.tform = bounded && (emax * (tcell / (double)r.n_samples) * 1.001 <= 1.0);
         }
+        // ColorTransferFunction pattern (transfer_functions.py:385-408) in
+        // transmittance form gets its own kernel path (MODE bit 0); grey or
+        // unbounded CTFs take the general per-table path.
+        ctf = ctf && q.tform;
+        if (ctf) {
+            // Transmittance form: per bin, the three emissions e_c(f) =
+            // (y_c + f dy_c)(y_a + f dy_a) as quadratics in the in-bin
+            // coordinate f, dy = y[b+1] - y[b] (FIT_get_value's lerp,
+            // field_interpolation_tables.pxd:63-82, times the alpha weight
+            // table, transfer_functions.py:385-408), coefficients rounded
+            // once from FP64.  Entry 0 continues bin 0 below its first node
+            // and entry nb continues bin nb-2 above its last, so the bin
+            // index needs no clamp; entry nb+1 is zero for values outside
+            // (b0, b1).  u = 1 + (v - b0) * idbin selects the entry.
+            const int nb = tf.nbins[0];
+            const double b0 = tf.bounds[0], b1 = tf.bounds[1];
+            const double idbin = 1.0 / ((b1 - b0) / (double)(nb - 1));
+            std::vector<float4> lay((size_t)3 * (nb + 2), make_float4(0.f, 0.f, 0.f, 0.f));
+            for (int k = 0; k <= nb; k++) {
+                const int bb = std::min(std::max(k - 1, 0), nb - 2);
+                double y[4], dy[4];
+                for (int t = 0; t < 4; t++) {
+                    const double *yt = tf.y[t];
+                    dy[t] = yt[bb + 1] - yt[bb];
+                    y[t] = (k == 0) ? yt[0] - dy[t] : (k == nb ? yt[nb - 1] : yt[bb]);
+                }
+                float A[3], B[3], Cq[3];
+                for (int c = 0; c < 3; c++) {
+                    A[c] = (float)(y[c] * y[3]);
+                    B[c] = (float)(y[c] * dy[3] + dy[c] * y[3]);
+                    Cq[c] = (float)(dy[c] * dy[3]);
+                }
+                lay[3 * k] = make_float4(A[0], A[1], A[2], B[0]);
+                lay[3 * k + 1] = make_float4(B[1], B[2], Cq[0], Cq[1]);
+                lay[3 * k + 2] = make_float4(Cq[2], 0.f, 0.f, 0.f);
+            }
+            P.tft = std::move(lay);
+            T.idbin[0] = (float)idbin;
+            T.qc1 = (float)(1.0 - b0 * idbin);
+            T.qzb = (float)(nb + 1);
+        }
+        if (ctf) {
+            tot = (int)P.tft.size();
+            T.mode = 1;
+        }
+        q.mode = T.mode | (r.n_samples == 10 ? 2 : 0) | ((r.flags & YTRC_FLAG_ERT) ? 4 : 0);
         T.total_bins = tot;
         q.lds = (size_t)tot * sizeof(float4);
         if (q.lds > 64 * 1024) return fail(YTRC_EINVAL, tag + "transfer-function tables exceed 64 KiB of LDS");

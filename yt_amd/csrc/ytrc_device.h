@@ -42,6 +42,9 @@ struct alignas(16) DTF {
     float b0[6];
     float b1[6];
     float idbin[6];
+    // ColorTransferFunction tables in transmittance form: bin-coordinate
+    // offset (u = v * idbin + qc1) and the index of the all-zero entry.
+    float qc1, qzb;
     const float4 *table;  // total_bins float4 in global memory
 };
 
