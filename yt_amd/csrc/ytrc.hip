@@ -145,17 +145,17 @@ __device__ __forceinline__ void walk_projection(const DBrick &b, Walk &w, double
                     c0 += st0;
                     t0 += td0;
                     cn = c0;
-                    dn = b.dims[0];
+                    dn = S.dims0;
                 } else if (ax == 1) {
                     c1 += st1;
                     t1 += td1;
                     cn = c1;
-                    dn = b.dims[1];
+                    dn = S.d1;
                 } else {
                     c2 += st2;
                     t2 += td2;
                     cn = c2;
-                    dn = b.dims[2];
+                    dn = S.d2;
                 }
                 enter_t = tm;
                 live = !(cn < 0 || cn >= dn || enter_t >= max_t);
@@ -291,17 +291,22 @@ __device__ __forceinline__ void fit_eval_general(const TFCache &tf, const float4
 // four tables, e_c = table_c(v) * table_alpha(v) for c < 3, e_3 = alpha.
 // An out-of-range or non-normal v reads bin 0 and gets alpha 0, so every
 // channel is 0 without a select per channel.
-// ColorTransferFunction emissions dt-free, transmittance-form layout: entry
-// floor(u) of the per-bin quadratics in the in-bin coordinate fract(u).
+// ColorTransferFunction emissions in the transmittance-form layout: entry
+// floor(u) of the per-bin quadratics, in-bin coordinate fract(u), where
+// u = 1 + (v - b0) * idbin.  FIT_get_value's range test v in (b0, b1)
+// (field_interpolation_tables.pxd:63-82) becomes u in [1, nb) through the
+// zero entries 0 and nb, nb + 1 and a clamp of u to [0, nb + 1]; the test
+// is thereby made on the FP32 bin coordinate, exact up to its rounding.
+// Values that are not normal (0, subnormal, inf, NaN) select entry 0.
 __device__ __forceinline__ void fit_eval_quad(const TFCache &tf, const float4 *__restrict__ lt,
                                               float v, float e[3])
 {
-    const bool ok = (v < tf.b1) && (v > tf.b0) && __builtin_isnormal(v);
     float u = __builtin_fmaf(v, tf.idbin, tf.qc1);
-    u = ok ? u : tf.qzb;
-    const int bin = (int)u;
+    u = __builtin_isnormal(v) ? u : 0.0f;
+    u = __builtin_amdgcn_fmed3f(u, 0.0f, tf.qzb);
+    const unsigned bin = (unsigned)u;
     const float f = __builtin_amdgcn_fractf(u);
-    const float4 *q = lt + 3 * bin;
+    const float4 *q = (const float4 *)((const char *)lt + __umul24(bin, 48u));
     const float4 q0 = q[0];
     const float4 q1 = q[1];
     const float c2 = q[2].x;
@@ -323,13 +328,13 @@ __device__ __forceinline__ BrickReg brick_reg(const DBrick &b)
     BrickReg r;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        r.le[i] = b.le[i];
-        r.dds[i] = b.dds[i];
-        r.idds[i] = b.idds[i];
+        r.le[i] = uni_d(b.le[i]);
+        r.dds[i] = uni_d(b.dds[i]);
+        r.idds[i] = uni_d(b.idds[i]);
     }
-    r.d0 = b.dims[0];
-    r.d1 = b.dims[1];
-    r.d2 = b.dims[2];
+    r.d0 = uni(b.dims[0]);
+    r.d1 = uni(b.dims[1]);
+    r.d2 = uni(b.dims[2]);
     return r;
 }
 
@@ -425,10 +430,14 @@ struct VRSampler {
             float Q[3] = {0.0f, 0.0f, 0.0f};
             float G[3] = {dtf, dtf, dtf};
             const int nsq = (NS > 0) ? NS : ns;
-#pragma unroll 2
-            for (int s = 0; s < nsq; s++) {
+            float sf = 0.0f;
+#pragma unroll
+            for (int s = 0; s < nsq; s++, sf += 1.0f) {
                 float dvs[NFP], e[3];
-                value(s, dvs);
+#pragma unroll
+                for (int f = 0; f < NFP; f++)
+                    dvs[f] = __builtin_fmaf(
+                        __builtin_fmaf(__builtin_fmaf(k[f][3], sf, k[f][2]), sf, k[f][1]), sf, k[f][0]);
                 fit_eval_quad(T, lt, (NFP == 1) ? dvs[0] : pick(dvs, T.fid), e);
 #pragma unroll
                 for (int ch = 0; ch < 3; ch++) {
@@ -623,9 +632,9 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
             if (KIND == 0) {
                 ProjSampler<NFP, REC> s;
                 s.data = im.pool + b.data_off;
-                s.d1 = b.dims[1];
-                s.d2 = b.dims[2];
-                s.dims0 = b.dims[0];
+                s.d1 = uni(b.dims[1]);
+                s.d2 = uni(b.dims[2]);
+                s.dims0 = uni(b.dims[0]);
 #pragma unroll
                 for (int f = 0; f < NFP; f++) {
                     s.s[f] = ps[f];
@@ -741,7 +750,7 @@ __global__ __launch_bounds__(256) void k_render(const DImg *__restrict__ imgs, i
 // no record/ERT) fit 128 VGPRs, i.e. 4 waves per SIMD, without spilling.
 template <int KIND, int NFP, bool REC, int MODE>
 struct MinWaves {
-    static constexpr int value = (KIND == 1 && NFP == 1 && !REC && (MODE == 1 || MODE == 3)) ? 4 : 1;
+    static constexpr int value = (KIND == 1 && NFP == 1 && !REC && (MODE == 1 || MODE == 3)) ? 5 : 1;
 };
 
 template <int KIND, int NFP, bool REC, int MODE>
@@ -1360,21 +1369,18 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
             // coordinate f, dy = y[b+1] - y[b] (FIT_get_value's lerp,
             // field_interpolation_tables.pxd:63-82, times the alpha weight
             // table, transfer_functions.py:385-408), coefficients rounded
-            // once from FP64.  Entry 0 continues bin 0 below its first node
-            // and entry nb continues bin nb-2 above its last, so the bin
-            // index needs no clamp; entry nb+1 is zero for values outside
-            // (b0, b1).  u = 1 + (v - b0) * idbin selects the entry.
+            // once from FP64.  Entry k = 1..nb-1 holds bin k-1; entries 0,
+            // nb and nb+1 are zero (values outside (b0, b1)).
             const int nb = tf.nbins[0];
             const double b0 = tf.bounds[0], b1 = tf.bounds[1];
             const double idbin = 1.0 / ((b1 - b0) / (double)(nb - 1));
             std::vector<float4> lay((size_t)3 * (nb + 2), make_float4(0.f, 0.f, 0.f, 0.f));
-            for (int k = 0; k <= nb; k++) {
-                const int bb = std::min(std::max(k - 1, 0), nb - 2);
+            for (int k = 1; k < nb; k++) {
+                const int bb = k - 1;
                 double y[4], dy[4];
                 for (int t = 0; t < 4; t++) {
-                    const double *yt = tf.y[t];
-                    dy[t] = yt[bb + 1] - yt[bb];
-                    y[t] = (k == 0) ? yt[0] - dy[t] : (k == nb ? yt[nb - 1] : yt[bb]);
+                    dy[t] = tf.y[t][bb + 1] - tf.y[t][bb];
+                    y[t] = tf.y[t][bb];
                 }
                 float A[3], B[3], Cq[3];
                 for (int c = 0; c < 3; c++) {

@@ -201,6 +201,9 @@ __device__ __forceinline__ int walk_loop(const DBrick &b, Walk &w, double max_t,
     double t0 = w.tmax[0], t1 = w.tmax[1], t2 = w.tmax[2];
     int c0 = w.cur[0], c1 = w.cur[1], c2 = w.cur[2];
     double enter_t = w.enter_t;
+    // Brick dims in scalar registers: the axis-indexed bound below would
+    // otherwise be a memory load on every step.
+    const int D0 = uni(b.dims[0]), D1 = uni(b.dims[1]), D2 = uni(b.dims[2]);
     while (true) {
         hit += 1;
         const bool a01 = t0 < t1;
@@ -215,17 +218,17 @@ __device__ __forceinline__ int walk_loop(const DBrick &b, Walk &w, double max_t,
             c0 += w.step[0];
             t0 += w.tdelta[0];
             cn = c0;
-            dn = b.dims[0];
+            dn = D0;
         } else if (ax == 1) {
             c1 += w.step[1];
             t1 += w.tdelta[1];
             cn = c1;
-            dn = b.dims[1];
+            dn = D1;
         } else {
             c2 += w.step[2];
             t2 += w.tdelta[2];
             cn = c2;
-            dn = b.dims[2];
+            dn = D2;
         }
         enter_t = tm;
         if (cn < 0 || cn >= dn || enter_t >= max_t) break;
