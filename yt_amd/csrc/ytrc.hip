@@ -94,6 +94,7 @@ struct ProjSampler {
     const float *data;
     int d1, d2;
     int dims0;
+    int nbj, nbk;  // 4-cell blocks along axes 1 and 2
     float s[NFP];
     float c[NFP];
     Rec *rec;
@@ -135,9 +136,13 @@ __device__ __forceinline__ void walk_projection(const DBrick &b, Walk &w, double
                 }
                 if ((unsigned)c0 < (unsigned)S.dims0 && (unsigned)c1 < (unsigned)S.d1 &&
                     (unsigned)c2 < (unsigned)S.d2) {
-                    // in-range indices of a brick below 2^24 points per axis
-                    // product: full-rate 24-bit multiplies are exact here
-                    off[u] = (int)__umul24(__umul24(c0, S.d1) + c1, S.d2) + c2;
+                    // 4x4x4-blocked cell offset (blocked_offset); block
+                    // counts per face are below 2^24, so 24-bit multiplies
+                    // are exact here
+                    const unsigned blk =
+                        __umul24(__umul24((unsigned)c0 >> 2, (unsigned)S.nbj) + ((unsigned)c1 >> 2),
+                                 (unsigned)S.nbk) + ((unsigned)c2 >> 2);
+                    off[u] = (int)((blk << 6) | ((c0 & 3) << 4) | ((c1 & 3) << 2) | (c2 & 3));
                     dl[u] = (float)(exit_t - enter_t);
                 }
                 int cn, dn;
@@ -635,6 +640,8 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
                 s.d1 = uni(b.dims[1]);
                 s.d2 = uni(b.dims[2]);
                 s.dims0 = uni(b.dims[0]);
+                s.nbj = (s.d1 + 3) >> 2;
+                s.nbk = (s.d2 + 3) >> 2;
 #pragma unroll
                 for (int f = 0; f < NFP; f++) {
                     s.s[f] = ps[f];
@@ -740,6 +747,10 @@ __global__ __launch_bounds__(256) void k_render(const DImg *__restrict__ imgs, i
     render_tile<KIND, NFP, REC, MODE>(im, tile, threadIdx.x & 63, s_tf);
 }
 
+// Tile-queue partitions of a persistent launch: one per XCD of the MI355X
+// (256 CUs in 8 XCDs of 32, each XCD with its own L2).
+constexpr int QSPLIT = 8;
+
 // Persistent launch: every wave repeatedly takes the next (image, tile)
 // item of a global queue until the queue is drained.  All transfer-function
 // tables of the batch sit in LDS; each image reads its own slot.  The item
@@ -766,12 +777,26 @@ void k_persist(const DImg *__restrict__ imgs,
         __syncthreads();
     }
     const int lane = threadIdx.x & 63;
+    // The item list is split into QSPLIT contiguous ranges with a counter
+    // each.  Workgroups are dispatched round-robin over the XCDs, so
+    // blockIdx.x % QSPLIT names this workgroup's XCD: it drains the range of
+    // neighbouring tiles that XCD's L2 already holds lines for, then helps
+    // with the other ranges in turn.  Every wave exits after finding all
+    // QSPLIT ranges empty.
+    int part = (int)(blockIdx.x % QSPLIT);
+    int left = QSPLIT;
     while (true) {
         unsigned int it = 0;
-        if (lane == 0) it = atomicAdd(counter, 1u);
+        if (lane == 0) it = atomicAdd(counter + part, 1u);
         it = (unsigned int)__builtin_amdgcn_readfirstlane((int)__shfl(it, 0));
-        if (it >= (unsigned int)n_items) break;
-        const int2 w = items[it];
+        const unsigned int lo = (unsigned int)(((int64_t)n_items * part) / QSPLIT);
+        const unsigned int hi = (unsigned int)(((int64_t)n_items * (part + 1)) / QSPLIT);
+        if (it >= hi - lo) {
+            if (--left == 0) break;
+            part = (part + 1 == QSPLIT) ? 0 : part + 1;
+            continue;
+        }
+        const int2 w = items[lo + it];
         const int img = __builtin_amdgcn_readfirstlane(w.x);
         const int tile = __builtin_amdgcn_readfirstlane(w.y);
         const DImg &im = imgs[img];
@@ -1101,7 +1126,7 @@ int ytrc_dataset_upload(ytrc_ctx *ctx, int32_t n_bricks, const ytrc_brick_desc *
         h.vertex = d.vertex_centred ? 1 : 0;
         const int64_t np = (int64_t)(d.dims[0] + h.vertex) * (d.dims[1] + h.vertex) *
                            (d.dims[2] + h.vertex);
-        if (np >= (int64_t)1 << 31)
+        if (stored_points(h) >= (int64_t)1 << 31)
             return fail(YTRC_EINVAL, "ytrc_dataset_upload: brick " + std::to_string(b) +
                                          " exceeds 2^31 points (int32 cell index, image_samplers.pyx:341)");
         // The kernels form cell and vertex offsets with 24-bit multiplies.
@@ -1110,7 +1135,7 @@ int ytrc_dataset_upload(ytrc_ctx *ctx, int32_t n_bricks, const ytrc_brick_desc *
             return fail(YTRC_EINVAL, "ytrc_dataset_upload: brick " + std::to_string(b) +
                                          " has a face of 2^24 or more points; split it into smaller bricks");
         h.data_off = pool_floats;
-        pool_floats += align_up((size_t)np * ds.nfp, 64);
+        pool_floats += align_up((size_t)stored_points(h) * ds.nfp, 64);
         bool has_mask = mask_ptrs != nullptr && mask_ptrs[b] != nullptr;
         if (has_mask) {
             // An all-ones mask is the same as none: skip its per-visit load.
@@ -1138,17 +1163,27 @@ int ytrc_dataset_upload(ytrc_ctx *ctx, int32_t n_bricks, const ytrc_brick_desc *
         const DBrick &h = ds.h_bricks[b];
         const int64_t np = (int64_t)(h.dims[0] + h.vertex) * (h.dims[1] + h.vertex) *
                            (h.dims[2] + h.vertex);
-        stage.assign((size_t)np * ds.nfp, 0.0f);
+        stage.assign((size_t)stored_points(h) * ds.nfp, 0.0f);
+        // Destination point of each source point: C order for vertex-centred
+        // bricks, 4x4x4 blocks for cell-centred ones.
+        std::vector<int64_t> dst;
+        if (!h.vertex) {
+            dst.resize((size_t)np);
+            const int nbj = (h.dims[1] + 3) >> 2, nbk = (h.dims[2] + 3) >> 2;
+            int64_t p = 0;
+            for (int i = 0; i < h.dims[0]; i++)
+                for (int j = 0; j < h.dims[1]; j++)
+                    for (int k = 0; k < h.dims[2]; k++)
+                        dst[(size_t)p++] = blocked_offset(i, j, k, nbj, nbk);
+        }
         for (int f = 0; f < n_fields; f++) {
             const void *src = field_ptrs[(size_t)b * n_fields + f];
             if (src == nullptr)
                 return fail(YTRC_EINVAL, "ytrc_dataset_upload: NULL field pointer");
-            if (src_dtype == 0) {
-                const double *s = (const double *)src;
-                for (int64_t p = 0; p < np; p++) stage[(size_t)p * ds.nfp + f] = (float)s[p];
-            } else {
-                const float *s = (const float *)src;
-                for (int64_t p = 0; p < np; p++) stage[(size_t)p * ds.nfp + f] = s[p];
+            for (int64_t p = 0; p < np; p++) {
+                const float v = (src_dtype == 0) ? (float)((const double *)src)[p] : ((const float *)src)[p];
+                const int64_t q = h.vertex ? p : dst[(size_t)p];
+                stage[(size_t)q * ds.nfp + f] = v;
             }
         }
         HIPCHK(hipMemcpy(ds.d_pool + h.data_off, stage.data(), stage.size() * 4,
@@ -1611,7 +1646,7 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
         gitems.push_back(std::move(items));
     }
     const size_t counters_off = off;
-    off = align_up(off + sizeof(unsigned int) * std::max<size_t>(1, B->groups.size()), 256);
+    off = align_up(off + sizeof(unsigned int) * QSPLIT * std::max<size_t>(1, B->groups.size()), 256);
     const size_t imgs_off = off;
     off = align_up(off + sizeof(DImg) * n_req, 256);
     if (ctx->budget != 0 && ctx->used + off > ctx->budget)
@@ -1678,7 +1713,8 @@ namespace {
 int enqueue_persistent(ytrc_batch *b, hipStream_t s)
 {
     const int ng = (int)b->groups.size();
-    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, b->counters, ng);
+    hipLaunchKernelGGL(k_zero_counters, dim3((ng * QSPLIT + 255) / 256), dim3(256), 0, s, b->counters,
+                       ng * QSPLIT);
     HIPCHK(hipGetLastError());
     const int threads = 64 * b->waves_per_group;
     for (auto &g : b->groups) {
@@ -1692,7 +1728,8 @@ int enqueue_persistent(ytrc_batch *b, hipStream_t s)
         g.grid = std::max(1, std::min(per_cu * b->ctx->n_cu,
                                       (g.n_items + b->waves_per_group - 1) / b->waves_per_group));
         hipLaunchKernelGGL(g.fn, dim3(g.grid), dim3(threads), g.lds, s, (const DImg *)b->d_imgs,
-                           g.items, g.n_items, b->counters + g.counter, b->tf_all, b->tf_bins);
+                           g.items, g.n_items, b->counters + (size_t)g.counter * QSPLIT, b->tf_all,
+                           b->tf_bins);
         HIPCHK(hipGetLastError());
     }
     return YTRC_OK;
@@ -1958,6 +1995,7 @@ struct SrvChunk {
 };
 
 constexpr int SRV_CHUNKS = 3;
+constexpr int SRV_MAX_VARIANTS = 64;  // distinct persistent kernels one server launches
 constexpr int SRV_TF_CAP = 3072;  // float4 entries of the server's table (48 KiB)
 
 }  // namespace
@@ -1998,6 +2036,7 @@ int srv_variant(ytrc_server *S, const PrepReq &P)
     auto key = std::make_tuple(P.q.r.kind, P.q.nfp, rec ? 1 : 0, P.q.mode);
     for (size_t i = 0; i < S->vkeys.size(); i++)
         if (S->vkeys[i] == key) return (int)i;
+    if ((int)S->vkeys.size() >= SRV_MAX_VARIANTS) return -1;
     S->vkeys.push_back(key);
     S->vfns.push_back(pick_persist(P.q.r.kind, P.q.nfp, rec, P.q.mode));
     return (int)S->vkeys.size() - 1;
@@ -2099,7 +2138,8 @@ int srv_pump(ytrc_server *S)
             for (auto &it : per[v]) C.h_items[n++] = it;
         }
         HIPCHK(hipMemcpyAsync(C.d_items, C.h_items, sizeof(int2) * n, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, C.d_counters, 64);
+        hipLaunchKernelGGL(k_zero_counters, dim3((SRV_MAX_VARIANTS * QSPLIT + 255) / 256), dim3(256), 0, s,
+                           C.d_counters, SRV_MAX_VARIANTS * QSPLIT);
         HIPCHK(hipGetLastError());
         for (int v = 0; v < nv; v++) {
             if (vcnt[v] == 0) continue;
@@ -2109,7 +2149,7 @@ int srv_pump(ytrc_server *S)
             HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, S->vfns[v], 256, lds));
             const int grid = std::max(1, std::min(std::max(1, nb) * S->ctx->n_cu, (vcnt[v] + 3) / 4));
             hipLaunchKernelGGL(S->vfns[v], dim3(grid), dim3(256), lds, s, (const DImg *)S->d_slots,
-                               (const int2 *)(C.d_items + voff[v]), vcnt[v], C.d_counters + v,
+                               (const int2 *)(C.d_items + voff[v]), vcnt[v], C.d_counters + (size_t)v * QSPLIT,
                                (const float4 *)S->d_tf, vr ? S->tf_hi : 0);
             HIPCHK(hipGetLastError());
         }
@@ -2196,7 +2236,7 @@ int ytrc_server_create(ytrc_ctx *ctx, int32_t max_requests, uint64_t budget_byte
         if (e != hipSuccess) return bad(e, "hipHostMalloc(items)");
         e = hipMalloc(&c.d_items, sizeof(int2) * chunk_tiles);
         if (e != hipSuccess) return bad(e, "hipMalloc(items)");
-        e = hipMalloc(&c.d_counters, sizeof(unsigned int) * 64);
+        e = hipMalloc(&c.d_counters, sizeof(unsigned int) * SRV_MAX_VARIANTS * QSPLIT);
         if (e != hipSuccess) return bad(e, "hipMalloc(counters)");
         e = hipEventCreateWithFlags(&c.done, hipEventDisableTiming);
         if (e != hipSuccess) return bad(e, "hipEventCreate");
@@ -2250,6 +2290,10 @@ int ytrc_server_submit(ytrc_server *S, const ytrc_request *r, int64_t *id_out)
         return rc;
     }
     R.variant = srv_variant(S, R.P);
+    if (R.variant < 0) {
+        srv_release(S, R);
+        return fail(YTRC_ENOTIMPL, "server: too many distinct kernel variants");
+    }
     R.id = S->next_id++;
     R.P.q.r.order = nullptr;  // caller-owned pointers are not kept
     R.P.q.r.zbuffer = nullptr;

@@ -76,6 +76,23 @@ struct alignas(16) DImg {
     double inv_ns;
 };
 
+// Cell-centred bricks (the projection path) are stored in blocks of 4x4x4
+// cells: blocks in C order over the brick padded to whole blocks, cells in C
+// order inside a block (256 B per block and field).  A ray bundle walking
+// through a block touches two 128-B lines instead of up to 16 rows of the C
+// order.  Vertex-centred bricks stay in C order.
+__host__ __device__ inline int64_t blocked_offset(int i, int j, int k, int nbj, int nbk)
+{
+    const int64_t blk = ((int64_t)(i >> 2) * nbj + (j >> 2)) * nbk + (k >> 2);
+    return (blk << 6) | ((i & 3) << 4) | ((j & 3) << 2) | (k & 3);
+}
+
+inline int64_t stored_points(const DBrick &h)
+{
+    if (h.vertex) return (int64_t)(h.dims[0] + 1) * (h.dims[1] + 1) * (h.dims[2] + 1);
+    return (int64_t)((h.dims[0] + 3) >> 2) * ((h.dims[1] + 3) >> 2) * ((h.dims[2] + 3) >> 2) * 64;
+}
+
 // One 64-lane wavefront casts one tile of 64 rays: 8x8 by default, 4x16 or
 // 16x4 when a batch is built with another shape.
 constexpr int TILE_SHIFT_DEFAULT = 3;
