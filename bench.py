@@ -276,7 +276,7 @@ def main():
         byt = sum(alg_bytes(kind, samples[i], npix[i]) for i in idx)
         kinds[kind] = dict(ms=ms, bytes=byt, samples=sum(samples[i] for i in idx))
         log(f"{'projection' if kind == 0 else 'VR'} persistent kernel: {ms:.3f} ms, "
-            f"{byt / ms / 1e6:.1f} GB/s algorithmic")
+            f"{kinds[kind]['samples']:.4e} samples, {byt / ms / 1e6:.1f} GB/s algorithmic")
     dom = max(kinds, key=lambda k: kinds[k]["ms"])
     ach = kinds[dom]["bytes"] / kinds[dom]["ms"] / 1e6
     traffic = measured_traffic(args.traffic_file, dom, args)
@@ -366,6 +366,8 @@ def main():
             "frac": round(ach / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "kernel_ms": round(kinds[dom]["ms"], 4),
+            "algorithmic_bytes": int(kinds[dom]["bytes"]),
+            "samples": int(kinds[dom]["samples"]),
             "copy_probe_gbps": round(copy_gbps, 1),
         },
         "cpu_baseline": cpu,
