@@ -87,6 +87,42 @@ typedef struct {
 int ytrc_dataset_upload(ytrc_ctx *ctx, int32_t n_bricks, const ytrc_brick_desc *bricks,
                         int32_t n_fields, int32_t src_dtype, const void *const *field_ptrs,
                         const uint8_t *const *mask_ptrs, int32_t *ds_id_out);
+/* Device brick preparation for VolumeRenderSampler: AMRKDTree.get_brick_data
+ * (amr_kdtree.py:325-374) run on the GPU.  The caller passes each grid's
+ * cell-centred data once (n_grids*n_fields host pointers, grid-major, C-order
+ * dims^3 arrays of src_dtype) and the kd leaves (the grid each lies in, its
+ * cell slice [li, ri) and its edges); the library computes every leaf's
+ * vertex data with no-ghost vertex centring (grid_patch.py:270-296,
+ * interpolators.pyx:177-226) in FP64, applies log10 with v <= 0 -> NaN to
+ * the fields flagged in log_fields (amr_kdtree.py:346-352), and stores the
+ * leaves as vertex-centred bricks, in leaf order, exactly as
+ * ytrc_dataset_upload would store the host-computed bricks.  mask_ptrs:
+ * NULL or n_leaves child-mask slices (dims^3 uint8, NULL = all ones).
+ * Grids need at least 2 cells per axis. */
+typedef struct {
+    double left_edge[3];
+    double right_edge[3];
+    int32_t dims[3];
+    int32_t pad;
+} ytrc_grid_desc;
+typedef struct {
+    int32_t grid;
+    int32_t li[3];
+    int32_t ri[3];
+    int32_t pad;
+    double left_edge[3];
+    double right_edge[3];
+} ytrc_leaf_desc;
+int ytrc_dataset_build_vr(ytrc_ctx *ctx, int32_t n_grids, const ytrc_grid_desc *grids,
+                          int32_t n_fields, int32_t src_dtype, const void *const *cell_ptrs,
+                          const int32_t *log_fields, int32_t n_leaves, const ytrc_leaf_desc *leaves,
+                          const uint8_t *const *mask_ptrs, int32_t *ds_id_out);
+/* Copies brick `brick`'s stored points (float32, fields interleaved, in
+ * storage order) to out; *n_out gets the float count (out may be NULL to
+ * query it).  No counterpart in the reference: used to check the device
+ * brick preparation against the host's. */
+int ytrc_dataset_read_points(ytrc_ctx *ctx, int32_t ds_id, int32_t brick, float *out, int64_t cap,
+                             int64_t *n_out);
 int ytrc_dataset_free(ytrc_ctx *ctx, int32_t ds_id);
 int ytrc_dataset_bytes(const ytrc_ctx *ctx, int32_t ds_id, uint64_t *bytes);
 

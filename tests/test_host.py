@@ -173,3 +173,21 @@ def test_autotune_signature_and_cache(tmp_path):
     autotune._store(path, {"k": {"cfg": autotune.DEFAULT}})
     assert autotune._load(path)["k"]["cfg"] == autotune.DEFAULT
     assert autotune._load(str(tmp_path / "missing.json")) == {}
+
+
+def test_leaf_slice_matches_brick_data():
+    """The device-prep inputs (leaf_slice) are the slices get_brick_data
+    takes (amr_kdtree.py:325-374)."""
+    from tests.helpers import small_amr
+    from yt_amd.amr_kdtree import AMRKDTree
+
+    ds = small_amr(root_n=16, levels=2, seed=5)
+    kd = AMRKDTree(ds)
+    kd.set_fields(["density"], [True])
+    for node in kd.leaves():
+        grid, li, ri, nle, nre, mask = kd.leaf_slice(node)
+        pg = kd.get_brick_data(node)
+        assert np.array_equal(ri - li, pg.dims)
+        assert np.array_equal(nle, pg.LeftEdge) and np.array_equal(nre, pg.RightEdge)
+        assert np.array_equal(mask, pg.source_mask)
+        node.data = None

@@ -62,6 +62,8 @@ EXPORTED = [
     "ytrc_server_cancel",
     "ytrc_server_stats",
     "ytrc_server_destroy",
+    "ytrc_dataset_build_vr",
+    "ytrc_dataset_read_points",
 ]
 
 
@@ -85,6 +87,26 @@ class BrickDesc(C.Structure):
         ("right_edge", C.c_double * 3),
         ("dims", C.c_int32 * 3),
         ("vertex_centred", C.c_int32),
+    ]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [
+        ("left_edge", C.c_double * 3),
+        ("right_edge", C.c_double * 3),
+        ("dims", C.c_int32 * 3),
+        ("pad", C.c_int32),
+    ]
+
+
+class LeafDesc(C.Structure):
+    _fields_ = [
+        ("grid", C.c_int32),
+        ("li", C.c_int32 * 3),
+        ("ri", C.c_int32 * 3),
+        ("pad", C.c_int32),
+        ("left_edge", C.c_double * 3),
+        ("right_edge", C.c_double * 3),
     ]
 
 
@@ -140,6 +162,12 @@ def _declare(lib):
     lib.ytrc_dataset_upload.argtypes = [
         P, C.c_int32, C.POINTER(BrickDesc), C.c_int32, C.c_int32,
         C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_int32)]
+    lib.ytrc_dataset_build_vr.argtypes = [
+        P, C.c_int32, C.POINTER(GridDesc), C.c_int32, C.c_int32, C.POINTER(C.c_void_p),
+        C.POINTER(C.c_int32), C.c_int32, C.POINTER(LeafDesc), C.POINTER(C.c_void_p),
+        C.POINTER(C.c_int32)]
+    lib.ytrc_dataset_read_points.argtypes = [
+        P, C.c_int32, C.c_int32, C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_int64)]
     lib.ytrc_dataset_free.argtypes = [P, C.c_int32]
     lib.ytrc_dataset_bytes.argtypes = [P, C.c_int32, C.POINTER(C.c_uint64)]
     lib.ytrc_batch_create.argtypes = [P, C.c_int32, C.POINTER(Request), C.c_uint32, C.POINTER(P)]
@@ -287,6 +315,72 @@ class Context:
             self._h, n, descs, nf, 0 if dt == np.float64 else 1, ptrs,
             C.cast(mptr, C.POINTER(C.c_void_p)) if mptr is not None else None, C.byref(ds)))
         return ds.value
+
+    def build_vr(self, grids, cells, log_fields, leaves, masks=None):
+        """Device brick preparation (ytrc_dataset_build_vr).  grids: list of
+        (left_edge, right_edge, dims); cells: per grid, a list of C-order
+        dims-shaped cell arrays (one per field, all float64 or all float32);
+        log_fields: per field, log10 or not; leaves: list of (grid index,
+        li, ri, left_edge, right_edge); masks: None or per-leaf uint8 arrays
+        of shape ri - li (None = all ones)."""
+        ng, nl = len(grids), len(leaves)
+        if ng == 0 or nl == 0:
+            raise ValueError("build_vr: no grids or no leaves")
+        nf = len(cells[0])
+        gd = (GridDesc * ng)()
+        keep = []
+        ptrs = (C.c_void_p * (ng * nf))()
+        dt = None
+        for g, (le, re, dims) in enumerate(grids):
+            gd[g].left_edge[:] = [float(x) for x in le]
+            gd[g].right_edge[:] = [float(x) for x in re]
+            gd[g].dims[:] = [int(x) for x in dims]
+            if len(cells[g]) != nf:
+                raise ValueError("build_vr: every grid needs the same number of fields")
+            for f, arr in enumerate(cells[g]):
+                a = np.asarray(arr)
+                if dt is None:
+                    dt = np.float32 if a.dtype == np.float32 else np.float64
+                a = np.ascontiguousarray(a, dtype=dt)
+                if a.shape != tuple(int(d) for d in dims):
+                    raise ValueError(f"build_vr: grid {g} field {f} has shape {a.shape}")
+                keep.append(a)
+                ptrs[g * nf + f] = a.ctypes.data
+        ld = (LeafDesc * nl)()
+        for l, (g, li, ri, le, re) in enumerate(leaves):
+            ld[l].grid = int(g)
+            ld[l].li[:] = [int(x) for x in li]
+            ld[l].ri[:] = [int(x) for x in ri]
+            ld[l].left_edge[:] = [float(x) for x in le]
+            ld[l].right_edge[:] = [float(x) for x in re]
+        mptr = None
+        if masks is not None:
+            mptr = (C.c_void_p * nl)()
+            for l, m in enumerate(masks):
+                if m is None:
+                    continue
+                mm = np.ascontiguousarray(m, dtype=np.uint8)
+                want = tuple(int(r) - int(i) for i, r in zip(leaves[l][1], leaves[l][2]))
+                if mm.shape != want:
+                    raise ValueError(f"build_vr: mask {l} has shape {mm.shape}, expected {want}")
+                keep.append(mm)
+                mptr[l] = mm.ctypes.data
+        logs = (C.c_int32 * nf)(*[int(bool(x)) for x in log_fields])
+        ds = C.c_int32()
+        _check(lib().ytrc_dataset_build_vr(
+            self._h, ng, gd, nf, 0 if dt == np.float64 else 1, ptrs, logs, nl, ld,
+            C.cast(mptr, C.POINTER(C.c_void_p)) if mptr is not None else None, C.byref(ds)))
+        return ds.value
+
+    def read_points(self, ds_id, brick):
+        """A brick's stored float32 points, fields interleaved, storage order."""
+        n = C.c_int64()
+        _check(lib().ytrc_dataset_read_points(self._h, int(ds_id), int(brick), None, 0, C.byref(n)))
+        out = np.empty(n.value, dtype=np.float32)
+        _check(lib().ytrc_dataset_read_points(
+            self._h, int(ds_id), int(brick), out.ctypes.data_as(C.POINTER(C.c_float)), n.value,
+            C.byref(n)))
+        return out
 
     def free(self, ds_id):
         _check(lib().ytrc_dataset_free(self._h, int(ds_id)))

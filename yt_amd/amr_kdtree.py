@@ -204,6 +204,21 @@ class AMRKDTree:
             self._vcd_cache[key] = dds
         return self._vcd_cache[key]
 
+    def leaf_slice(self, node):
+        """The grid, cell slice [li, ri), edges and child-mask slice of a
+        leaf, as get_brick_data (amr_kdtree.py:325-374) derives them, without
+        the vertex data: the inputs of device brick preparation."""
+        grid = self._grid_by_id[node.grid]
+        gle = grid.LeftEdge
+        nle = node.get_left_edge()
+        nre = node.get_right_edge()
+        li = np.rint((nle - gle) / grid.dds).astype("int32")
+        ri = np.rint((nre - gle) / grid.dds).astype("int32")
+        assert np.all(grid.LeftEdge <= nle)
+        assert np.all(grid.RightEdge >= nre)
+        mask = grid.child_mask[li[0]:ri[0], li[1]:ri[1], li[2]:ri[2]].astype("uint8")
+        return grid, li, ri, nle.copy(), nre.copy(), mask
+
     def get_brick_data(self, node):
         """amr_kdtree.py:325-374: the leaf clipped to its grid, vertex data
         sliced [li:ri+1], mask = child mask slice."""

@@ -813,6 +813,71 @@ __global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src,
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+// Device brick preparation: the vertex data of one kd leaf, computed from its
+// grid's cell data (AMRKDTree.get_brick_data, amr_kdtree.py:325-374, with
+// Grid.get_vertex_centered_data(no_ghost=True), grid_patch.py:270-296).
+struct VertexJob {
+    int gdims[3];  // the grid's cells per axis
+    int li[3];     // the leaf's first cell in the grid
+    int vd[3];     // the leaf's vertices per axis (dims + 1)
+    int nf, nfp, logmask;
+    int64_t n;     // vertices in the leaf
+    float *out;    // the leaf's points in the pool (nfp floats per point)
+};
+
+// Vertex o along an axis of n cells, as ghost_zone_interpolate
+// (interpolators.pyx:177-226) places it with rf = 1, input_left = 0.5,
+// output_left = 0: cells ii and ii + 1 with weights xm and xp (linear
+// extrapolation on the faces).
+__device__ __forceinline__ void vc_axis(int o, int n, int &ii, double &xp, double &xm)
+{
+    const double ropos = ((double)o - 0.5) * 1.0;
+    int i = (int)ropos;  // truncation toward zero
+    i = i < 0 ? 0 : (i > n - 2 ? n - 2 : i);
+    ii = i;
+    xp = ropos - (double)i;
+    xm = 1.0 - xp;
+}
+
+// One thread per leaf vertex: the 8-cell weighted sum in FP64 in the
+// reference's term order, then log10 with v <= 0 -> NaN where the field is a
+// log field (amr_kdtree.py:346-352), stored as float32.
+template <class T>
+__global__ __launch_bounds__(256) void k_vertex_centre(VertexJob J, const T *__restrict__ cells, int64_t nc)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= J.n) return;
+    const int c = (int)(p % J.vd[2]);
+    const int64_t q = p / J.vd[2];
+    const int b = (int)(q % J.vd[1]);
+    const int a = (int)(q / J.vd[1]);
+    int ii, ij, ik;
+    double xp, xm, yp, ym, zp, zm;
+    vc_axis(J.li[0] + a, J.gdims[0], ii, xp, xm);
+    vc_axis(J.li[1] + b, J.gdims[1], ij, yp, ym);
+    vc_axis(J.li[2] + c, J.gdims[2], ik, zp, zm);
+    const int64_t s1 = J.gdims[2];
+    const int64_t s0 = (int64_t)J.gdims[1] * J.gdims[2];
+    float *o = J.out + p * J.nfp;
+    for (int f = 0; f < J.nfp; f++) {
+        if (f >= J.nf) {
+            o[f] = 0.0f;
+            continue;
+        }
+        const T *g = cells + (int64_t)f * nc + ii * s0 + ij * s1 + ik;
+        double v = (double)g[0] * (xm * ym * zm);
+        v = v + (double)g[s0] * (xp * ym * zm);
+        v = v + (double)g[s1] * (xm * yp * zm);
+        v = v + (double)g[1] * (xm * ym * zp);
+        v = v + (double)g[s0 + 1] * (xp * ym * zp);
+        v = v + (double)g[s1 + 1] * (xm * yp * zp);
+        v = v + (double)g[s0 + s1] * (xp * yp * zm);
+        v = v + (double)g[s0 + s1 + 1] * (xp * yp * zp);
+        if ((J.logmask >> f) & 1) v = (v <= 0.0) ? __builtin_nan("") : log10(v);
+        o[f] = (float)v;
+    }
+}
+
 __global__ void k_zero_counters(unsigned int *c, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1034,6 +1099,114 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 }  // namespace
 
+namespace {
+
+void dataset_release(ytrc_ctx *ctx, Dataset &ds);
+
+// Validates brick descriptors, lays out the point pool (stored_points per
+// brick) and the masks, allocates device memory, and copies descriptors and
+// masks.  Field data is filled in by the caller.
+int dataset_alloc(ytrc_ctx *ctx, const char *who, int n_bricks, const ytrc_brick_desc *bricks,
+                  int n_fields, const uint8_t *const *mask_ptrs, Dataset &ds)
+{
+    const std::string w(who);
+    if (n_bricks <= 0) return fail(YTRC_EINVAL, w + ": n_bricks must be > 0");
+    if (n_fields <= 0 || n_fields > 6) return fail(YTRC_EINVAL, w + ": n_fields must be in 1..6");
+    ds.n_bricks = n_bricks;
+    ds.nf = n_fields;
+    ds.nfp = nfp_of(n_fields);
+    ds.h_bricks.resize(n_bricks);
+    int64_t pool_floats = 0, mask_bytes = 0;
+    for (int b = 0; b < n_bricks; b++) {
+        const ytrc_brick_desc &d = bricks[b];
+        DBrick &h = ds.h_bricks[b];
+        memset(&h, 0, sizeof(h));
+        for (int i = 0; i < 3; i++) {
+            if (d.dims[i] <= 0)
+                return fail(YTRC_EINVAL, w + ": brick " + std::to_string(b) + " has non-positive dims");
+            if (!(d.right_edge[i] > d.left_edge[i]))
+                return fail(YTRC_EINVAL, w + ": brick " + std::to_string(b) + " has right_edge <= left_edge");
+            h.le[i] = d.left_edge[i];
+            h.re[i] = d.right_edge[i];
+            h.dims[i] = d.dims[i];
+            // partitioned_grid.pyx:50-55
+            h.dds[i] = (h.re[i] - h.le[i]) / (double)d.dims[i];
+            h.idds[i] = 1.0 / h.dds[i];
+        }
+        h.vertex = d.vertex_centred ? 1 : 0;
+        if (stored_points(h) >= (int64_t)1 << 31)
+            return fail(YTRC_EINVAL, w + ": brick " + std::to_string(b) +
+                                         " exceeds 2^31 points (int32 cell index, image_samplers.pyx:341)");
+        // The kernels form cell and vertex offsets with 24-bit multiplies.
+        if ((int64_t)(d.dims[0] + 1) * (d.dims[1] + 1) >= (int64_t)1 << 24 ||
+            (int64_t)(d.dims[1] + 1) * (d.dims[2] + 1) >= (int64_t)1 << 24)
+            return fail(YTRC_EINVAL, w + ": brick " + std::to_string(b) +
+                                         " has a face of 2^24 or more points; split it into smaller bricks");
+        h.data_off = pool_floats;
+        pool_floats += align_up((size_t)stored_points(h) * ds.nfp, 64);
+        bool has_mask = mask_ptrs != nullptr && mask_ptrs[b] != nullptr;
+        if (has_mask) {
+            // An all-ones mask is the same as none: skip its per-visit load.
+            const size_t nc = (size_t)d.dims[0] * d.dims[1] * d.dims[2];
+            const uint8_t *m = mask_ptrs[b];
+            bool all1 = true;
+            for (size_t q = 0; q < nc && all1; q++) all1 = (m[q] == 1);
+            has_mask = !all1;
+        }
+        h.mask_off = has_mask ? mask_bytes : -1;
+        if (has_mask) mask_bytes += align_up((size_t)d.dims[0] * d.dims[1] * d.dims[2], 256);
+    }
+    const uint64_t bytes = (uint64_t)pool_floats * 4 + (uint64_t)mask_bytes +
+                           (uint64_t)n_bricks * sizeof(DBrick);
+    if (ctx->budget != 0 && ctx->used + bytes > ctx->budget)
+        return fail(YTRC_EBUDGET, w + ": " + std::to_string(bytes) + " bytes exceed the HBM budget");
+    auto dev = [&](hipError_t e, const char *what) {
+        dataset_release(ctx, ds);
+        return fail(YTRC_EDEVICE, w + ": " + what + ": " + hipGetErrorString(e));
+    };
+    hipError_t e = hipMalloc(&ds.d_pool, (size_t)pool_floats * 4);
+    if (e != hipSuccess) return dev(e, "hipMalloc(pool)");
+    if (mask_bytes > 0) {
+        e = hipMalloc(&ds.d_masks, (size_t)mask_bytes);
+        if (e != hipSuccess) return dev(e, "hipMalloc(masks)");
+    }
+    e = hipMalloc(&ds.d_bricks, (size_t)n_bricks * sizeof(DBrick));
+    if (e != hipSuccess) return dev(e, "hipMalloc(bricks)");
+    for (int b = 0; b < n_bricks; b++) {
+        const DBrick &h = ds.h_bricks[b];
+        if (h.mask_off < 0) continue;
+        e = hipMemcpy(ds.d_masks + h.mask_off, mask_ptrs[b], (size_t)h.dims[0] * h.dims[1] * h.dims[2],
+                      hipMemcpyHostToDevice);
+        if (e != hipSuccess) return dev(e, "hipMemcpy(mask)");
+    }
+    e = hipMemcpy(ds.d_bricks, ds.h_bricks.data(), (size_t)n_bricks * sizeof(DBrick), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return dev(e, "hipMemcpy(bricks)");
+    ds.bytes = bytes;
+    return YTRC_OK;
+}
+
+void dataset_release(ytrc_ctx *ctx, Dataset &ds)
+{
+    (void)ctx;
+    if (ds.d_bricks) hipFree(ds.d_bricks);
+    if (ds.d_pool) hipFree(ds.d_pool);
+    if (ds.d_masks) hipFree(ds.d_masks);
+    ds.d_bricks = nullptr;
+    ds.d_pool = nullptr;
+    ds.d_masks = nullptr;
+}
+
+int dataset_commit(ytrc_ctx *ctx, Dataset &ds, int32_t *ds_id_out)
+{
+    ctx->used += ds.bytes;
+    const int id = ctx->next_ds++;
+    ctx->datasets[id] = std::move(ds);
+    *ds_id_out = id;
+    return YTRC_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *ytrc_last_error(void) { return g_err.c_str(); }
@@ -1093,70 +1266,15 @@ int ytrc_dataset_upload(ytrc_ctx *ctx, int32_t n_bricks, const ytrc_brick_desc *
 {
     if (ctx == nullptr || bricks == nullptr || field_ptrs == nullptr || ds_id_out == nullptr)
         return fail(YTRC_EINVAL, "ytrc_dataset_upload: NULL argument");
-    if (n_bricks <= 0) return fail(YTRC_EINVAL, "ytrc_dataset_upload: n_bricks must be > 0");
-    if (n_fields <= 0 || n_fields > 6)
-        return fail(YTRC_EINVAL, "ytrc_dataset_upload: n_fields must be in 1..6");
     if (src_dtype != 0 && src_dtype != 1)
         return fail(YTRC_EINVAL, "ytrc_dataset_upload: src_dtype must be 0 (f64) or 1 (f32)");
+    for (int64_t q = 0; q < (int64_t)n_bricks * std::max(0, n_fields); q++)
+        if (field_ptrs[q] == nullptr) return fail(YTRC_EINVAL, "ytrc_dataset_upload: NULL field pointer");
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
     Dataset ds;
-    ds.n_bricks = n_bricks;
-    ds.nf = n_fields;
-    ds.nfp = nfp_of(n_fields);
-    ds.h_bricks.resize(n_bricks);
-    int64_t pool_floats = 0, mask_bytes = 0;
-    for (int b = 0; b < n_bricks; b++) {
-        const ytrc_brick_desc &d = bricks[b];
-        DBrick &h = ds.h_bricks[b];
-        for (int i = 0; i < 3; i++) {
-            if (d.dims[i] <= 0)
-                return fail(YTRC_EINVAL, "ytrc_dataset_upload: brick " + std::to_string(b) +
-                                             " has non-positive dims");
-            if (!(d.right_edge[i] > d.left_edge[i]))
-                return fail(YTRC_EINVAL, "ytrc_dataset_upload: brick " + std::to_string(b) +
-                                             " has right_edge <= left_edge");
-            h.le[i] = d.left_edge[i];
-            h.re[i] = d.right_edge[i];
-            h.dims[i] = d.dims[i];
-            // partitioned_grid.pyx:50-55
-            h.dds[i] = (h.re[i] - h.le[i]) / (double)d.dims[i];
-            h.idds[i] = 1.0 / h.dds[i];
-        }
-        h.vertex = d.vertex_centred ? 1 : 0;
-        const int64_t np = (int64_t)(d.dims[0] + h.vertex) * (d.dims[1] + h.vertex) *
-                           (d.dims[2] + h.vertex);
-        if (stored_points(h) >= (int64_t)1 << 31)
-            return fail(YTRC_EINVAL, "ytrc_dataset_upload: brick " + std::to_string(b) +
-                                         " exceeds 2^31 points (int32 cell index, image_samplers.pyx:341)");
-        // The kernels form cell and vertex offsets with 24-bit multiplies.
-        if ((int64_t)(d.dims[0] + 1) * (d.dims[1] + 1) >= (int64_t)1 << 24 ||
-            (int64_t)(d.dims[1] + 1) * (d.dims[2] + 1) >= (int64_t)1 << 24)
-            return fail(YTRC_EINVAL, "ytrc_dataset_upload: brick " + std::to_string(b) +
-                                         " has a face of 2^24 or more points; split it into smaller bricks");
-        h.data_off = pool_floats;
-        pool_floats += align_up((size_t)stored_points(h) * ds.nfp, 64);
-        bool has_mask = mask_ptrs != nullptr && mask_ptrs[b] != nullptr;
-        if (has_mask) {
-            // An all-ones mask is the same as none: skip its per-visit load.
-            const size_t nc = (size_t)d.dims[0] * d.dims[1] * d.dims[2];
-            const uint8_t *m = mask_ptrs[b];
-            bool all1 = true;
-            for (size_t q = 0; q < nc && all1; q++) all1 = (m[q] == 1);
-            has_mask = !all1;
-        }
-        h.mask_off = has_mask ? mask_bytes : -1;
-        if (has_mask)
-            mask_bytes += align_up((size_t)d.dims[0] * d.dims[1] * d.dims[2], 256);
-    }
-    const uint64_t bytes = (uint64_t)pool_floats * 4 + (uint64_t)mask_bytes +
-                           (uint64_t)n_bricks * sizeof(DBrick);
-    if (ctx->budget != 0 && ctx->used + bytes > ctx->budget)
-        return fail(YTRC_EBUDGET, "ytrc_dataset_upload: " + std::to_string(bytes) +
-                                      " bytes exceed the HBM budget");
-    HIPCHK(hipMalloc(&ds.d_pool, (size_t)pool_floats * 4));
-    if (mask_bytes > 0) HIPCHK(hipMalloc(&ds.d_masks, (size_t)mask_bytes));
-    HIPCHK(hipMalloc(&ds.d_bricks, (size_t)n_bricks * sizeof(DBrick)));
+    int rc = dataset_alloc(ctx, "ytrc_dataset_upload", n_bricks, bricks, n_fields, mask_ptrs, ds);
+    if (rc != YTRC_OK) return rc;
     // Interleave fields per point, convert to float32, stage brick by brick.
     std::vector<float> stage;
     for (int b = 0; b < n_bricks; b++) {
@@ -1178,28 +1296,145 @@ int ytrc_dataset_upload(ytrc_ctx *ctx, int32_t n_bricks, const ytrc_brick_desc *
         }
         for (int f = 0; f < n_fields; f++) {
             const void *src = field_ptrs[(size_t)b * n_fields + f];
-            if (src == nullptr)
-                return fail(YTRC_EINVAL, "ytrc_dataset_upload: NULL field pointer");
             for (int64_t p = 0; p < np; p++) {
                 const float v = (src_dtype == 0) ? (float)((const double *)src)[p] : ((const float *)src)[p];
                 const int64_t q = h.vertex ? p : dst[(size_t)p];
                 stage[(size_t)q * ds.nfp + f] = v;
             }
         }
-        HIPCHK(hipMemcpy(ds.d_pool + h.data_off, stage.data(), stage.size() * 4,
-                         hipMemcpyHostToDevice));
-        if (h.mask_off >= 0) {
-            HIPCHK(hipMemcpy(ds.d_masks + h.mask_off, mask_ptrs[b],
-                             (size_t)h.dims[0] * h.dims[1] * h.dims[2], hipMemcpyHostToDevice));
+        const hipError_t e = hipMemcpy(ds.d_pool + h.data_off, stage.data(), stage.size() * 4,
+                                       hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            dataset_release(ctx, ds);
+            return fail(YTRC_EDEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
         }
     }
-    HIPCHK(hipMemcpy(ds.d_bricks, ds.h_bricks.data(), (size_t)n_bricks * sizeof(DBrick),
-                     hipMemcpyHostToDevice));
-    ds.bytes = bytes;
-    ctx->used += bytes;
-    const int id = ctx->next_ds++;
-    ctx->datasets[id] = std::move(ds);
-    *ds_id_out = id;
+    return dataset_commit(ctx, ds, ds_id_out);
+}
+
+int ytrc_dataset_build_vr(ytrc_ctx *ctx, int32_t n_grids, const ytrc_grid_desc *grids,
+                          int32_t n_fields, int32_t src_dtype, const void *const *cell_ptrs,
+                          const int32_t *log_fields, int32_t n_leaves, const ytrc_leaf_desc *leaves,
+                          const uint8_t *const *mask_ptrs, int32_t *ds_id_out)
+{
+    const char *who = "ytrc_dataset_build_vr";
+    if (ctx == nullptr || grids == nullptr || cell_ptrs == nullptr || leaves == nullptr ||
+        ds_id_out == nullptr)
+        return fail(YTRC_EINVAL, std::string(who) + ": NULL argument");
+    if (n_grids <= 0) return fail(YTRC_EINVAL, std::string(who) + ": n_grids must be > 0");
+    if (src_dtype != 0 && src_dtype != 1)
+        return fail(YTRC_EINVAL, std::string(who) + ": src_dtype must be 0 (f64) or 1 (f32)");
+    if (n_fields <= 0 || n_fields > 6)
+        return fail(YTRC_EINVAL, std::string(who) + ": n_fields must be in 1..6");
+    // Grid cell data, back to back per (grid, field), in the source dtype.
+    const size_t esz = src_dtype == 0 ? 8 : 4;
+    std::vector<int64_t> goff((size_t)n_grids);
+    int64_t cells = 0;
+    for (int g = 0; g < n_grids; g++) {
+        for (int a = 0; a < 3; a++)
+            if (grids[g].dims[a] < 2)
+                return fail(YTRC_EINVAL, std::string(who) + ": grid " + std::to_string(g) +
+                                             " needs at least 2 cells per axis (interpolators.pyx:177-226)");
+        goff[(size_t)g] = cells;
+        cells += (int64_t)grids[g].dims[0] * grids[g].dims[1] * grids[g].dims[2] * n_fields;
+    }
+    for (int64_t q = 0; q < (int64_t)n_grids * n_fields; q++)
+        if (cell_ptrs[q] == nullptr) return fail(YTRC_EINVAL, std::string(who) + ": NULL cell pointer");
+    // Leaves become vertex-centred brick descriptors (PartitionedGrid(...,
+    // node edges, dims = ri - li), amr_kdtree.py:360-367).
+    std::vector<ytrc_brick_desc> bd((size_t)std::max(0, n_leaves));
+    for (int l = 0; l < n_leaves; l++) {
+        const ytrc_leaf_desc &L = leaves[l];
+        if (L.grid < 0 || L.grid >= n_grids)
+            return fail(YTRC_EINVAL, std::string(who) + ": leaf " + std::to_string(l) + " names no grid");
+        for (int a = 0; a < 3; a++) {
+            if (L.li[a] < 0 || L.ri[a] > grids[L.grid].dims[a] || L.ri[a] <= L.li[a])
+                return fail(YTRC_EINVAL, std::string(who) + ": leaf " + std::to_string(l) +
+                                             " slice lies outside its grid");
+            bd[(size_t)l].left_edge[a] = L.left_edge[a];
+            bd[(size_t)l].right_edge[a] = L.right_edge[a];
+            bd[(size_t)l].dims[a] = L.ri[a] - L.li[a];
+        }
+        bd[(size_t)l].vertex_centred = 1;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    Dataset ds;
+    int rc = dataset_alloc(ctx, who, n_leaves, bd.data(), n_fields, mask_ptrs, ds);
+    if (rc != YTRC_OK) return rc;
+    void *d_cells = nullptr;
+    hipError_t e = hipMalloc(&d_cells, (size_t)cells * esz);
+    if (e != hipSuccess) {
+        dataset_release(ctx, ds);
+        return fail(YTRC_EDEVICE, std::string("hipMalloc(cells): ") + hipGetErrorString(e));
+    }
+    auto bail = [&](hipError_t err, const char *what) {
+        hipFree(d_cells);
+        dataset_release(ctx, ds);
+        return fail(YTRC_EDEVICE, std::string(what) + ": " + hipGetErrorString(err));
+    };
+    for (int g = 0; g < n_grids; g++) {
+        const size_t nc = (size_t)grids[g].dims[0] * grids[g].dims[1] * grids[g].dims[2];
+        for (int f = 0; f < n_fields; f++) {
+            e = hipMemcpyAsync((char *)d_cells + ((size_t)goff[(size_t)g] + (size_t)f * nc) * esz,
+                               cell_ptrs[(size_t)g * n_fields + f], nc * esz, hipMemcpyHostToDevice,
+                               ctx->stream);
+            if (e != hipSuccess) return bail(e, "hipMemcpyAsync(cells)");
+        }
+    }
+    int logmask = 0;
+    for (int f = 0; f < n_fields; f++)
+        if (log_fields != nullptr && log_fields[f]) logmask |= 1 << f;
+    for (int l = 0; l < n_leaves; l++) {
+        const ytrc_leaf_desc &L = leaves[l];
+        const DBrick &h = ds.h_bricks[(size_t)l];
+        VertexJob J;
+        for (int a = 0; a < 3; a++) {
+            J.gdims[a] = grids[L.grid].dims[a];
+            J.li[a] = L.li[a];
+            J.vd[a] = h.dims[a] + 1;
+        }
+        J.nf = n_fields;
+        J.nfp = ds.nfp;
+        J.logmask = logmask;
+        J.n = (int64_t)J.vd[0] * J.vd[1] * J.vd[2];
+        J.out = ds.d_pool + h.data_off;
+        const int64_t nc = (int64_t)J.gdims[0] * J.gdims[1] * J.gdims[2];
+        const unsigned grid = (unsigned)((J.n + 255) / 256);
+        if (src_dtype == 0)
+            hipLaunchKernelGGL(k_vertex_centre<double>, dim3(grid), dim3(256), 0, ctx->stream, J,
+                               (const double *)d_cells + goff[(size_t)L.grid], nc);
+        else
+            hipLaunchKernelGGL(k_vertex_centre<float>, dim3(grid), dim3(256), 0, ctx->stream, J,
+                               (const float *)d_cells + goff[(size_t)L.grid], nc);
+        e = hipGetLastError();
+        if (e != hipSuccess) return bail(e, "k_vertex_centre");
+    }
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return bail(e, "hipStreamSynchronize");
+    hipFree(d_cells);
+    return dataset_commit(ctx, ds, ds_id_out);
+}
+
+int ytrc_dataset_read_points(ytrc_ctx *ctx, int32_t ds_id, int32_t brick, float *out, int64_t cap,
+                             int64_t *n_out)
+{
+    if (ctx == nullptr || n_out == nullptr) return fail(YTRC_EINVAL, "ytrc_dataset_read_points: NULL");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->datasets.find(ds_id);
+    if (it == ctx->datasets.end())
+        return fail(YTRC_EINVAL, "ytrc_dataset_read_points: unknown dataset " + std::to_string(ds_id));
+    const Dataset &ds = it->second;
+    if (brick < 0 || brick >= ds.n_bricks)
+        return fail(YTRC_EINVAL, "ytrc_dataset_read_points: brick index out of range");
+    const DBrick &h = ds.h_bricks[(size_t)brick];
+    const int64_t n = stored_points(h) * ds.nfp;
+    *n_out = n;
+    if (out == nullptr) return YTRC_OK;
+    if (cap < n) return fail(YTRC_EINVAL, "ytrc_dataset_read_points: output too small");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(out, ds.d_pool + h.data_off, (size_t)n * 4, hipMemcpyDeviceToHost));
     return YTRC_OK;
 }
 

@@ -16,7 +16,13 @@ from .camera import off_axis_camera
 
 
 class ResidentVolume:
-    def __init__(self, ctx, ds, proj_fields=None, weight=None, vr_field=None, log_field=True):
+    """vr_prep: "device" computes the kd bricks' vertex data on the GPU from
+    each grid's cells (ytrc_dataset_build_vr); "host" computes it with numpy
+    and uploads the bricks (ytrc_dataset_upload).  Both store the same
+    bricks."""
+
+    def __init__(self, ctx, ds, proj_fields=None, weight=None, vr_field=None, log_field=True,
+                 vr_prep="device"):
         self.ctx = ctx
         self.ds = ds
         self.proj_ds_id = None
@@ -47,16 +53,31 @@ class ResidentVolume:
             self.kd.set_fields([vr_field], [log_field], no_ghost=True)
             self.leaves = self.kd.leaves()
             self._leaf_index = {id(n): i for i, n in enumerate(self.leaves)}
-            bricks, fields, masks = [], [], []
-            for node in self.leaves:
-                pg = self.kd.get_brick_data(node)
-                bricks.append((pg.LeftEdge, pg.RightEdge, pg.dims, True))
-                fields.append(pg.my_data)
-                masks.append(pg.source_mask)
-                node.data = None  # keep only the device copy
-            self.vr_ds_id = ctx.upload(bricks, fields, masks)
-            self.n_vr_bricks = len(bricks)
-            self.kd._vcd_cache = {}
+            if vr_prep == "device":
+                gindex, grids, cells, leaves, masks = {}, [], [], [], []
+                for node in self.leaves:
+                    grid, li, ri, nle, nre, mask = self.kd.leaf_slice(node)
+                    if grid.id not in gindex:
+                        gindex[grid.id] = len(grids)
+                        grids.append((grid.LeftEdge, grid.RightEdge, grid.ActiveDimensions))
+                        cells.append([np.asarray(grid[vr_field])])
+                    leaves.append((gindex[grid.id], li, ri, nle, nre))
+                    masks.append(None if mask.all() else mask)
+                self.vr_ds_id = ctx.build_vr(grids, cells, [log_field], leaves, masks)
+                self.n_vr_bricks = len(leaves)
+            elif vr_prep == "host":
+                bricks, fields, masks = [], [], []
+                for node in self.leaves:
+                    pg = self.kd.get_brick_data(node)
+                    bricks.append((pg.LeftEdge, pg.RightEdge, pg.dims, True))
+                    fields.append(pg.my_data)
+                    masks.append(pg.source_mask)
+                    node.data = None  # keep only the device copy
+                self.vr_ds_id = ctx.upload(bricks, fields, masks)
+                self.n_vr_bricks = len(bricks)
+                self.kd._vcd_cache = {}
+            else:
+                raise ValueError("vr_prep must be 'device' or 'host'")
 
     def device_bytes(self):
         b = 0
