@@ -40,8 +40,12 @@ extern "C" {
 #define YTRC_PROJECTION 0
 #define YTRC_VOLUME_RENDER 1
 
-/* Lens: plane-parallel (lenses.pyx:22-55, :179-192). */
+/* Lenses: plane-parallel (lenses.pyx:22-55, :179-192), and per-pixel rays
+ * (generate_vector_info_null, lenses.pyx:196-207), which is how yt runs the
+ * perspective, stereo, fisheye and spherical lenses (lens.py:145-800): the
+ * lens computes an origin and a direction per pixel on the host. */
 #define YTRC_LENS_PLANE_PARALLEL 0
+#define YTRC_LENS_PER_PIXEL 1
 
 /* Request flags. */
 #define YTRC_FLAG_RECORD 1u /* per-pixel visit count, visit hash and sample count */
@@ -164,6 +168,16 @@ typedef struct {
     const double *image_in;      /* nv0*nv1*4 initial image or NULL (zeros)        */
     const ytrc_tf_desc *tf;      /* VR only                                        */
     uint32_t flags;
+    /* YTRC_LENS_PER_PIXEL only: ray origins and directions, nv0*nv1*3
+     * doubles each (the sampler's vp_pos[vi, vj, :] and vp_dir[vi, vj, :]);
+     * camera_data: NULL (every brick covers the whole image,
+     * calculate_extent_null, lenses.pyx:169-175) or the perspective
+     * camera's 5x3 rows position, width, east, north, normal (camera.py:280)
+     * for calculate_extent_perspective (lenses.pyx:60-161,
+     * image_samplers.pyx:98-99). */
+    const double *pix_pos;
+    const double *pix_dir;
+    const double *camera_data;
 } ytrc_request;
 
 /* Builds a batch: validates requests, computes the per-brick pixel extents

@@ -48,6 +48,11 @@ def lib():
             C.c_int, D, D, D, D, D, D, D, C.c_int, C.c_int, C.c_int, D, D, I32, C.c_int,
             C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int, I32, C.c_void_p, C.c_int,
             D, D, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]
+        L.orc_cast_lens.restype = C.c_int
+        L.orc_cast_lens.argtypes = [C.c_int, C.c_int, D, D, D] + L.orc_cast.argtypes[1:]
+        L.orc_extents_perspective.restype = C.c_int
+        L.orc_extents_perspective.argtypes = [D, C.c_int, C.c_int, C.c_int, D, D, I32,
+                                              C.POINTER(C.c_int64)]
         L.orc_extents.restype = C.c_int
         L.orc_extents.argtypes = [D, D, D, D, C.c_int, C.c_int, C.c_int, D, D, I32,
                                   C.POINTER(C.c_int64)]
@@ -153,15 +158,27 @@ def cast(kind, params, bricks, order=None, tf=None, n_samples=10, image=None, zb
     if record:
         visits = np.zeros((nv0, nv1), dtype=np.int64)
         hashes = np.zeros((nv0, nv1), dtype=np.uint64)
-    vp_pos = _v3(params["vp_pos"], 12)
-    vp_dir = _v3(params["vp_dir"], 3)
+    lens = 0 if params.get("lens_type", "plane-parallel") == "plane-parallel" else 1
+    pix_pos = pix_dir = cam = None
+    if lens:
+        pix_pos = np.ascontiguousarray(params["vp_pos"], dtype=np.float64).reshape(nv0, nv1, 3)
+        pix_dir = np.ascontiguousarray(params["vp_dir"], dtype=np.float64).reshape(nv0, nv1, 3)
+        if params.get("lens_type") == "perspective" and params.get("camera_data") is not None:
+            cam = _v3(params["camera_data"], 15)
+        vp_pos = np.zeros(12)
+        vp_dir = np.zeros(3)
+    else:
+        vp_pos = _v3(params["vp_pos"], 12)
+        vp_dir = _v3(params["vp_dir"], 3)
     center = _v3(params["center"], 3)
     bounds = _v3(params["bounds"], 4)
     x_vec = _v3(params["x_vec"], 3)
     y_vec = _v3(params["y_vec"], 3)
     width = _v3(params["width"], 3)
-    rc = L.orc_cast(
-        int(kind), _d(vp_pos), _d(vp_dir), _d(center), _d(bounds), _d(x_vec), _d(y_vec),
+    rc = L.orc_cast_lens(
+        int(kind), lens, _d(pix_pos) if lens else None, _d(pix_dir) if lens else None,
+        _d(cam) if cam is not None else None,
+        _d(vp_pos), _d(vp_dir), _d(center), _d(bounds), _d(x_vec), _d(y_vec),
         _d(width), nv0, nv1, n, _d(les), _d(res), _i(dims), nf, fptr, mptr, len(order),
         _i(order), C.cast(C.pointer(tfd), C.c_void_p) if tfd is not None else None,
         int(n_samples), _d(image), _d(zb) if zb is not None else None, int(num_threads),
@@ -188,6 +205,20 @@ def extents(params, bricks):
     L.orc_extents(_d(center), _d(bounds), _d(x_vec), _d(y_vec), int(params["nv"][0]),
                   int(params["nv"][1]), n, _d(les), _d(res), _i(dims),
                   out.ctypes.data_as(C.POINTER(C.c_int64)))
+    return out
+
+
+def extents_perspective(camera_data, nv, bricks):
+    """Clipped calculate_extent_perspective rectangles, n_bricks x 4."""
+    L = lib()
+    n = len(bricks)
+    les = np.ascontiguousarray([b["le"] for b in bricks], dtype=np.float64)
+    res = np.ascontiguousarray([b["re"] for b in bricks], dtype=np.float64)
+    dims = np.ascontiguousarray([b["dims"] for b in bricks], dtype=np.int32)
+    out = np.zeros((n, 4), dtype=np.int64)
+    cam = _v3(camera_data, 15)
+    L.orc_extents_perspective(_d(cam), int(nv[0]), int(nv[1]), n, _d(les), _d(res), _i(dims),
+                              out.ctypes.data_as(C.POINTER(C.c_int64)))
     return out
 
 

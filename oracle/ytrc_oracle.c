@@ -19,6 +19,9 @@
 #include <string.h>
 #include <omp.h>
 
+/* libm's M_PI (math.h outside strict ISO C) */
+#define ORC_PI 3.14159265358979323846
+
 /* ---------------------------------------------------------------------- */
 /* fp_utils.pxd:17-45 -- yt's own fmin/fmax/fclip/iclip (non-IEEE NaN rules) */
 static inline double y_fmax(double f0, double f1) { return (f0 > f1) ? f0 : f1; }
@@ -448,6 +451,84 @@ static inline void orc_vector_plane_parallel(const orc_cam *cam, int64_t vi, int
 }
 
 /* ---------------------------------------------------------------------- */
+/* Per-pixel lenses (perspective, stereo, fisheye, spherical): the sampler
+ * reads ray origin and direction from (nv0, nv1, 3) arrays
+ * (generate_vector_info_null, lenses.pyx:196-207).  The extent is the whole
+ * image (calculate_extent_null, lenses.pyx:169-175) unless the lens is
+ * perspective and camera data are given (image_samplers.pyx:98-99). */
+
+/* calculate_extent_perspective: lenses.pyx:60-161.  camera_data rows:
+ * position, width, east, north, normal (camera.py:280).  The reference
+ * declares sight_vector_norm, sight_angle_cos, sight_length, dx and dy as C
+ * float, so each is rounded to float32 where it is assigned; every other
+ * quantity is double.  The vector helpers (vec3_ops.pxd) are the double
+ * specialisations.  Pinned by tests/test_oracle_lens.py (every pixel whose
+ * ray meets the brick lies in the clipped extent). */
+static void orc_extent_perspective(const double *camera_data, int nv0, int nv1, const orc_vc *vc,
+                                   int64_t rv[4])
+{
+    double cam_pos[3], cam_width[3], east[3], north[3], normal[3];
+    double vertex[3], pos1[3], sight_vector[3], sight_center[3];
+    double corners[3][8];
+    float sight_vector_norm, sight_angle_cos, sight_length, dx, dy;
+    int min_px = 32767, min_py = 32767, max_px = -32767, max_py = -32767;
+    const double *le = vc->left_edge, *re = vc->right_edge;
+    /* lenses.pyx:80-103: the corner order of the reference */
+    const int cx[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+    const int cy[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+    const int cz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    for (int iv = 0; iv < 8; iv++) {
+        corners[0][iv] = cx[iv] ? re[0] : le[0];
+        corners[1][iv] = cy[iv] ? re[1] : le[1];
+        corners[2][iv] = cz[iv] ? re[2] : le[2];
+    }
+    for (int i = 0; i < 3; i++) {
+        cam_pos[i] = camera_data[0 * 3 + i];
+        cam_width[i] = camera_data[1 * 3 + i];
+        east[i] = camera_data[2 * 3 + i];
+        north[i] = camera_data[3 * 3 + i];
+        normal[i] = camera_data[4 * 3 + i];
+    }
+    for (int iv = 0; iv < 8; iv++) {
+        vertex[0] = corners[0][iv];
+        vertex[1] = corners[1][iv];
+        vertex[2] = corners[2][iv];
+        cam_width[1] = cam_width[0] * nv1 / nv0;
+        for (int i = 0; i < 3; i++) sight_vector[i] = vertex[i] - cam_pos[i];
+        for (int i = 0; i < 3; i++) sight_center[i] = cam_width[2] * normal[i] + cam_pos[i];
+        sight_vector_norm = (float)sqrt(sight_vector[0] * sight_vector[0] +
+                                        sight_vector[1] * sight_vector[1] +
+                                        sight_vector[2] * sight_vector[2]);
+        if (sight_vector_norm != 0)
+            for (int i = 0; i < 3; i++) sight_vector[i] /= (double)sight_vector_norm;
+        sight_angle_cos = (float)(sight_vector[0] * normal[0] + sight_vector[1] * normal[1] +
+                                  sight_vector[2] * normal[2]);
+        sight_angle_cos = (float)fmin(fmax((double)sight_angle_cos, -1.0), 1.0);
+        if (acos((double)sight_angle_cos) < 0.5 * ORC_PI && sight_angle_cos != 0.0) {
+            sight_length = (float)(cam_width[2] / (double)sight_angle_cos);
+        } else {
+            sight_length = (float)sqrt(cam_width[0] * cam_width[0] + cam_width[1] * cam_width[1]);
+            const float c2 = sight_angle_cos * sight_angle_cos;
+            sight_length = (float)((double)sight_length / sqrt(1.0 - (double)c2));
+        }
+        for (int i = 0; i < 3; i++) pos1[i] = (double)sight_length * sight_vector[i] + cam_pos[i];
+        for (int i = 0; i < 3; i++) pos1[i] = pos1[i] - sight_center[i];
+        dx = (float)(pos1[0] * east[0] + pos1[1] * east[1] + pos1[2] * east[2]);
+        dy = (float)(pos1[0] * north[0] + pos1[1] * north[1] + pos1[2] * north[2]);
+        const int px = (int)(nv0 * 0.5 + nv0 / cam_width[0] * (double)dx);
+        const int py = (int)(nv1 * 0.5 + nv1 / cam_width[1] * (double)dy);
+        if (px < min_px) min_px = px;
+        if (px > max_px) max_px = px;
+        if (py < min_py) min_py = py;
+        if (py > max_py) max_py = py;
+    }
+    rv[0] = min_px > 0 ? min_px : 0;
+    rv[1] = max_px < nv0 ? max_px : nv0;
+    rv[2] = min_py > 0 ? min_py : 0;
+    rv[3] = max_py < nv1 ? max_py : nv1;
+}
+
+/* ---------------------------------------------------------------------- */
 /* Public oracle entry points (ctypes). */
 
 typedef struct {
@@ -463,23 +544,27 @@ typedef struct {
 } orc_tf_desc;
 
 /*
- * orc_cast: ImageSampler.__call__ -> cast_through_kdtree
- * (image_samplers.pyx:132-197) for each brick of `order` in sequence, with
- * the plane-parallel lens.  OpenMP: static schedule, chunk 100, as the
+ * orc_cast_lens: ImageSampler.__call__ -> cast_through_kdtree
+ * (image_samplers.pyx:132-197) for each brick of `order` in sequence.
+ * lens 0: plane-parallel (vp_pos/vp_dir/center/bounds/x_vec/y_vec/width);
+ * lens 1: per-pixel rays pix_pos/pix_dir (nv0, nv1, 3), with the perspective
+ * extent when camera_data (5 x 3) is given, else the whole image.  OpenMP: static schedule, chunk 100, as the
  * reference's prange.  kind 0 = ProjectionSampler, 1 = VolumeRenderSampler.
  * image: float64 (nv0, nv1, 4), read-modify-write.  zbuffer: (nv0, nv1) or NULL
  * (NULL means ones, utils.py:136).  visits/hashes: per-pixel instrumentation
  * arrays or NULL.  Returns 0, or -1 on bad arguments.
  */
-int orc_cast(int kind, const double *vp_pos, const double *vp_dir, const double *center,
-             const double *bounds, const double *x_vec, const double *y_vec, const double *width,
-             int nv0, int nv1, int n_bricks, const double *les, const double *res,
-             const int32_t *dims, int n_fields, const double *const *field_ptrs,
-             const uint8_t *const *mask_ptrs, int n_order, const int32_t *order,
-             const orc_tf_desc *tf, int n_samples, double *image, const double *zbuffer,
-             int num_threads, int64_t *visits, uint64_t *hashes)
+int orc_cast_lens(int kind, int lens, const double *pix_pos, const double *pix_dir,
+                  const double *camera_data, const double *vp_pos, const double *vp_dir,
+                  const double *center, const double *bounds, const double *x_vec,
+                  const double *y_vec, const double *width, int nv0, int nv1, int n_bricks,
+                  const double *les, const double *res, const int32_t *dims, int n_fields,
+                  const double *const *field_ptrs, const uint8_t *const *mask_ptrs, int n_order,
+                  const int32_t *order, const orc_tf_desc *tf, int n_samples, double *image,
+                  const double *zbuffer, int num_threads, int64_t *visits, uint64_t *hashes)
 {
     if (nv0 <= 0 || nv1 <= 0 || n_fields <= 0 || n_fields > 6) return -1;
+    if (lens != 0 && (pix_pos == NULL || pix_dir == NULL)) return -1;
     if (kind == 1 && (tf == NULL || tf->n_fits > 6 || n_samples <= 0)) return -1;
     orc_cam cam;
     memcpy(cam.vp_pos, vp_pos, sizeof(cam.vp_pos));
@@ -526,7 +611,16 @@ int orc_cast(int kind, const double *vp_pos, const double *vp_dir, const double 
         vc.data = field_ptrs + (size_t)b * n_fields;
         vc.mask = mask_ptrs ? mask_ptrs[b] : NULL;
         int64_t it[4];
-        orc_extent_plane_parallel(&cam, &vc, it);
+        if (lens == 0) {
+            orc_extent_plane_parallel(&cam, &vc, it);
+        } else if (camera_data != NULL) {
+            orc_extent_perspective(camera_data, nv0, nv1, &vc, it);
+        } else {
+            it[0] = 0;
+            it[1] = nv0;
+            it[2] = 0;
+            it[3] = nv1;
+        }
         /* image_samplers.pyx:160-166 */
         it[0] = y_iclip(it[0] - 1, 0, cam.nv[0]);
         it[1] = y_iclip(it[1] + 1, 0, cam.nv[0]);
@@ -544,8 +638,16 @@ int orc_cast(int kind, const double *vp_pos, const double *vp_dir, const double 
             int64_t vj = j % ny;
             int64_t vi = (j - vj) / ny + it[0];
             vj = vj + it[2];
-            orc_vector_plane_parallel(&cam, vi, vj, wid, v_dir, v_pos);
             size_t pix = (size_t)vi * (size_t)nv1 + (size_t)vj;
+            if (lens == 0) {
+                orc_vector_plane_parallel(&cam, vi, vj, wid, v_dir, v_pos);
+            } else {
+                /* generate_vector_info_null: lenses.pyx:196-207 */
+                for (int i = 0; i < 3; i++) {
+                    v_pos[i] = pix_pos[pix * 3 + i];
+                    v_dir[i] = pix_dir[pix * 3 + i];
+                }
+            }
             for (int c = 0; c < 4; c++) acc.rgba[c] = image[pix * 4 + c];
             acc.vra = &vra;
             acc.brick_id = b;
@@ -565,6 +667,37 @@ int orc_cast(int kind, const double *vp_pos, const double *vp_dir, const double 
             free(fits[i].d0);
             free(fits[i].dy);
         }
+    return 0;
+}
+
+int orc_cast(int kind, const double *vp_pos, const double *vp_dir, const double *center,
+             const double *bounds, const double *x_vec, const double *y_vec, const double *width,
+             int nv0, int nv1, int n_bricks, const double *les, const double *res,
+             const int32_t *dims, int n_fields, const double *const *field_ptrs,
+             const uint8_t *const *mask_ptrs, int n_order, const int32_t *order,
+             const orc_tf_desc *tf, int n_samples, double *image, const double *zbuffer,
+             int num_threads, int64_t *visits, uint64_t *hashes)
+{
+    return orc_cast_lens(kind, 0, NULL, NULL, NULL, vp_pos, vp_dir, center, bounds, x_vec, y_vec,
+                         width, nv0, nv1, n_bricks, les, res, dims, n_fields, field_ptrs, mask_ptrs,
+                         n_order, order, tf, n_samples, image, zbuffer, num_threads, visits, hashes);
+}
+
+/* Clipped perspective extents (lenses.pyx:60-161 + image_samplers.pyx:159-166).
+ * out: n_bricks x 4 int64. */
+int orc_extents_perspective(const double *camera_data, int nv0, int nv1, int n_bricks,
+                            const double *les, const double *res, const int32_t *dims, int64_t *out)
+{
+    for (int b = 0; b < n_bricks; b++) {
+        orc_vc vc;
+        orc_vc_init(&vc, les + 3 * b, res + 3 * b, dims + 3 * b);
+        int64_t *it = out + 4 * b;
+        orc_extent_perspective(camera_data, nv0, nv1, &vc, it);
+        it[0] = y_iclip(it[0] - 1, 0, nv0);
+        it[1] = y_iclip(it[1] + 1, 0, nv0);
+        it[2] = y_iclip(it[2] - 1, 0, nv1);
+        it[3] = y_iclip(it[3] + 1, 0, nv1);
+    }
     return 0;
 }
 

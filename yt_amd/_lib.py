@@ -24,6 +24,7 @@ YTRC_ECANCELLED = -5
 PROJECTION = 0
 VOLUME_RENDER = 1
 LENS_PLANE_PARALLEL = 0
+LENS_PER_PIXEL = 1
 
 FLAG_RECORD = 1
 FLAG_ERT = 2
@@ -144,6 +145,9 @@ class Request(C.Structure):
         ("image_in", C.POINTER(C.c_double)),
         ("tf", C.POINTER(TFDesc)),
         ("flags", C.c_uint32),
+        ("pix_pos", C.POINTER(C.c_double)),
+        ("pix_dir", C.POINTER(C.c_double)),
+        ("camera_data", C.POINTER(C.c_double)),
     ]
 
 
@@ -442,15 +446,39 @@ class RequestSpec:
         self.image_in = None if image_in is None else np.ascontiguousarray(image_in, dtype=np.float64)
         self.record = record
         self.ert = ert
+        # Per-pixel lenses (perspective, fisheye, ...): vp_pos / vp_dir are
+        # (nv0, nv1, 3) arrays (image_samplers.pyx:89-96).
+        self.lens_type = params.get("lens_type", "plane-parallel")
+        self.pix_pos = self.pix_dir = self.camera_data = None
+        if self.lens_type != "plane-parallel":
+            pos = np.asarray(params["vp_pos"], dtype=np.float64)
+            dirs = np.asarray(params["vp_dir"], dtype=np.float64)
+            if pos.shape[:2] != self.nv or dirs.shape[:2] != self.nv:
+                raise RuntimeError(
+                    f"Bad lens shape / direction for {self.lens_type}\n"
+                    f"Shapes: ({pos.shape[0]} - {dirs.shape[0]} - {self.nv[0]}) and "
+                    f"({pos.shape[1]} - {dirs.shape[1]} - {self.nv[1]})")
+            self.pix_pos = np.ascontiguousarray(pos.reshape(self.nv[0], self.nv[1], 3))
+            self.pix_dir = np.ascontiguousarray(dirs.reshape(self.nv[0], self.nv[1], 3))
+            cd = params.get("camera_data")
+            if self.lens_type == "perspective" and cd is not None:
+                self.camera_data = np.ascontiguousarray(cd, dtype=np.float64).reshape(15)
 
     def to_c(self):
         r = Request()
         p = self.params
         r.kind = self.kind
-        r.lens = LENS_PLANE_PARALLEL
         r.nv[:] = list(self.nv)
-        r.vp_pos[:] = [float(x) for x in np.ravel(p["vp_pos"])]
-        r.vp_dir[:] = [float(x) for x in np.ravel(p["vp_dir"])]
+        if self.pix_pos is not None:
+            r.lens = LENS_PER_PIXEL
+            r.pix_pos = _dptr(self.pix_pos)
+            r.pix_dir = _dptr(self.pix_dir)
+            if self.camera_data is not None:
+                r.camera_data = _dptr(self.camera_data)
+        else:
+            r.lens = LENS_PLANE_PARALLEL
+            r.vp_pos[:] = [float(x) for x in np.ravel(p["vp_pos"])]
+            r.vp_dir[:] = [float(x) for x in np.ravel(p["vp_dir"])]
         r.center[:] = [float(x) for x in np.ravel(p["center"])]
         r.bounds[:] = [float(x) for x in p["bounds"]]
         r.x_vec[:] = [float(x) for x in np.ravel(p["x_vec"])]

@@ -6,6 +6,7 @@ Restates, in numpy and without units (all lengths in code units):
   Lens.setup_box_properties               volume_rendering/lens.py:45-61
   PlaneParallelLens._get_sampler_params   volume_rendering/lens.py:83-112
   PlaneParallelLens.set_viewpoint         volume_rendering/lens.py:114-121
+  PerspectiveLens                         volume_rendering/lens.py:145-294
 The sampler parameters (vp_pos, vp_dir, center, bounds, x_vec, y_vec, width)
 are exactly what ImageSampler's constructor receives (utils.py:115-139).
 """
@@ -130,6 +131,58 @@ class PlaneParallelLens:
         }
 
 
+class PerspectiveLens(PlaneParallelLens):
+    """lens.py:145-294: rays from the camera position through a plane of
+    width x height at distance width[2], one origin and direction per pixel
+    (generate_vector_info_null, lenses.pyx:196-207)."""
+
+    lens_type = "perspective"
+
+    def set_viewpoint(self, camera):
+        """lens.py:236-240: the front centre."""
+        self.viewpoint = self.front_center
+
+    def _get_sampler_params(self, camera, render_source=None):
+        """lens.py:166-234, in the same numpy operations and order."""
+        # Enforce width[1] / width[0] = resolution[1] / resolution[0]
+        camera.width[1] = camera.width[0] * (camera.resolution[1] / camera.resolution[0])
+        res0, res1 = camera.resolution
+        east_vec = camera.unit_vectors[0]
+        north_vec = camera.unit_vectors[1]
+        normal_vec = camera.unit_vectors[2]
+        px = np.linspace(-0.5, 0.5, res0)[np.newaxis, :]
+        py = np.linspace(-0.5, 0.5, res1)[np.newaxis, :]
+        sample_x = camera.width[0] * np.array(east_vec.reshape(3, 1) * px)
+        sample_x = sample_x.transpose()
+        sample_y = camera.width[1] * np.array(north_vec.reshape(3, 1) * py)
+        sample_y = sample_y.transpose()
+        sample_x = np.repeat(sample_x.reshape(res0, 1, 3), res1, axis=1)
+        sample_y = np.repeat(sample_y.reshape(1, res1, 3), res0, axis=0)
+        normal_vecs = np.tile(normal_vec, res0 * res1).reshape(res0, res1, 3)
+        # The maximum possible length of ray
+        max_length = np.linalg.norm(camera.position - camera._domain_center) + 0.5 * np.linalg.norm(
+            camera._domain_width)
+        # Rescale the ray to be long enough to cover the entire domain
+        vectors = (sample_x + sample_y + normal_vecs * camera.width[2]) * (
+            max_length / camera.width[2])
+        positions = np.tile(camera.position, res0 * res1).reshape(res0, res1, 3)
+        uv = np.ones(3, dtype="float64")
+        return {
+            "vp_pos": positions,
+            "vp_dir": vectors,
+            "center": self.back_center,
+            "bounds": (0.0, 1.0, 0.0, 1.0),
+            "x_vec": uv,
+            "y_vec": uv,
+            "width": np.zeros(3, dtype="float64"),
+            "image": self.new_image(camera),
+            "lens_type": "perspective",
+        }
+
+
+LENSES = {"plane-parallel": PlaneParallelLens, "perspective": PerspectiveLens}
+
+
 class Camera(Orientation):
     """camera.py:47-428, restricted to what the ray-caster reads.
 
@@ -138,7 +191,7 @@ class Camera(Orientation):
 
     def __init__(self, scene=None, data_source=None, lens_type="plane-parallel",
                  domain_left_edge=(0.0, 0.0, 0.0), domain_right_edge=(1.0, 1.0, 1.0)):
-        if lens_type != "plane-parallel":
+        if lens_type not in LENSES:
             raise NotImplementedError(f"lens type {lens_type!r} is not implemented")
         self.scene = scene
         self.lens = None
@@ -152,11 +205,20 @@ class Camera(Orientation):
         else:
             dle = np.asarray(domain_left_edge, dtype="float64")
             dre = np.asarray(domain_right_edge, dtype="float64")
+        self._domain_center = 0.5 * (dle + dre)
+        self._domain_width = dre - dle
         self._focus = 0.5 * (dle + dre)
         self._position = dre.copy()
         self._width = np.array([1.5 * (dre - dle).max()] * 3)
         super().__init__(self._focus - self._position, self.north_vector, steady_north=False)
-        self.lens = PlaneParallelLens()
+        self.lens = LENSES[lens_type]()
+        self.lens.setup_box_properties(self)
+
+    def set_lens(self, lens_type):
+        """camera.py:283-303."""
+        if lens_type not in LENSES:
+            raise RuntimeError(f"Lens type {lens_type!r} not in available list {sorted(LENSES)}")
+        self.lens = LENSES[lens_type]()
         self.lens.setup_box_properties(self)
 
     @property

@@ -558,10 +558,24 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
     const int64_t npix = (int64_t)im.nv0 * im.nv1;
     const int64_t pix = active ? (int64_t)vi * im.nv1 + vj : 0;
 
+    // MODE bit 3: per-pixel lens (generate_vector_info_null,
+    // lenses.pyx:196-207); rays no longer share a direction.
+    constexpr bool PP = (MODE & 8) != 0;
     double pos[3], dir[3], iv[3];
-    ray_setup(im, vi, vj, pos, dir);
+    if constexpr (PP) {
 #pragma unroll
-    for (int i = 0; i < 3; i++) iv[i] = uni_d((dir[i] != 0.0) ? 1.0 / dir[i] : 0.0);
+        for (int i = 0; i < 3; i++) {
+            pos[i] = im.pix_pos[pix * 3 + i];
+            dir[i] = im.pix_dir[pix * 3 + i];
+        }
+    } else {
+        ray_setup(im, vi, vj, pos, dir);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double v = (dir[i] != 0.0) ? 1.0 / dir[i] : 0.0;
+        iv[i] = PP ? v : uni_d(v);
+    }
     double max_t = 1.0;
     if (im.zbuf != nullptr && active) max_t = y_fmin(y_fmax(im.zbuf[pix], 0.0), 1.0);
 
@@ -631,7 +645,7 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
         const int bid = im.order[k];
         const DBrick &b = im.bricks[bid];
         Walk w;
-        if (go) go = walk_init(b, pos, dir, iv, max_t, w);
+        if (go) go = walk_init<!PP>(b, pos, dir, iv, max_t, w);
         if (go) {
             rec.brick = bid;
             if (KIND == 0) {
@@ -661,7 +675,7 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
                 s.b = brick_reg(b);
 #pragma unroll
                 for (int i = 0; i < 3; i++) {
-                    s.pa[i] = uni_d(dir[i] * b.idds[i]);
+                    s.pa[i] = PP ? dir[i] * b.idds[i] : uni_d(dir[i] * b.idds[i]);
                     s.pb[i] = (pos[i] - b.le[i]) * b.idds[i];
                 }
                 s.tf = &tfc;
@@ -887,14 +901,15 @@ __global__ void k_zero_counters(unsigned int *c, int n)
 typedef void (*pfn_t)(const DImg *, const int2 *, int, unsigned int *, const float4 *, int);
 typedef void (*kfn_t)(const DImg *, int);
 
+// NFP 4 and 8 keep only the table-path and lens bits of MODE.
 template <int KIND, bool REC, int MODE>
 pfn_t pick_persist_nfp(int nfp)
 {
     switch (nfp) {
     case 1: return k_persist<KIND, 1, REC, MODE>;
     case 2: return k_persist<KIND, 2, REC, MODE>;
-    case 4: return (MODE & 4) ? nullptr : k_persist<KIND, 4, REC, (MODE & 1)>;
-    default: return (KIND == 1 && !(MODE & 4)) ? k_persist<KIND, 8, REC, (MODE & 1)> : nullptr;
+    case 4: return (MODE & 4) ? nullptr : k_persist<KIND, 4, REC, (MODE & 9)>;
+    default: return (KIND == 1 && !(MODE & 4)) ? k_persist<KIND, 8, REC, (MODE & 9)> : nullptr;
     }
 }
 
@@ -904,17 +919,18 @@ kfn_t pick_kernel_nfp(int nfp)
     switch (nfp) {
     case 1: return k_render<KIND, 1, REC, MODE>;
     case 2: return k_render<KIND, 2, REC, MODE>;
-    case 4: return (MODE & 4) ? nullptr : k_render<KIND, 4, REC, (MODE & 1)>;
-    default: return (KIND == 1 && !(MODE & 4)) ? k_render<KIND, 8, REC, (MODE & 1)> : nullptr;
+    case 4: return (MODE & 4) ? nullptr : k_render<KIND, 4, REC, (MODE & 9)>;
+    default: return (KIND == 1 && !(MODE & 4)) ? k_render<KIND, 8, REC, (MODE & 9)> : nullptr;
     }
 }
 
-// mode (VR only): bit 0 ColorTransferFunction path, bit 1 n_samples == 10,
-// bit 2 front-to-back early ray termination.
+// mode (VR): bit 0 ColorTransferFunction path, bit 1 n_samples == 10,
+// bit 2 front-to-back early ray termination, bit 3 per-pixel lens (not
+// combined with bit 2).
 template <bool REC>
 pfn_t pick_persist_vr(int nfp, int mode)
 {
-    switch (mode & 7) {
+    switch (mode & 15) {
     case 0: return pick_persist_nfp<1, REC, 0>(nfp);
     case 1: return pick_persist_nfp<1, REC, 1>(nfp);
     case 2: return pick_persist_nfp<1, REC, 2>(nfp);
@@ -922,14 +938,19 @@ pfn_t pick_persist_vr(int nfp, int mode)
     case 4: return pick_persist_nfp<1, REC, 4>(nfp);
     case 5: return pick_persist_nfp<1, REC, 5>(nfp);
     case 6: return pick_persist_nfp<1, REC, 6>(nfp);
-    default: return pick_persist_nfp<1, REC, 7>(nfp);
+    case 7: return pick_persist_nfp<1, REC, 7>(nfp);
+    case 8: return pick_persist_nfp<1, REC, 8>(nfp);
+    case 9: return pick_persist_nfp<1, REC, 9>(nfp);
+    case 10: return pick_persist_nfp<1, REC, 10>(nfp);
+    case 11: return pick_persist_nfp<1, REC, 11>(nfp);
+    default: return nullptr;
     }
 }
 
 template <bool REC>
 kfn_t pick_kernel_vr(int nfp, int mode)
 {
-    switch (mode & 7) {
+    switch (mode & 15) {
     case 0: return pick_kernel_nfp<1, REC, 0>(nfp);
     case 1: return pick_kernel_nfp<1, REC, 1>(nfp);
     case 2: return pick_kernel_nfp<1, REC, 2>(nfp);
@@ -937,19 +958,31 @@ kfn_t pick_kernel_vr(int nfp, int mode)
     case 4: return pick_kernel_nfp<1, REC, 4>(nfp);
     case 5: return pick_kernel_nfp<1, REC, 5>(nfp);
     case 6: return pick_kernel_nfp<1, REC, 6>(nfp);
-    default: return pick_kernel_nfp<1, REC, 7>(nfp);
+    case 7: return pick_kernel_nfp<1, REC, 7>(nfp);
+    case 8: return pick_kernel_nfp<1, REC, 8>(nfp);
+    case 9: return pick_kernel_nfp<1, REC, 9>(nfp);
+    case 10: return pick_kernel_nfp<1, REC, 10>(nfp);
+    case 11: return pick_kernel_nfp<1, REC, 11>(nfp);
+    default: return nullptr;
     }
 }
 
+// mode (projection): bit 3 per-pixel lens.
 pfn_t pick_persist(int kind, int nfp, bool rec, int mode)
 {
-    if (kind == 0) return rec ? pick_persist_nfp<0, true, 0>(nfp) : pick_persist_nfp<0, false, 0>(nfp);
+    if (kind == 0) {
+        if (mode & 8) return rec ? pick_persist_nfp<0, true, 8>(nfp) : pick_persist_nfp<0, false, 8>(nfp);
+        return rec ? pick_persist_nfp<0, true, 0>(nfp) : pick_persist_nfp<0, false, 0>(nfp);
+    }
     return rec ? pick_persist_vr<true>(nfp, mode) : pick_persist_vr<false>(nfp, mode);
 }
 
 kfn_t pick_kernel(int kind, int nfp, bool rec, int mode)
 {
-    if (kind == 0) return rec ? pick_kernel_nfp<0, true, 0>(nfp) : pick_kernel_nfp<0, false, 0>(nfp);
+    if (kind == 0) {
+        if (mode & 8) return rec ? pick_kernel_nfp<0, true, 8>(nfp) : pick_kernel_nfp<0, false, 8>(nfp);
+        return rec ? pick_kernel_nfp<0, true, 0>(nfp) : pick_kernel_nfp<0, false, 0>(nfp);
+    }
     return rec ? pick_kernel_vr<true>(nfp, mode) : pick_kernel_vr<false>(nfp, mode);
 }
 
@@ -1093,6 +1126,69 @@ void extent_plane_parallel(const ytrc_request &r, const DBrick &b, int64_t rv[4]
     rv[1] = clip(rv[1] + 1, 0, r.nv[0]);
     rv[2] = clip(rv[2] - 1, 0, r.nv[1]);
     rv[3] = clip(rv[3] + 1, 0, r.nv[1]);
+}
+
+// calculate_extent_perspective (lenses.pyx:60-161) + the +-1 clip, on the
+// host.  The reference declares sight_vector_norm, sight_angle_cos,
+// sight_length, dx and dy as C float: each is rounded to float32 where it
+// is assigned; everything else is double (vec3_ops.pxd's double versions).
+void extent_perspective(const ytrc_request &r, const DBrick &b, int64_t rv[4])
+{
+    const int nv0 = r.nv[0], nv1 = r.nv[1];
+    const double *cd = r.camera_data;
+    double cam_pos[3], cam_width[3], east[3], north[3], normal[3];
+    for (int i = 0; i < 3; i++) {
+        cam_pos[i] = cd[i];
+        cam_width[i] = cd[3 + i];
+        east[i] = cd[6 + i];
+        north[i] = cd[9 + i];
+        normal[i] = cd[12 + i];
+    }
+    // corner order of lenses.pyx:80-103
+    const int cx[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+    const int cy[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+    const int cz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    int min_px = 32767, min_py = 32767, max_px = -32767, max_py = -32767;
+    for (int iv = 0; iv < 8; iv++) {
+        const double vertex[3] = {cx[iv] ? b.re[0] : b.le[0], cy[iv] ? b.re[1] : b.le[1],
+                                  cz[iv] ? b.re[2] : b.le[2]};
+        cam_width[1] = cam_width[0] * nv1 / nv0;
+        double sv[3], sc[3], pos1[3];
+        for (int i = 0; i < 3; i++) sv[i] = vertex[i] - cam_pos[i];
+        for (int i = 0; i < 3; i++) sc[i] = cam_width[2] * normal[i] + cam_pos[i];
+        const float norm = (float)std::sqrt(sv[0] * sv[0] + sv[1] * sv[1] + sv[2] * sv[2]);
+        if (norm != 0)
+            for (int i = 0; i < 3; i++) sv[i] /= (double)norm;
+        float cosa = (float)(sv[0] * normal[0] + sv[1] * normal[1] + sv[2] * normal[2]);
+        cosa = (float)std::fmin(std::fmax((double)cosa, -1.0), 1.0);
+        float len;
+        if (std::acos((double)cosa) < 0.5 * 3.14159265358979323846 && cosa != 0.0) {
+            len = (float)(cam_width[2] / (double)cosa);
+        } else {
+            len = (float)std::sqrt(cam_width[0] * cam_width[0] + cam_width[1] * cam_width[1]);
+            const float c2 = cosa * cosa;
+            len = (float)((double)len / std::sqrt(1.0 - (double)c2));
+        }
+        for (int i = 0; i < 3; i++) pos1[i] = (double)len * sv[i] + cam_pos[i];
+        for (int i = 0; i < 3; i++) pos1[i] = pos1[i] - sc[i];
+        const float dx = (float)(pos1[0] * east[0] + pos1[1] * east[1] + pos1[2] * east[2]);
+        const float dy = (float)(pos1[0] * north[0] + pos1[1] * north[1] + pos1[2] * north[2]);
+        const int px = (int)(nv0 * 0.5 + nv0 / cam_width[0] * (double)dx);
+        const int py = (int)(nv1 * 0.5 + nv1 / cam_width[1] * (double)dy);
+        min_px = std::min(min_px, px);
+        max_px = std::max(max_px, px);
+        min_py = std::min(min_py, py);
+        max_py = std::max(max_py, py);
+    }
+    rv[0] = std::max(min_px, 0);
+    rv[1] = std::min(max_px, nv0);
+    rv[2] = std::max(min_py, 0);
+    rv[3] = std::min(max_py, nv1);
+    auto clip = [](int64_t v, int64_t a, int64_t bnd) { return v < a ? a : (v > bnd ? bnd : v); };
+    rv[0] = clip(rv[0] - 1, 0, nv0);
+    rv[1] = clip(rv[1] + 1, 0, nv0);
+    rv[2] = clip(rv[2] - 1, 0, nv1);
+    rv[3] = clip(rv[3] + 1, 0, nv1);
 }
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -1478,7 +1574,7 @@ struct PrepReq {
     DTF T;
     std::vector<float4> tft;
     size_t o_order = 0, o_ext = 0, o_zbuf = 0, o_init = 0, o_out = 0;
-    size_t o_rv = 0, o_rh = 0, o_rs = 0, o_tf = 0;
+    size_t o_rv = 0, o_rh = 0, o_rs = 0, o_tf = 0, o_ppos = 0, o_pdir = 0;
     size_t bytes = 0;
     int tsh = TILE_SHIFT_DEFAULT;
 };
@@ -1491,8 +1587,14 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
     q.r = r;
     if (r.kind != YTRC_PROJECTION && r.kind != YTRC_VOLUME_RENDER)
         return fail(YTRC_ENOTIMPL, tag + "unknown sampler kind");
-    if (r.lens != YTRC_LENS_PLANE_PARALLEL)
-        return fail(YTRC_ENOTIMPL, tag + "only the plane-parallel lens is implemented");
+    if (r.lens != YTRC_LENS_PLANE_PARALLEL && r.lens != YTRC_LENS_PER_PIXEL)
+        return fail(YTRC_ENOTIMPL, tag + "unknown lens");
+    const bool pp = r.lens == YTRC_LENS_PER_PIXEL;
+    if (pp && (r.pix_pos == nullptr || r.pix_dir == nullptr))
+        return fail(YTRC_EINVAL, tag + "a per-pixel lens needs per-pixel origins and directions "
+                                       "(image_samplers.pyx:89-96)");
+    if (pp && (r.flags & YTRC_FLAG_ERT))
+        return fail(YTRC_ENOTIMPL, tag + "early ray termination is implemented for the plane-parallel lens only");
     if (r.nv[0] <= 0 || r.nv[1] <= 0) return fail(YTRC_EINVAL, tag + "image size must be > 0");
     auto dsit = ctx->datasets.find(r.ds_id);
     if (dsit == ctx->datasets.end()) return fail(YTRC_EINVAL, tag + "unknown dataset");
@@ -1620,12 +1722,29 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
                 emax = std::max(emax, ec);
             }
             double tcell = 0.0;
-            for (int k2 = 0; bounded && k2 < r.n_order; k2++) {
-                const DBrick &hb = ds.h_bricks[r.order[k2]];
-                double cmin = 1.0;
-                for (int a = 0; a < 3; a++)
-                    if (r.vp_dir[a] != 0.0) cmin = std::min(cmin, hb.dds[a] / std::fabs(r.vp_dir[a]));
-                tcell = std::max(tcell, cmin);
+            if (!pp) {
+                for (int k2 = 0; bounded && k2 < r.n_order; k2++) {
+                    const DBrick &hb = ds.h_bricks[r.order[k2]];
+                    double cmin = 1.0;
+                    for (int a = 0; a < 3; a++)
+                        if (r.vp_dir[a] != 0.0) cmin = std::min(cmin, hb.dds[a] / std::fabs(r.vp_dir[a]));
+                    tcell = std::max(tcell, cmin);
+                }
+            } else {
+                // Any chord of a cell is at most its diagonal: the longest
+                // in-cell t-interval is below diag / |v_dir| over all rays.
+                double dmin = 1e300, diag = 0.0;
+                for (int64_t p = 0; p < (int64_t)r.nv[0] * r.nv[1]; p++) {
+                    const double *v = r.pix_dir + 3 * p;
+                    dmin = std::min(dmin, std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+                }
+                for (int k2 = 0; k2 < r.n_order; k2++) {
+                    const DBrick &hb = ds.h_bricks[r.order[k2]];
+                    diag = std::max(diag, std::sqrt(hb.dds[0] * hb.dds[0] + hb.dds[1] * hb.dds[1] +
+                                                    hb.dds[2] * hb.dds[2]));
+                }
+                tcell = dmin > 0.0 ? diag / dmin : 1e300;
+                bounded = bounded && std::isfinite(tcell);
             }
             q.tform = bounded && (emax * (tcell / (double)r.n_samples) * 1.001 <= 1.0);
         }
@@ -1671,19 +1790,29 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
             tot = (int)P.tft.size();
             T.mode = 1;
         }
-        q.mode = T.mode | (r.n_samples == 10 ? 2 : 0) | ((r.flags & YTRC_FLAG_ERT) ? 4 : 0);
+        q.mode = T.mode | (r.n_samples == 10 ? 2 : 0) | ((r.flags & YTRC_FLAG_ERT) ? 4 : 0) | (pp ? 8 : 0);
         T.total_bins = tot;
         q.lds = (size_t)tot * sizeof(float4);
         if (q.lds > 64 * 1024) return fail(YTRC_EINVAL, tag + "transfer-function tables exceed 64 KiB of LDS");
         q.nch_out = T.grey ? 4 : 3;
     }
+    if (r.kind == YTRC_PROJECTION) q.mode = pp ? 8 : 0;
     q.kfn = pick_kernel(r.kind, q.nfp, rec, q.mode);
     if (q.kfn == nullptr) return fail(YTRC_ENOTIMPL, tag + "no kernel for this field count");
     if (r.image_in != nullptr) q.image_in.assign(r.image_in, r.image_in + q.npix * 4);
     P.ext.resize(std::max(1, r.n_order));
     for (int k = 0; k < r.n_order; k++) {
         int64_t rv[4];
-        extent_plane_parallel(r, ds.h_bricks[r.order[k]], rv);
+        if (!pp) {
+            extent_plane_parallel(r, ds.h_bricks[r.order[k]], rv);
+        } else if (r.camera_data != nullptr) {
+            extent_perspective(r, ds.h_bricks[r.order[k]], rv);
+        } else {
+            rv[0] = 0;  // calculate_extent_null (lenses.pyx:169-175); the
+            rv[1] = r.nv[0];  // +-1 clip leaves the whole image
+            rv[2] = 0;
+            rv[3] = r.nv[1];
+        }
         P.ext[k] = make_int4((int)rv[0], (int)rv[1], (int)rv[2], (int)rv[3]);
     }
     // Device region layout.
@@ -1709,6 +1838,13 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
     }
     P.o_tf = off;
     if (r.kind == YTRC_VOLUME_RENDER) off = align_up(off + sizeof(DTF), 256);
+    P.o_ppos = P.o_pdir = off;
+    if (pp) {
+        P.o_ppos = off;
+        off = align_up(off + sizeof(double) * 3 * q.npix, 256);
+        P.o_pdir = off;
+        off = align_up(off + sizeof(double) * 3 * q.npix, 256);
+    }
     P.bytes = off;
     return YTRC_OK;
 }
@@ -1759,6 +1895,12 @@ int stage_request(ytrc_ctx *ctx, PrepReq &P, char *A, const float4 *tf_table, in
     }
     if (r.zbuffer)
         HIPCHK(hipMemcpy(A + P.o_zbuf, r.zbuffer, sizeof(double) * q.npix, hipMemcpyHostToDevice));
+    if (r.lens == YTRC_LENS_PER_PIXEL) {
+        d.pix_pos = (const double *)(A + P.o_ppos);
+        d.pix_dir = (const double *)(A + P.o_pdir);
+        HIPCHK(hipMemcpy(A + P.o_ppos, r.pix_pos, sizeof(double) * 3 * q.npix, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(A + P.o_pdir, r.pix_dir, sizeof(double) * 3 * q.npix, hipMemcpyHostToDevice));
+    }
     if (r.image_in) {
         stage.resize((size_t)q.npix * 4);
         for (int64_t p = 0; p < q.npix; p++)
@@ -2534,6 +2676,9 @@ int ytrc_server_submit(ytrc_server *S, const ytrc_request *r, int64_t *id_out)
     R.P.q.r.zbuffer = nullptr;
     R.P.q.r.image_in = nullptr;
     R.P.q.r.tf = nullptr;
+    R.P.q.r.pix_pos = nullptr;
+    R.P.q.r.pix_dir = nullptr;
+    R.P.q.r.camera_data = nullptr;
     *id_out = R.id;
     const int64_t id = R.id;
     S->reqs.emplace(id, std::move(R));

@@ -65,6 +65,8 @@ struct alignas(16) DImg {
     uint64_t *rec_hash;
     int64_t *rec_samples;
     const DTF *tf;
+    const double *pix_pos;  // per-pixel lens: nv0*nv1*3 origins, else null
+    const double *pix_dir;  // per-pixel lens: nv0*nv1*3 directions
     int32_t nv0, nv1;
     int32_t kind, n_samples;
     int32_t n_order, nf;
@@ -136,6 +138,7 @@ struct Walk {
 
 // grid_traversal.pyx:42-118.  Returns false where walk_volume returns 0
 // before its loop.  iv[i] = 1/v_dir[i] (0 where v_dir[i] == 0).
+template <bool UNIFORM_DIR = true>
 __device__ __forceinline__ bool walk_init(const DBrick &b, const double pos[3],
                                           const double dir[3], const double iv[3],
                                           double max_t, Walk &w)
@@ -147,10 +150,12 @@ __device__ __forceinline__ bool walk_init(const DBrick &b, const double pos[3],
         intersect_t = 0.0;
         direction = 3;
     }
-    // Plane-parallel rays share v_dir, so step and tdelta are wave-uniform.
+    // Plane-parallel rays share v_dir, so step and tdelta are wave-uniform;
+    // per-pixel lenses give every lane its own.
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        w.step[i] = uni((dir[i] < 0) ? -1 : ((dir[i] == 0.0) ? 0 : 1));
+        const int st = (dir[i] < 0) ? -1 : ((dir[i] == 0.0) ? 0 : 1);
+        w.step[i] = UNIFORM_DIR ? uni(st) : st;
     }
     if (direction != 3) {
 #pragma unroll
@@ -179,7 +184,8 @@ __device__ __forceinline__ bool walk_init(const DBrick &b, const double pos[3],
     if (!((0.0 <= intersect_t) && (intersect_t < max_t))) return false;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        w.tdelta[i] = uni_d((double)w.step[i] * iv[i] * b.dds[i]);
+        const double td = (double)w.step[i] * iv[i] * b.dds[i];
+        w.tdelta[i] = UNIFORM_DIR ? uni_d(td) : td;
         double ty = 0.0;
         if (i == direction && w.step[i] > 0) {
             w.cur[i] = 0;

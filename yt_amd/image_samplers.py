@@ -45,9 +45,7 @@ class ImageSampler:
                     f"Bad lens shape / direction for {self.lens_type}\n"
                     f"Shapes: ({vp_pos.shape[0]} - {vp_dir.shape[0]} - {image.shape[0]}) and "
                     f"({vp_pos.shape[1]} - {vp_dir.shape[1]} - {image.shape[1]})")
-            raise NotImplementedError(
-                f"lens {self.lens_type!r}: only the plane-parallel lens runs on the GPU path")
-        if vp_pos.size != 12 or vp_dir.size != 3:
+        elif vp_pos.size != 12 or vp_dir.size != 3:
             raise ValueError("plane-parallel lens needs 12 vp_pos and 3 vp_dir values")
         self.vp_pos = vp_pos
         self.vp_dir = vp_dir
@@ -91,6 +89,8 @@ class ImageSampler:
             "vp_pos": self.vp_pos, "vp_dir": self.vp_dir, "center": self.acenter,
             "bounds": self.bounds, "x_vec": self.ax_vec, "y_vec": self.ay_vec,
             "width": self.width, "nv": self.nv,
+            "lens_type": self.lens_type if self.lens_type is not None else "null",
+            "camera_data": self.camera_data,
         }
 
     def _tf(self):

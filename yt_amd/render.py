@@ -120,6 +120,27 @@ def camera_params(center, normal, width, resolution, north=None):
     return p
 
 
+def perspective_params(position, focus, width, resolution, north=None,
+                       domain_left_edge=(0.0, 0.0, 0.0), domain_right_edge=(1.0, 1.0, 1.0)):
+    """Sampler parameters (+ kd viewpoint, the front centre) of a perspective
+    camera at `position` looking at `focus` (lens.py:145-240, camera.py:275-281)."""
+    from .camera import Camera
+
+    cam = Camera(lens_type="perspective", domain_left_edge=domain_left_edge,
+                 domain_right_edge=domain_right_edge)
+    if np.ndim(resolution) == 0:
+        resolution = (resolution, resolution)
+    cam.resolution = resolution
+    cam.position = position
+    cam.focus = focus
+    cam.width = width
+    if north is not None:
+        cam.switch_orientation(north_vector=north)
+    p = cam._get_sampler_params()
+    p["viewpoint"] = cam.lens.viewpoint.copy()
+    return p
+
+
 def finalize_projection(image, width, weighted, method="integrate"):
     """off_axis_projection.py:526-547 + VolumeSource.finalize_image
     (render_source.py:522-537): alpha := 1, then scale or divide."""
