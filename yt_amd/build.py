@@ -4,11 +4,17 @@ The library is compiled with -ffp-contract=off so that the FP64 ray set-up
 and DDA keep the reference's separately rounded IEEE operations (yt's
 Cython is compiled by gcc for baseline x86-64, which never fuses), and with
 packed FP32 instructions disabled.
+
+Each translation unit (the host code, and one per family of kernel
+variants) compiles to an object in parallel; the objects link into the
+shared library.
 """
 
 import os
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -24,7 +30,6 @@ FLAGS = [
     f"--offload-arch={ARCH}",
     "-ffp-contract=off",
     "-fPIC",
-    "-shared",
     "-Xclang",
     "-target-feature",
     "-Xclang",
@@ -34,20 +39,31 @@ FLAGS = [
 
 
 def sources():
-    return [os.path.join(CSRC, "ytrc.hip")]
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
-def build(verbose=False):
-    """Compiles every HIP source into yt_amd/libytrc.so (always rebuilds)."""
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp, *sources()]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout)
         sys.stderr.write(res.stderr)
-        raise RuntimeError(f"hipcc failed with status {res.returncode}")
+        raise RuntimeError(f"hipcc failed with status {res.returncode}: {' '.join(cmd[-3:])}")
+
+
+def build(verbose=False, jobs=None):
+    """Compiles every HIP source into yt_amd/libytrc.so (always rebuilds)."""
+    srcs = sources()
+    jobs = jobs or min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
+    inc = ["-I", os.path.join(ROOT, "include")]
+    with tempfile.TemporaryDirectory(prefix="ytrc_build_") as tmpd:
+        objs = [os.path.join(tmpd, os.path.basename(s)[:-4] + ".o") for s in srcs]
+        cmds = [[HIPCC, *FLAGS, *inc, "-c", s, "-o", o] for s, o in zip(srcs, objs)]
+        with ThreadPoolExecutor(max_workers=jobs) as ex:
+            list(ex.map(lambda c: _run(c, verbose), cmds))
+        tmp = LIB + ".tmp"
+        _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp], verbose)
     # The host pass warns that it does not know the packed-fp32 feature.
     os.replace(tmp, LIB)
     return LIB

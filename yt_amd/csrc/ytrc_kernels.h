@@ -1,0 +1,797 @@
+// ytrc_kernels.h -- the ray-casting kernel templates of libytrc.so.
+//
+// Included by the translation units that instantiate them (ytrc_k_*.hip, one
+// per family of kernel variants, compiled in parallel) and by ytrc.hip for
+// the shared constants.  Everything is in an unnamed namespace: each unit
+// instantiates and exports (through ytrc_pick.h) only its own variants.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ytrc_device.h"
+
+namespace {
+
+using namespace ytrc;
+
+
+template <int NFP>
+struct FieldLoad;
+template <>
+struct FieldLoad<1> {
+    static __device__ __forceinline__ void get(const float *p, float v[1]) { v[0] = p[0]; }
+};
+template <>
+struct FieldLoad<2> {
+    static __device__ __forceinline__ void get(const float *p, float v[2])
+    {
+        float2 t = *reinterpret_cast<const float2 *>(p);
+        v[0] = t.x;
+        v[1] = t.y;
+    }
+};
+template <>
+struct FieldLoad<4> {
+    static __device__ __forceinline__ void get(const float *p, float v[4])
+    {
+        float4 t = *reinterpret_cast<const float4 *>(p);
+        v[0] = t.x;
+        v[1] = t.y;
+        v[2] = t.z;
+        v[3] = t.w;
+    }
+};
+template <>
+struct FieldLoad<8> {
+    static __device__ __forceinline__ void get(const float *p, float v[8])
+    {
+        float4 a = *reinterpret_cast<const float4 *>(p);
+        float4 b = *reinterpret_cast<const float4 *>(p + 4);
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = a.z;
+        v[3] = a.w;
+        v[4] = b.x;
+        v[5] = b.y;
+        v[6] = b.z;
+        v[7] = b.w;
+    }
+};
+
+__device__ __forceinline__ void kahan_add(float &s, float &c, float x)
+{
+    float y = x - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// ProjectionSampler.sample (image_samplers.pyx:330-343): rgba[i] +=
+// data[i][cell] * (exit_t - enter_t) for i < min(4, n_fields).  No mask.
+template <int NFP, bool REC>
+struct ProjSampler {
+    const float *data;
+    int d1, d2;
+    int dims0;
+    int nbj, nbk;  // 4-cell blocks along axes 1 and 2
+    float s[NFP];
+    float c[NFP];
+    Rec *rec;
+};
+
+// walk_volume's loop (grid_traversal.pyx:120-138) fused with
+// ProjectionSampler.sample and software-pipelined: the DDA never depends
+// on the data, so each round takes up to PF steps, issues their PF loads
+// together and then accumulates them in visit order (the same additions, in
+// the same order, as one visit at a time).
+template <int NFP, bool REC>
+__device__ __forceinline__ void walk_projection(const DBrick &b, Walk &w, double max_t,
+                                                ProjSampler<NFP, REC> &S)
+{
+    constexpr int PF = 8;
+    double t0 = w.tmax[0], t1 = w.tmax[1], t2 = w.tmax[2];
+    int c0 = w.cur[0], c1 = w.cur[1], c2 = w.cur[2];
+    double enter_t = w.enter_t;
+    const int st0 = w.step[0], st1 = w.step[1], st2 = w.step[2];
+    const double td0 = w.tdelta[0], td1 = w.tdelta[1], td2 = w.tdelta[2];
+    bool live = true;
+    while (live) {
+        int off[PF];
+        float dl[PF];
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            off[u] = -1;
+            dl[u] = 0.0f;
+            if (live) {
+                const bool a01 = t0 < t1;
+                const bool a02 = t0 < t2;
+                const bool a12 = t1 < t2;
+                const int ax = a01 ? (a02 ? 0 : 2) : (a12 ? 1 : 2);
+                const double tm = (ax == 0) ? t0 : ((ax == 1) ? t1 : t2);
+                const double exit_t = y_fmin(tm, max_t);
+                if (REC) {
+                    S.rec->visit(enter_t, exit_t, c0, c1, c2);
+                    if (exit_t > enter_t) S.rec->samples += 1;
+                }
+                if ((unsigned)c0 < (unsigned)S.dims0 && (unsigned)c1 < (unsigned)S.d1 &&
+                    (unsigned)c2 < (unsigned)S.d2) {
+                    // 4x4x4-blocked cell offset (blocked_offset); block
+                    // counts per face are below 2^24, so 24-bit multiplies
+                    // are exact here
+                    const unsigned blk =
+                        __umul24(__umul24((unsigned)c0 >> 2, (unsigned)S.nbj) + ((unsigned)c1 >> 2),
+                                 (unsigned)S.nbk) + ((unsigned)c2 >> 2);
+                    off[u] = (int)((blk << 6) | ((c0 & 3) << 4) | ((c1 & 3) << 2) | (c2 & 3));
+                    dl[u] = (float)(exit_t - enter_t);
+                }
+                // Step the chosen axis and test only its new index, as the
+                // reference does, with lane masks rather than an axis-indexed
+                // table (which the compiler would place in scratch memory).
+                const bool s0 = (ax == 0), s1 = (ax == 1), s2 = (ax == 2);
+                c0 += s0 ? st0 : 0;
+                c1 += s1 ? st1 : 0;
+                c2 += s2 ? st2 : 0;
+                t0 = s0 ? t0 + td0 : t0;
+                t1 = s1 ? t1 + td1 : t1;
+                t2 = s2 ? t2 + td2 : t2;
+                const bool out = (s0 & ((unsigned)c0 >= (unsigned)S.dims0)) |
+                                 (s1 & ((unsigned)c1 >= (unsigned)S.d1)) |
+                                 (s2 & ((unsigned)c2 >= (unsigned)S.d2));
+                enter_t = tm;
+                live = !(out || enter_t >= max_t);
+            }
+        }
+        float v[PF][NFP];
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            if (off[u] >= 0) {
+                FieldLoad<NFP>::get(S.data + (size_t)off[u] * NFP, v[u]);
+            } else {
+#pragma unroll
+                for (int f = 0; f < NFP; f++) v[u][f] = 0.0f;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            if (off[u] >= 0) {
+#pragma unroll
+                for (int f = 0; f < NFP; f++) kahan_add(S.s[f], S.c[f], v[u][f] * dl[u]);
+            }
+        }
+    }
+}
+
+template <int N>
+__device__ __forceinline__ float pick(const float (&a)[N], int i)
+{
+    float r = a[0];
+#pragma unroll
+    for (int k = 1; k < N; k++) r = (i == k) ? a[k] : r;
+    return r;
+}
+
+// Transfer-function parameters of one image, loaded once per tile into
+// (uniform) registers.  mode 1 is the ColorTransferFunction pattern
+// (transfer_functions.py:385-408): four tables on one field with shared
+// bounds, red/green/blue weighted by the alpha table, channels linked 0..3;
+// its LDS table holds per bin {y_r, y_g, y_b, y_a}, {dy_r..dy_a} and d0.
+// mode 0 is the general FieldInterpolationTable set.
+struct TFCache {
+    int mode;
+    int n_fits;
+    int grey;
+    int fid;       // mode 1: the one field
+    float b0, b1, idbin, nb0idbin;  // nb0idbin = -b0 * idbin
+    float qc1, qzb;                 // transmittance-form CTF entry selection
+    int nbm2;      // mode 1: nbins - 2
+    int nbins[6];
+    int off[6];
+    int field_id[6];
+    int wfield[6];
+    int chexp[4];
+    float tb0[6], tb1[6], tidbin[6];
+};
+
+__device__ __forceinline__ float unif(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// Loads the (wave-uniform) table parameters into scalar registers.
+__device__ __forceinline__ void load_tf(const DTF &T, TFCache &c)
+{
+    c.mode = uni(T.mode);
+    c.n_fits = uni(T.n_fits);
+    c.grey = uni(T.grey);
+    c.fid = uni(T.field_id[0]);
+    c.b0 = unif(T.b0[0]);
+    c.b1 = unif(T.b1[0]);
+    c.idbin = unif(T.idbin[0]);
+    c.nb0idbin = -c.b0 * c.idbin;
+    c.qc1 = unif(T.qc1);
+    c.qzb = unif(T.qzb);
+    c.nbm2 = uni(T.nbins[0] - 2);
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        c.nbins[t] = uni(T.nbins[t]);
+        c.off[t] = uni(T.off[t]);
+        c.field_id[t] = uni(T.field_id[t]);
+        c.wfield[t] = uni(T.wfield[t]);
+        c.tb0[t] = unif(T.b0[t]);
+        c.tb1[t] = unif(T.b1[t]);
+        c.tidbin[t] = unif(T.idbin[t]);
+    }
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++) c.chexp[ch] = uni(T.chexp[ch]);
+}
+
+// FIT_get_value + FIT_eval_transfer (field_interpolation_tables.pxd:66-110),
+// general table set, tables in LDS as float4 {y, dy, d0, 0}.  The
+// reference's sequential weight-table products (istorage[i] *=
+// istorage[weight_table_id]) and channel links (trgba[c] =
+// istorage[field_table_ids[c]]) are resolved on the host into chexp[c]:
+// 2 bits per raw table u giving the power of table u's value in channel c.
+// Table reads are branch-free: out-of-range values read a clamped bin and
+// are then zeroed.
+template <int NFP>
+__device__ __forceinline__ void fit_eval_general(const TFCache &tf, const float4 *__restrict__ lt,
+                                                 const float (&dvs)[NFP], float e[4])
+{
+    float raw[6];
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        raw[t] = 0.0f;
+        if (t < tf.n_fits) {
+            const float v = (NFP == 1) ? dvs[0] : pick(dvs, tf.field_id[t]);
+            const bool ok = (v < tf.tb1[t]) && (v > tf.tb0[t]) && __builtin_isnormal(v);
+            int bin = (int)((v - tf.tb0[t]) * tf.tidbin[t]);
+            bin = bin < 0 ? 0 : (bin > tf.nbins[t] - 2 ? tf.nbins[t] - 2 : bin);
+            const float4 q = lt[tf.off[t] + bin];
+            float r = __builtin_fmaf(v - q.z, q.y, q.x);
+            if (tf.wfield[t] != -1) r *= (NFP == 1) ? dvs[0] : pick(dvs, tf.wfield[t]);
+            raw[t] = ok ? r : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++) {
+        const int ce = tf.chexp[ch];
+        float r = (ce < 0) ? 0.0f : 1.0f;
+#pragma unroll
+        for (int u = 0; u < 6; u++) {
+            const int ex = (ce >> (2 * u)) & 3;
+            if (ex >= 1) r *= raw[u];
+            if (ex >= 2) r *= raw[u];
+            if (ex >= 3) r *= raw[u];
+        }
+        e[ch] = r;
+    }
+}
+
+// The ColorTransferFunction case of the same evaluation: one bin for the
+// four tables, e_c = table_c(v) * table_alpha(v) for c < 3, e_3 = alpha.
+// An out-of-range or non-normal v reads bin 0 and gets alpha 0, so every
+// channel is 0 without a select per channel.
+// ColorTransferFunction emissions in the transmittance-form layout: entry
+// floor(u) of the per-bin quadratics, in-bin coordinate fract(u), where
+// u = 1 + (v - b0) * idbin.  FIT_get_value's range test v in (b0, b1)
+// (field_interpolation_tables.pxd:63-82) becomes u in [1, nb) through the
+// zero entries 0 and nb, nb + 1 and a clamp of u to [0, nb + 1]; the test
+// is thereby made on the FP32 bin coordinate, exact up to its rounding.
+// Values that are not normal (0, subnormal, inf, NaN) select entry 0.
+__device__ __forceinline__ void fit_eval_quad(const TFCache &tf, const float4 *__restrict__ lt,
+                                              float v, float e[3])
+{
+    float u = __builtin_fmaf(v, tf.idbin, tf.qc1);
+    u = __builtin_isnormal(v) ? u : 0.0f;
+    u = __builtin_amdgcn_fmed3f(u, 0.0f, tf.qzb);
+    const unsigned bin = (unsigned)u;
+    const float f = __builtin_amdgcn_fractf(u);
+    const float4 *q = (const float4 *)((const char *)lt + __umul24(bin, 48u));
+    const float4 q0 = q[0];
+    const float4 q1 = q[1];
+    const float c2 = q[2].x;
+    e[0] = __builtin_fmaf(__builtin_fmaf(q1.z, f, q0.w), f, q0.x);
+    e[1] = __builtin_fmaf(__builtin_fmaf(q1.w, f, q1.x), f, q0.y);
+    e[2] = __builtin_fmaf(__builtin_fmaf(c2, f, q1.y), f, q0.z);
+}
+
+// Brick fields the samplers read, copied by value (uniform across a wave).
+struct BrickReg {
+    double le[3];
+    double dds[3];
+    double idds[3];
+    int d0, d1, d2;
+};
+
+__device__ __forceinline__ BrickReg brick_reg(const DBrick &b)
+{
+    BrickReg r;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        r.le[i] = uni_d(b.le[i]);
+        r.dds[i] = uni_d(b.dds[i]);
+        r.idds[i] = uni_d(b.idds[i]);
+    }
+    r.d0 = uni(b.dims[0]);
+    r.d1 = uni(b.dims[1]);
+    r.d2 = uni(b.dims[2]);
+    return r;
+}
+
+// offset_interpolate (fixed_interpolator.cpp:34-52) restricted to the
+// sub-sample positions u(s) = dp + s*ds of one visit is a cubic in s.  Its
+// coefficients come from the 8 corners c[4i+2j+k] once per visit; each
+// sub-sample then costs 3 FMAs.
+__device__ __forceinline__ void trilinear_cubic(const float (&c)[8], const float dp[3],
+                                                const float ds[3], float k[4])
+{
+    const float A = c[0];
+    const float Bz = c[1] - c[0];
+    const float By = c[2] - c[0];
+    const float Bx = c[4] - c[0];
+    const float Cyz = (c[3] - c[2]) - Bz;
+    const float Cxz = (c[5] - c[4]) - Bz;
+    const float Cxy = (c[6] - c[4]) - By;
+    const float D = ((c[7] - c[6]) - (c[5] - c[4])) - Cyz;
+    const float x0 = dp[0], y0 = dp[1], z0 = dp[2];
+    const float dx = ds[0], dy = ds[1], dz = ds[2];
+    const float fxy = __builtin_fmaf(D, z0, Cxy);
+    const float fxz = __builtin_fmaf(D, y0, Cxz);
+    const float fyz = __builtin_fmaf(D, x0, Cyz);
+    const float tyz = __builtin_fmaf(Cyz, z0, By);
+    const float ex = __builtin_fmaf(y0, fxy, __builtin_fmaf(Cxz, z0, Bx));
+    const float ey = __builtin_fmaf(x0, fxy, tyz);
+    const float ez = __builtin_fmaf(x0, fxz, __builtin_fmaf(Cyz, y0, Bz));
+    k[0] = __builtin_fmaf(x0, ex, __builtin_fmaf(y0, tyz, __builtin_fmaf(z0, Bz, A)));
+    k[1] = __builtin_fmaf(ex, dx, __builtin_fmaf(ey, dy, ez * dz));
+    const float dydz = dy * dz;
+    k[2] = __builtin_fmaf(fxy, dx * dy, __builtin_fmaf(fxz, dx * dz, fyz * dydz));
+    k[3] = D * dx * dydz;
+}
+
+// VolumeRenderSampler.sample (image_samplers.pyx:453-487).
+// MODE bit 0: ColorTransferFunction in transmittance form; bit 1: n_samples == 10 unrolled;
+// bit 2: early ray termination.
+// st[] is the compositing state: rgba, or, when `tform` is set (non-grey
+// and the host has shown dt * e <= 1 for every sub-sample), the per-channel
+// transmittance T = 1 - rgba, for which rgba' = a + max(1 - a, 0) rgba
+// (field_interpolation_tables.pxd:107-110) is exactly T' = T (1 - a).
+template <int NFP, bool REC, int MODE>
+struct VRSampler {
+    const float *data;
+    const uint8_t *mask;
+    BrickReg b;
+    double pa[3];  // v_dir * idds (wave-uniform)
+    double pb[3];  // (v_pos - left_edge) * idds
+    const TFCache *tf;
+    const float4 *lt;
+    int nb;
+    int tform;
+    static constexpr bool ert = (MODE & 4) != 0;  // also track the brick's transmittance tb
+    double inv_ns;
+    int n_samples;
+    double st[4];
+    double tb[4];
+    Rec *rec;
+
+    __device__ __forceinline__ void eval(const float (&dvs)[NFP], float e[4]) const
+    {
+        fit_eval_general<NFP>(*tf, lt, dvs, e);
+    }
+
+    template <int NS, class Body>
+    __device__ __forceinline__ static void each_sample(int ns, Body &&body)
+    {
+        if constexpr (NS > 0) {
+#pragma unroll
+            for (int s = 0; s < NS; s++) body(s);
+        } else {
+            for (int s = 0; s < ns; s++) body(s);
+        }
+    }
+
+    template <int NS>
+    __device__ __forceinline__ void samples(const float (&k)[NFP][4], double dt, int ns)
+    {
+        const TFCache &T = *tf;
+        auto value = [&](int s, float (&dvs)[NFP]) {
+            const float sf = (float)s;
+#pragma unroll
+            for (int f = 0; f < NFP; f++)
+                dvs[f] = __builtin_fmaf(
+                    __builtin_fmaf(__builtin_fmaf(k[f][3], sf, k[f][2]), sf, k[f][1]), sf, k[f][0]);
+        };
+        if constexpr ((MODE & 1) != 0) {
+            // The visit's opacity Q_c = 1 - prod_s (1 - dt e_s) by the
+            // recurrence Q' = Q + e G, G = dt (1 - Q): Q stays small where
+            // the medium is thin, so FP32 keeps its relative precision;
+            // the FP64 transmittance then takes T (1 - Q) once per visit.
+            const float dtf = (float)dt;
+            float Q[3] = {0.0f, 0.0f, 0.0f};
+            float G[3] = {dtf, dtf, dtf};
+            const int nsq = (NS > 0) ? NS : ns;
+            float sf = 0.0f;
+#pragma unroll
+            for (int s = 0; s < nsq; s++, sf += 1.0f) {
+                float dvs[NFP], e[3];
+#pragma unroll
+                for (int f = 0; f < NFP; f++)
+                    dvs[f] = __builtin_fmaf(
+                        __builtin_fmaf(__builtin_fmaf(k[f][3], sf, k[f][2]), sf, k[f][1]), sf, k[f][0]);
+                fit_eval_quad(T, lt, (NFP == 1) ? dvs[0] : pick(dvs, T.fid), e);
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++) {
+                    Q[ch] = __builtin_fmaf(e[ch], G[ch], Q[ch]);
+                    G[ch] = __builtin_fmaf(-dtf, Q[ch], dtf);
+                }
+            }
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++) st[ch] = __builtin_fma(-st[ch], (double)Q[ch], st[ch]);
+        } else if (tform) {
+            each_sample<NS>(ns, [&](int s) {
+                float dvs[NFP], e[4];
+                value(s, dvs);
+                eval(dvs, e);
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++) st[ch] *= __builtin_fma(-dt, (double)e[ch], 1.0);
+            });
+        } else if (T.grey) {
+            each_sample<NS>(ns, [&](int s) {
+                float dvs[NFP], e[4];
+                value(s, dvs);
+                eval(dvs, e);
+                const double ta = y_fmax(1.0 - dt * (double)e[3], 0.0);
+#pragma unroll
+                for (int ch = 0; ch < 4; ch++) st[ch] = __builtin_fma(ta, st[ch], dt * (double)e[ch]);
+                if constexpr (ert) tb[0] *= ta;
+            });
+        } else {
+            each_sample<NS>(ns, [&](int s) {
+                float dvs[NFP], e[4];
+                value(s, dvs);
+                eval(dvs, e);
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++) {
+                    const double a = dt * (double)e[ch];
+                    const double ta = y_fmax(1.0 - a, 0.0);
+                    st[ch] = __builtin_fma(ta, st[ch], a);
+                    if constexpr (ert) tb[ch] *= ta;
+                }
+            });
+        }
+    }
+
+    __device__ __forceinline__ void operator()(double en, double ex, int c0, int c1, int c2)
+    {
+        if (REC) rec->visit(en, ex, c0, c1, c2);
+        const int d1 = b.d1, d2 = b.d2;
+        if (!((unsigned)c0 < (unsigned)b.d0 && (unsigned)c1 < (unsigned)d1 &&
+              (unsigned)c2 < (unsigned)d2))
+            return;
+        const int cell = (int)__umul24(__umul24(c0, d1) + c1, d2) + c2;
+        const int s1 = d2 + 1;
+        const int s0 = (d1 + 1) * (d2 + 1);
+        const int voff = (int)__umul24(c0, s0) + (int)__umul24(c1, s1) + c2;
+        // The mask byte and the corner values are independent loads: issue
+        // both before testing the mask.
+        const int mval = (mask != nullptr) ? (int)mask[cell] : 1;
+        // dt = (exit_t - enter_t) / n_samples; positions from FP64 enter_t
+        // in the order of image_samplers.pyx:475-479.
+        const double dt = (ex - en) * inv_ns;
+        const double tt = en + 0.5 * dt;
+        // Cell-relative position (tt*v_dir + v_pos - (idx*dds + LE)) * idds,
+        // as tt*pa + pb - idx with pa = v_dir*idds, pb = (v_pos - LE)*idds.
+        float dp[3], ds[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int ci = (i == 0) ? c0 : ((i == 1) ? c1 : c2);
+            dp[i] = (float)(__builtin_fma(tt, pa[i], pb[i]) - (double)ci);
+            ds[i] = (float)(pa[i] * dt);
+        }
+        float cv[8][NFP];
+        {
+            const float *p0 = data + (size_t)voff * NFP;
+            FieldLoad<NFP>::get(p0, cv[0]);
+            FieldLoad<NFP>::get(p0 + NFP, cv[1]);
+            FieldLoad<NFP>::get(p0 + (size_t)s1 * NFP, cv[2]);
+            FieldLoad<NFP>::get(p0 + (size_t)(s1 + 1) * NFP, cv[3]);
+            FieldLoad<NFP>::get(p0 + (size_t)s0 * NFP, cv[4]);
+            FieldLoad<NFP>::get(p0 + (size_t)(s0 + 1) * NFP, cv[5]);
+            FieldLoad<NFP>::get(p0 + (size_t)(s0 + s1) * NFP, cv[6]);
+            FieldLoad<NFP>::get(p0 + (size_t)(s0 + s1 + 1) * NFP, cv[7]);
+        }
+        if (mval != 1) return;
+        if (REC) rec->samples += n_samples;
+        float k[NFP][4];
+#pragma unroll
+        for (int f = 0; f < NFP; f++) {
+            const float cf[8] = {cv[0][f], cv[1][f], cv[2][f], cv[3][f],
+                                 cv[4][f], cv[5][f], cv[6][f], cv[7][f]};
+            trilinear_cubic(cf, dp, ds, k[f]);
+        }
+        if (MODE & 2)
+            samples<10>(k, dt, 10);
+        else
+            samples<0>(k, dt, n_samples);
+    }
+};
+
+// Casts the rays of one 8x8 tile of image `im` through its bricks.  `im`
+// and `tile` are wave-uniform.
+template <int KIND, int NFP, bool REC, int MODE>
+__device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
+                                            const float4 *lt)
+{
+    const int tsh = im.tsh;
+    const int tw = 1 << tsh, th = 64 >> tsh;
+    const int tx = tile / im.tiles1;
+    const int ty = tile - tx * im.tiles1;
+    const int ti0 = tx * th, tj0 = ty * tw;
+    const int vi = ti0 + (lane >> tsh);
+    const int vj = tj0 + (lane & (tw - 1));
+    const bool active = (vi < im.nv0) && (vj < im.nv1);
+    const int64_t npix = (int64_t)im.nv0 * im.nv1;
+    const int64_t pix = active ? (int64_t)vi * im.nv1 + vj : 0;
+
+    // MODE bit 3: per-pixel lens (generate_vector_info_null,
+    // lenses.pyx:196-207); rays no longer share a direction.
+    constexpr bool PP = (MODE & 8) != 0;
+    double pos[3], dir[3], iv[3];
+    if constexpr (PP) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            pos[i] = im.pix_pos[pix * 3 + i];
+            dir[i] = im.pix_dir[pix * 3 + i];
+        }
+    } else {
+        ray_setup(im, vi, vj, pos, dir);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double v = (dir[i] != 0.0) ? 1.0 / dir[i] : 0.0;
+        iv[i] = PP ? v : uni_d(v);
+    }
+    double max_t = 1.0;
+    if (im.zbuf != nullptr && active) max_t = y_fmin(y_fmax(im.zbuf[pix], 0.0), 1.0);
+
+    Rec rec;
+    rec.visits = 0;
+    rec.samples = 0;
+    rec.hash = 0;
+    rec.brick = 0;
+
+    TFCache tfc;
+    int nb = 0, tform = 0;
+    if (KIND == 1) {
+        load_tf(*im.tf, tfc);
+        nb = uni(im.tf->nbins[0]);
+        tform = uni(im.tform);
+    }
+
+    float ps[NFP], pc[NFP];
+    double st[4];
+#pragma unroll
+    for (int f = 0; f < NFP; f++) {
+        ps[f] = 0.0f;
+        pc[f] = 0.0f;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) st[c] = 0.0;
+    if (im.init != nullptr && active) {
+        if (KIND == 0) {
+#pragma unroll
+            for (int f = 0; f < NFP; f++)
+                if (f < 4) ps[f] = im.init[f * npix + pix];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; c++) st[c] = (double)im.init[c * npix + pix];
+        }
+    }
+    if (KIND == 1 && tform) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) st[c] = 1.0 - st[c];
+    }
+
+    // Early ray termination (YTRC_FLAG_ERT, VR only): the reference composites
+    // each new sample on top as t grows (field_interpolation_tables.pxd:107-110),
+    // so its eye is at t -> +inf.  Front to back is therefore the reverse brick
+    // order; each brick's cells are still walked forward (the same DDA) to form
+    // its per-channel affine map rgba -> C_b + T_b rgba, maps are composed as
+    // C += T C_b, T *= T_b, and a wave stops when every lane has T < ERT_EPS.
+    constexpr double ERT_EPS = 1e-4;
+    constexpr bool ert = (KIND == 1) && ((MODE & 4) != 0);
+    double rgba0[4], Cacc[4], Tacc[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        rgba0[c] = st[c];
+        Cacc[c] = 0.0;
+        Tacc[c] = 1.0;
+    }
+    if (ert && tform) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) rgba0[c] = 1.0 - st[c];
+    }
+
+    for (int kk = 0; kk < im.n_order; kk++) {
+        const int k = ert ? im.n_order - 1 - kk : kk;
+        const int4 e = im.ext[k];
+        if (e.y <= ti0 || e.x >= ti0 + th || e.w <= tj0 || e.z >= tj0 + tw) continue;
+        bool go = active && vi >= e.x && vi < e.y && vj >= e.z && vj < e.w;
+        const int bid = im.order[k];
+        const DBrick &b = im.bricks[bid];
+        Walk w;
+        if (go) go = walk_init<!PP>(b, pos, dir, iv, max_t, w);
+        if (go) {
+            rec.brick = bid;
+            if (KIND == 0) {
+                ProjSampler<NFP, REC> s;
+                s.data = im.pool + b.data_off;
+                s.d1 = uni(b.dims[1]);
+                s.d2 = uni(b.dims[2]);
+                s.dims0 = uni(b.dims[0]);
+                s.nbj = (s.d1 + 3) >> 2;
+                s.nbk = (s.d2 + 3) >> 2;
+#pragma unroll
+                for (int f = 0; f < NFP; f++) {
+                    s.s[f] = ps[f];
+                    s.c[f] = pc[f];
+                }
+                s.rec = &rec;
+                walk_projection(b, w, max_t, s);
+#pragma unroll
+                for (int f = 0; f < NFP; f++) {
+                    ps[f] = s.s[f];
+                    pc[f] = s.c[f];
+                }
+            } else {
+                VRSampler<NFP, REC, MODE> s;
+                s.data = im.pool + b.data_off;
+                s.mask = (b.mask_off >= 0) ? im.masks + b.mask_off : nullptr;
+                s.b = brick_reg(b);
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    s.pa[i] = PP ? dir[i] * b.idds[i] : uni_d(dir[i] * b.idds[i]);
+                    s.pb[i] = (pos[i] - b.le[i]) * b.idds[i];
+                }
+                s.tf = &tfc;
+                s.lt = lt;
+                s.nb = nb;
+                s.tform = tform;
+                s.inv_ns = im.inv_ns;
+                s.n_samples = im.n_samples;
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    s.st[c] = ert ? (tform ? 1.0 : 0.0) : st[c];
+                    s.tb[c] = 1.0;
+                }
+                s.rec = &rec;
+                walk_loop(b, w, max_t, s);
+                if constexpr (ert) {
+                    double cb[4], tb[4];
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        if (tform) {
+                            cb[c] = (c < 3) ? 1.0 - s.st[c] : 0.0;
+                            tb[c] = (c < 3) ? s.st[c] : 1.0;
+                        } else if (tfc.grey) {
+                            cb[c] = s.st[c];
+                            tb[c] = s.tb[0];
+                        } else {
+                            cb[c] = (c < 3) ? s.st[c] : 0.0;
+                            tb[c] = (c < 3) ? s.tb[c] : 1.0;
+                        }
+                        Cacc[c] = __builtin_fma(Tacc[c], cb[c], Cacc[c]);
+                        Tacc[c] *= tb[c];
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 4; c++) st[c] = s.st[c];
+                }
+            }
+        }
+        if constexpr (ert) {
+            const double tm = y_fmax(y_fmax(Tacc[0], Tacc[1]), y_fmax(Tacc[2], tfc.grey ? Tacc[3] : 0.0));
+            if (__all(!active || tm < ERT_EPS)) break;
+        }
+    }
+    if (!active) return;
+    if (KIND == 0) {
+#pragma unroll
+        for (int f = 0; f < NFP; f++)
+            if (f < im.nch_out) im.out[f * npix + pix] = ps[f] - pc[f];
+    } else if constexpr (ert) {
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            if (c < im.nch_out) im.out[c * npix + pix] = (float)__builtin_fma(Tacc[c], rgba0[c], Cacc[c]);
+    } else {
+        if (tform) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) st[c] = 1.0 - st[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            if (c < im.nch_out) im.out[c * npix + pix] = (float)st[c];
+    }
+    if (REC) {
+        im.rec_visits[pix] = rec.visits;
+        im.rec_hash[pix] = rec.hash;
+        im.rec_samples[pix] = rec.samples;
+    }
+}
+
+// Plain launch: one request; blockDim.x/64 tiles per workgroup.
+template <int KIND, int NFP, bool REC, int MODE>
+__global__ __launch_bounds__(256) void k_render(const DImg *__restrict__ imgs, int idx)
+{
+    extern __shared__ float4 s_tf[];
+    const DImg &im = imgs[idx];
+    if (KIND == 1) {
+        const DTF &T = *im.tf;
+        for (int i = threadIdx.x; i < T.total_bins; i += blockDim.x) s_tf[i] = T.table[i];
+        __syncthreads();
+    }
+    const int waves = blockDim.x >> 6;
+    const int tile = blockIdx.x * waves + (threadIdx.x >> 6);
+    if (tile >= im.tiles0 * im.tiles1) return;
+    render_tile<KIND, NFP, REC, MODE>(im, tile, threadIdx.x & 63, s_tf);
+}
+
+// Tile-queue partitions of a persistent launch: one per XCD of the MI355X
+// (256 CUs in 8 XCDs of 32, each XCD with its own L2).
+constexpr int QSPLIT = 8;
+
+// Persistent launch: every wave repeatedly takes the next (image, tile)
+// item of a global queue until the queue is drained.  All transfer-function
+// tables of the batch sit in LDS; each image reads its own slot.  The item
+// index is made wave-uniform so that image, brick and table descriptors are
+// read with scalar loads.
+// Minimum waves per SIMD requested from the register allocator: the
+// production volume-render variants (one field, ColorTransferFunction path,
+// no record/ERT) fit 128 VGPRs, i.e. 4 waves per SIMD, without spilling.
+template <int KIND, int NFP, bool REC, int MODE>
+struct MinWaves {
+    static constexpr int value = (KIND == 1 && NFP == 1 && !REC && (MODE == 1 || MODE == 3)) ? 5 : 1;
+};
+
+template <int KIND, int NFP, bool REC, int MODE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves<KIND, NFP, REC, MODE>::value)))
+void k_persist(const DImg *__restrict__ imgs,
+                                                 const int2 *__restrict__ items, int n_items,
+                                                 unsigned int *__restrict__ counter,
+                                                 const float4 *__restrict__ tf_all, int tf_bins)
+{
+    extern __shared__ float4 s_tf[];
+    if (KIND == 1) {
+        for (int i = threadIdx.x; i < tf_bins; i += blockDim.x) s_tf[i] = tf_all[i];
+        __syncthreads();
+    }
+    const int lane = threadIdx.x & 63;
+    // The item list is split into QSPLIT contiguous ranges with a counter
+    // each.  Workgroups are dispatched round-robin over the XCDs, so
+    // blockIdx.x % QSPLIT names this workgroup's XCD: it drains the range of
+    // neighbouring tiles that XCD's L2 already holds lines for, then helps
+    // with the other ranges in turn.  Every wave exits after finding all
+    // QSPLIT ranges empty.
+    int part = (int)(blockIdx.x % QSPLIT);
+    int left = QSPLIT;
+    while (true) {
+        unsigned int it = 0;
+        if (lane == 0) it = atomicAdd(counter + part, 1u);
+        it = (unsigned int)__builtin_amdgcn_readfirstlane((int)__shfl(it, 0));
+        const unsigned int lo = (unsigned int)(((int64_t)n_items * part) / QSPLIT);
+        const unsigned int hi = (unsigned int)(((int64_t)n_items * (part + 1)) / QSPLIT);
+        if (it >= hi - lo) {
+            if (--left == 0) break;
+            part = (part + 1 == QSPLIT) ? 0 : part + 1;
+            continue;
+        }
+        const int2 w = items[lo + it];
+        const int img = __builtin_amdgcn_readfirstlane(w.x);
+        const int tile = __builtin_amdgcn_readfirstlane(w.y);
+        const DImg &im = imgs[img];
+        render_tile<KIND, NFP, REC, MODE>(im, tile, lane, s_tf + im.tf_slot);
+    }
+}
+
+}  // namespace
