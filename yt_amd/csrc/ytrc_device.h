@@ -236,25 +236,19 @@ __device__ __forceinline__ int walk_loop(const DBrick &b, Walk &w, double max_t,
         const double tm = (ax == 0) ? t0 : ((ax == 1) ? t1 : t2);
         const double exit_t = y_fmin(tm, max_t);
         sample(enter_t, exit_t, c0, c1, c2);
-        int cn, dn;
-        if (ax == 0) {
-            c0 += w.step[0];
-            t0 += w.tdelta[0];
-            cn = c0;
-            dn = D0;
-        } else if (ax == 1) {
-            c1 += w.step[1];
-            t1 += w.tdelta[1];
-            cn = c1;
-            dn = D1;
-        } else {
-            c2 += w.step[2];
-            t2 += w.tdelta[2];
-            cn = c2;
-            dn = D2;
-        }
+        // Step the chosen axis and test only its new index (lane masks, no
+        // divergent branches).
+        const bool s0 = (ax == 0), s1 = (ax == 1), s2 = (ax == 2);
+        c0 += s0 ? w.step[0] : 0;
+        c1 += s1 ? w.step[1] : 0;
+        c2 += s2 ? w.step[2] : 0;
+        t0 = s0 ? t0 + w.tdelta[0] : t0;
+        t1 = s1 ? t1 + w.tdelta[1] : t1;
+        t2 = s2 ? t2 + w.tdelta[2] : t2;
+        const bool out = (s0 & ((unsigned)c0 >= (unsigned)D0)) | (s1 & ((unsigned)c1 >= (unsigned)D1)) |
+                         (s2 & ((unsigned)c2 >= (unsigned)D2));
         enter_t = tm;
-        if (cn < 0 || cn >= dn || enter_t >= max_t) break;
+        if (out || enter_t >= max_t) break;
     }
     return hit;
 }
