@@ -35,10 +35,15 @@ extern "C" {
 #define YTRC_ENOTIMPL (-4)  /* e.g. unknown volume method (image_samplers.pyx:137-141)    */
 #define YTRC_ECANCELLED (-5)
 
-/* Sampler kinds: ProjectionSampler (image_samplers.pyx:327-343) and
- * VolumeRenderSampler (image_samplers.pyx:405-494). */
+/* Sampler kinds: ProjectionSampler (image_samplers.pyx:327-343),
+ * VolumeRenderSampler (image_samplers.pyx:405-494) and
+ * InterpolatedProjectionSampler (image_samplers.pyx:344-402, the sampler of
+ * a "projection" VolumeSource, render_source.py:464-488; vertex-centred
+ * bricks, n_samples trilinear sub-samples per cell, min(3, n_fields)
+ * channels). */
 #define YTRC_PROJECTION 0
 #define YTRC_VOLUME_RENDER 1
+#define YTRC_INTERPOLATED_PROJECTION 2
 
 /* Lenses: plane-parallel (lenses.pyx:22-55, :179-192), and per-pixel rays
  * (generate_vector_info_null, lenses.pyx:196-207), which is how yt runs the

@@ -345,6 +345,36 @@ static void orc_volume_render_sample(const orc_vc *vc, const double v_pos[3], co
     }
 }
 
+/* InterpolatedProjectionSampler.sample: image_samplers.pyx:368-402.
+ * Vertex-centred data, no mask test; per sub-sample the trilinear value of
+ * each field times dt is added to rgba[j], j < min(3, n_fields). */
+static void orc_interpolated_projection_sample(const orc_vc *vc, const double v_pos[3],
+                                               const double v_dir[3], double enter_t, double exit_t,
+                                               const int index[3], void *data)
+{
+    orc_acc *im = (orc_acc *)data;
+    const orc_vra *vri = im->vra;
+    if (!orc_in_range(vc, index)) return;
+    int offset = index[0] * (vc->dims[1] + 1) * (vc->dims[2] + 1) + index[1] * (vc->dims[2] + 1) +
+                 index[2];
+    double dp[3], ds[3];
+    double dt = (exit_t - enter_t) / (double)vri->n_samples;
+    double dvs[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 3; i++) {
+        dp[i] = (enter_t + 0.5 * dt) * v_dir[i] + v_pos[i];
+        dp[i] -= (double)index[i] * vc->dds[i] + vc->left_edge[i];
+        dp[i] *= vc->idds[i];
+        ds[i] = v_dir[i] * vc->idds[i] * dt;
+    }
+    const int nch = vc->n_fields < 3 ? vc->n_fields : 3;
+    for (int s = 0; s < vri->n_samples; s++) {
+        for (int j = 0; j < vc->n_fields; j++)
+            dvs[j] = orc_offset_interpolate(vc->dims, dp, vc->data[j] + offset);
+        for (int j = 0; j < nch; j++) im->rgba[j] += dvs[j] * dt;
+        for (int j = 0; j < 3; j++) dp[j] += ds[j];
+    }
+}
+
 /* Visit instrumentation: a 64-bit order-sensitive hash over
  * (brick id, i, j, k, bits(enter_t), bits(exit_t)) of every sampler call. */
 static inline uint64_t orc_mix(uint64_t h, uint64_t w)
@@ -375,6 +405,13 @@ static void orc_projection_sample_rec(const orc_vc *vc, const double v_pos[3], c
 {
     orc_record((orc_acc *)data, index, enter_t, exit_t);
     orc_projection_sample(vc, v_pos, v_dir, enter_t, exit_t, index, data);
+}
+static void orc_interpolated_projection_sample_rec(const orc_vc *vc, const double v_pos[3],
+                                                   const double v_dir[3], double enter_t,
+                                                   double exit_t, const int index[3], void *data)
+{
+    orc_record((orc_acc *)data, index, enter_t, exit_t);
+    orc_interpolated_projection_sample(vc, v_pos, v_dir, enter_t, exit_t, index, data);
 }
 static void orc_volume_render_sample_rec(const orc_vc *vc, const double v_pos[3],
                                          const double v_dir[3], double enter_t, double exit_t,
@@ -546,6 +583,8 @@ typedef struct {
 /*
  * orc_cast_lens: ImageSampler.__call__ -> cast_through_kdtree
  * (image_samplers.pyx:132-197) for each brick of `order` in sequence.
+ * kind 0 = ProjectionSampler, 1 = VolumeRenderSampler,
+ * 2 = InterpolatedProjectionSampler.
  * lens 0: plane-parallel (vp_pos/vp_dir/center/bounds/x_vec/y_vec/width);
  * lens 1: per-pixel rays pix_pos/pix_dir (nv0, nv1, 3), with the perspective
  * extent when camera_data (5 x 3) is given, else the whole image.  OpenMP: static schedule, chunk 100, as the
@@ -597,8 +636,14 @@ int orc_cast_lens(int kind, int lens, const double *pix_pos, const double *pix_d
     orc_sampler fn;
     if (kind == 0)
         fn = rec ? orc_projection_sample_rec : orc_projection_sample;
+    else if (kind == 2)
+        fn = rec ? orc_interpolated_projection_sample_rec : orc_interpolated_projection_sample;
     else
         fn = rec ? orc_volume_render_sample_rec : orc_volume_render_sample;
+    if (kind == 2) {
+        if (n_samples <= 0) return -1;
+        vra.n_samples = n_samples;
+    }
 
     if (num_threads <= 0) num_threads = omp_get_max_threads();
 

@@ -224,3 +224,44 @@ def test_transfer_function_gaussian():
     assert ctf.n_field_tables == 4
     assert ctf.weight_table_ids[:4] == [3, 3, 3, -1]
     assert ctf.field_table_ids == [0, 1, 2, 3, 0, 0]
+
+
+def test_interpolated_projection_integrates_linear_fields():
+    """InterpolatedProjectionSampler (image_samplers.pyx:368-402): on a
+    vertex-centred linear field the trilinear interpolant is the field, and
+    the midpoint sub-samples integrate a linear function exactly, so each
+    channel is the line integral of its field over the ray's chord in t."""
+    n = 5
+    coef = [(0.3, 1.0, -2.0, 0.5), (1.5, 0.0, 0.7, 0.2), (-0.4, 0.25, 0.5, -1.0)]
+    g = np.linspace(0.0, 1.0, n + 1)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    fields = [a + bx * X + by * Y + bz * Z for a, bx, by, bz in coef]
+    p = _params(11, (0.35, -0.5, 0.79), width=1.3)
+    img = oracle.cast(2, p, [_brick(n, fields)], n_samples=7)
+    for vi in range(11):
+        for vj in range(11):
+            pos, d = _ray(p, vi, vj)
+            t0, t1 = 0.0, 1.0
+            for a in range(3):
+                ta, tb = (0.0 - pos[a]) / d[a], (1.0 - pos[a]) / d[a]
+                t0, t1 = max(t0, min(ta, tb)), min(t1, max(ta, tb))
+            for c, (a0, bx, by, bz) in enumerate(coef):
+                want = 0.0
+                if t1 > t0:
+                    b = np.array([bx, by, bz])
+                    mid = pos + 0.5 * (t0 + t1) * d
+                    want = (a0 + b @ mid) * (t1 - t0)
+                assert abs(img[vi, vj, c] - want) <= 1e-12, (vi, vj, c, img[vi, vj, c], want)
+            assert img[vi, vj, 3] == 0.0
+
+
+def test_interpolated_projection_ignores_mask_and_uses_three_channels():
+    n = 3
+    f = [np.full((n + 1,) * 3, float(k + 1)) for k in range(5)]
+    p = _params(6, (0.2, 0.3, 0.93))
+    a = oracle.cast(2, p, [_brick(n, f, mask=np.zeros((n,) * 3, np.uint8))], n_samples=4)
+    b = oracle.cast(2, p, [_brick(n, f)], n_samples=4)
+    assert np.array_equal(a, b)
+    assert np.all(a[..., 3] == 0.0)
+    hit = a[..., 0] > 0
+    assert np.allclose(a[..., 1][hit], 2 * a[..., 0][hit]) and np.allclose(a[..., 2][hit], 3 * a[..., 0][hit])

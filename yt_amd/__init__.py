@@ -10,9 +10,10 @@ the camera/lens parameters and the transfer functions.
 
 from . import _lib
 from .amr_kdtree import AMRKDTree
-from .camera import Camera, Orientation, PlaneParallelLens, off_axis_camera
+from .camera import Camera, Orientation, PerspectiveLens, PlaneParallelLens, off_axis_camera
 from .data import load_amr_grids, load_uniform_grid
-from .image_samplers import ImageSampler, ProjectionSampler, VolumeRenderSampler
+from .image_samplers import (ImageSampler, InterpolatedProjectionSampler, ProjectionSampler,
+                             VolumeRenderSampler)
 from .off_axis_projection import off_axis_projection
 from .partitioned_grid import PartitionedGrid
 from .transfer_functions import (
@@ -27,9 +28,11 @@ __all__ = [
     "Camera",
     "ColorTransferFunction",
     "ImageSampler",
+    "InterpolatedProjectionSampler",
     "MultiVariateTransferFunction",
     "Orientation",
     "PartitionedGrid",
+    "PerspectiveLens",
     "PlaneParallelLens",
     "ProjectionSampler",
     "ProjectionTransferFunction",

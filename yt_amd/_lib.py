@@ -23,6 +23,7 @@ YTRC_ECANCELLED = -5
 
 PROJECTION = 0
 VOLUME_RENDER = 1
+INTERPOLATED_PROJECTION = 2
 LENS_PLANE_PARALLEL = 0
 LENS_PER_PIXEL = 1
 

@@ -138,6 +138,7 @@ pfn_t pick_persist(int kind, int nfp, bool rec, int mode)
     const int m = variant_mode(kind, nfp, mode);
     if (m < 0) return nullptr;
     if (kind == 0) return pick_persist_proj(nfp, rec, m);
+    if (kind == 2) return pick_persist_iproj(nfp, rec, m);
     if (m < 2) return pick_persist_vr_a(nfp, rec, m);
     if (m < 4) return pick_persist_vr_b(nfp, rec, m);
     if (m < 8) return pick_persist_vr_c(nfp, rec, m);
@@ -149,6 +150,7 @@ kfn_t pick_kernel(int kind, int nfp, bool rec, int mode)
     const int m = variant_mode(kind, nfp, mode);
     if (m < 0) return nullptr;
     if (kind == 0) return pick_kernel_proj(nfp, rec, m);
+    if (kind == 2) return pick_kernel_iproj(nfp, rec, m);
     if (m < 2) return pick_kernel_vr_a(nfp, rec, m);
     if (m < 4) return pick_kernel_vr_b(nfp, rec, m);
     if (m < 8) return pick_kernel_vr_c(nfp, rec, m);
@@ -754,7 +756,8 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
     P.tsh = tsh;
     ReqHost &q = P.q;
     q.r = r;
-    if (r.kind != YTRC_PROJECTION && r.kind != YTRC_VOLUME_RENDER)
+    if (r.kind != YTRC_PROJECTION && r.kind != YTRC_VOLUME_RENDER &&
+        r.kind != YTRC_INTERPOLATED_PROJECTION)
         return fail(YTRC_ENOTIMPL, tag + "unknown sampler kind");
     if (r.lens != YTRC_LENS_PLANE_PARALLEL && r.lens != YTRC_LENS_PER_PIXEL)
         return fail(YTRC_ENOTIMPL, tag + "unknown lens");
@@ -778,6 +781,8 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
             return fail(YTRC_EINVAL, tag + "ProjectionSampler needs cell-centred bricks");
         if (r.kind == YTRC_VOLUME_RENDER && !hb.vertex)
             return fail(YTRC_EINVAL, tag + "VolumeRenderSampler needs vertex-centred bricks");
+        if (r.kind == YTRC_INTERPOLATED_PROJECTION && !hb.vertex)
+            return fail(YTRC_EINVAL, tag + "InterpolatedProjectionSampler needs vertex-centred bricks");
     }
     q.npix = (int64_t)r.nv[0] * r.nv[1];
     const int th = 64 >> tsh, tw = 1 << tsh;
@@ -789,6 +794,11 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
     memset(&T, 0, sizeof(T));
     if (r.kind == YTRC_PROJECTION) {
         q.nch_out = std::min(4, ds.nf);
+    } else if (r.kind == YTRC_INTERPOLATED_PROJECTION) {
+        if (r.n_samples <= 0) return fail(YTRC_EINVAL, tag + "n_samples must be > 0");
+        if (r.flags & YTRC_FLAG_ERT)
+            return fail(YTRC_ENOTIMPL, tag + "early ray termination applies to volume rendering only");
+        q.nch_out = std::min(3, ds.nf);  // image_samplers.pyx:399
     } else {
         if (r.tf == nullptr) return fail(YTRC_EINVAL, tag + "VolumeRenderSampler needs a transfer function");
         const ytrc_tf_desc &tf = *r.tf;
@@ -965,7 +975,7 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
         if (q.lds > 64 * 1024) return fail(YTRC_EINVAL, tag + "transfer-function tables exceed 64 KiB of LDS");
         q.nch_out = T.grey ? 4 : 3;
     }
-    if (r.kind == YTRC_PROJECTION) q.mode = pp ? 8 : 0;
+    if (r.kind != YTRC_VOLUME_RENDER) q.mode = pp ? 8 : 0;
     q.kfn = pick_kernel(r.kind, q.nfp, rec, q.mode);
     if (q.kfn == nullptr) return fail(YTRC_ENOTIMPL, tag + "no kernel for this field count");
     if (r.image_in != nullptr) q.image_in.assign(r.image_in, r.image_in + q.npix * 4);

@@ -18,7 +18,7 @@ ytrc::pfn_t variant_p(int nfp, bool rec)
         if constexpr (wide) return rec ? k_persist<KIND, 4, true, MODE> : k_persist<KIND, 4, false, MODE>;
         return nullptr;
     default:
-        if constexpr (wide && KIND == 1) return rec ? k_persist<KIND, 8, true, MODE> : k_persist<KIND, 8, false, MODE>;
+        if constexpr (wide && KIND != 0) return rec ? k_persist<KIND, 8, true, MODE> : k_persist<KIND, 8, false, MODE>;
         return nullptr;
     }
 }
@@ -34,7 +34,7 @@ ytrc::kfn_t variant_k(int nfp, bool rec)
         if constexpr (wide) return rec ? k_render<KIND, 4, true, MODE> : k_render<KIND, 4, false, MODE>;
         return nullptr;
     default:
-        if constexpr (wide && KIND == 1) return rec ? k_render<KIND, 8, true, MODE> : k_render<KIND, 8, false, MODE>;
+        if constexpr (wide && KIND != 0) return rec ? k_render<KIND, 8, true, MODE> : k_render<KIND, 8, false, MODE>;
         return nullptr;
     }
 }

@@ -517,6 +517,70 @@ struct VRSampler {
     }
 };
 
+// InterpolatedProjectionSampler.sample (image_samplers.pyx:368-402): the
+// trilinear value of each field at the visit's n_samples midpoint
+// sub-samples, times dt, summed into rgba[j], j < min(3, n_fields); no mask
+// test.  The interpolant along the sub-samples is the visit's cubic in s, so
+// the sum over s = 0..n-1 is n k0 + S1 k1 + S2 k2 + S3 k3 with the power
+// sums S1, S2, S3 of 0..n-1, formed once per visit in FP64.
+template <int NFP, bool REC>
+struct IProjSampler {
+    const float *data;
+    BrickReg b;
+    double pa[3];
+    double pb[3];
+    double inv_ns;
+    int n_samples;
+    double s1, s2, s3;
+    int nch;
+    double st[3];
+    Rec *rec;
+
+    __device__ __forceinline__ void operator()(double en, double ex, int c0, int c1, int c2)
+    {
+        if (REC) rec->visit(en, ex, c0, c1, c2);
+        const int d1 = b.d1, d2 = b.d2;
+        if (!((unsigned)c0 < (unsigned)b.d0 && (unsigned)c1 < (unsigned)d1 &&
+              (unsigned)c2 < (unsigned)d2))
+            return;
+        if (REC) rec->samples += n_samples;
+        const int s1i = d2 + 1;
+        const int s0i = (d1 + 1) * (d2 + 1);
+        const int voff = (int)__umul24(c0, s0i) + (int)__umul24(c1, s1i) + c2;
+        const double dt = (ex - en) * inv_ns;
+        const double tt = en + 0.5 * dt;
+        float dp[3], ds[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int ci = (i == 0) ? c0 : ((i == 1) ? c1 : c2);
+            dp[i] = (float)(__builtin_fma(tt, pa[i], pb[i]) - (double)ci);
+            ds[i] = (float)(pa[i] * dt);
+        }
+        float cv[8][NFP];
+        const float *p0 = data + (size_t)voff * NFP;
+        FieldLoad<NFP>::get(p0, cv[0]);
+        FieldLoad<NFP>::get(p0 + NFP, cv[1]);
+        FieldLoad<NFP>::get(p0 + (size_t)s1i * NFP, cv[2]);
+        FieldLoad<NFP>::get(p0 + (size_t)(s1i + 1) * NFP, cv[3]);
+        FieldLoad<NFP>::get(p0 + (size_t)s0i * NFP, cv[4]);
+        FieldLoad<NFP>::get(p0 + (size_t)(s0i + 1) * NFP, cv[5]);
+        FieldLoad<NFP>::get(p0 + (size_t)(s0i + s1i) * NFP, cv[6]);
+        FieldLoad<NFP>::get(p0 + (size_t)(s0i + s1i + 1) * NFP, cv[7]);
+        const double ns = (double)n_samples;
+#pragma unroll
+        for (int f = 0; f < (NFP < 3 ? NFP : 3); f++) {
+            if (f >= nch) break;
+            const float cf[8] = {cv[0][f], cv[1][f], cv[2][f], cv[3][f],
+                                 cv[4][f], cv[5][f], cv[6][f], cv[7][f]};
+            float k[4];
+            trilinear_cubic(cf, dp, ds, k);
+            const double sum = __builtin_fma(
+                s3, (double)k[3], __builtin_fma(s2, (double)k[2], __builtin_fma(s1, (double)k[1], ns * (double)k[0])));
+            st[f] = __builtin_fma(sum, dt, st[f]);
+        }
+    }
+};
+
 // Casts the rays of one 8x8 tile of image `im` through its bricks.  `im`
 // and `tile` are wave-uniform.
 template <int KIND, int NFP, bool REC, int MODE>
@@ -644,6 +708,28 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
                     ps[f] = s.s[f];
                     pc[f] = s.c[f];
                 }
+            } else if constexpr (KIND == 2) {
+                IProjSampler<NFP, REC> s;
+                s.data = im.pool + b.data_off;
+                s.b = brick_reg(b);
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    s.pa[i] = PP ? dir[i] * b.idds[i] : uni_d(dir[i] * b.idds[i]);
+                    s.pb[i] = (pos[i] - b.le[i]) * b.idds[i];
+                }
+                s.inv_ns = im.inv_ns;
+                s.n_samples = im.n_samples;
+                const double ns = (double)im.n_samples;
+                s.s1 = 0.5 * ns * (ns - 1.0);
+                s.s2 = (ns - 1.0) * ns * (2.0 * ns - 1.0) / 6.0;
+                s.s3 = s.s1 * s.s1;
+                s.nch = im.nch_out;
+#pragma unroll
+                for (int c = 0; c < 3; c++) s.st[c] = st[c];
+                s.rec = &rec;
+                walk_loop(b, w, max_t, s);
+#pragma unroll
+                for (int c = 0; c < 3; c++) st[c] = s.st[c];
             } else {
                 VRSampler<NFP, REC, MODE> s;
                 s.data = im.pool + b.data_off;

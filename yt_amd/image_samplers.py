@@ -102,14 +102,14 @@ class ImageSampler:
             return None
         pgs, self._pending = self._pending, []
         ctx = _lib.default_context()
-        vc = self.kind == _lib.VOLUME_RENDER
+        vc = self.kind != _lib.PROJECTION
         bricks = [(pg.LeftEdge, pg.RightEdge, pg.dims, vc) for pg in pgs]
         fields = [pg.my_data[:pg.n_fields] for pg in pgs]
         for pg in pgs:
             if pg.vertex_centred != vc:
-                raise ValueError("ProjectionSampler needs cell-centred bricks and "
-                                 "VolumeRenderSampler vertex-centred ones")
-        masks = [pg.source_mask for pg in pgs] if vc else None
+                raise ValueError("ProjectionSampler needs cell-centred bricks; VolumeRenderSampler "
+                                 "and InterpolatedProjectionSampler vertex-centred ones")
+        masks = [pg.source_mask for pg in pgs] if self.kind == _lib.VOLUME_RENDER else None
         ds_id = ctx.upload(bricks, fields, masks)
         try:
             spec = _lib.RequestSpec(
@@ -132,6 +132,20 @@ class ProjectionSampler(ImageSampler):
     """image_samplers.pyx:327-343: line integral of min(4, n_fields) fields."""
 
     kind = _lib.PROJECTION
+
+
+class InterpolatedProjectionSampler(ImageSampler):
+    """image_samplers.pyx:344-402: vertex-centred bricks, n_samples trilinear
+    sub-samples per cell, each field times dt into rgba[j], j < min(3,
+    n_fields); no mask test."""
+
+    kind = _lib.INTERPOLATED_PROJECTION
+
+    def __init__(self, vp_pos, vp_dir, center, bounds, image, x_vec, y_vec, width, volume_method,
+                 n_samples=10, **kwargs):
+        super().__init__(vp_pos, vp_dir, center, bounds, image, x_vec, y_vec, width,
+                         volume_method, **kwargs)
+        self.n_samples = int(n_samples)
 
 
 class VolumeRenderSampler(ImageSampler):

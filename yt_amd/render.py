@@ -105,6 +105,16 @@ class ResidentVolume:
                                 self.vr_order(params["viewpoint"]), n_samples=n_samples, tf=tfc,
                                 record=record, ert=ert, zbuffer=zbuffer)
 
+    def interpolated_projection_request(self, params, n_samples=10, record=False, zbuffer=None):
+        """InterpolatedProjectionSampler over the kd bricks (the sampler of a
+        "projection" VolumeSource, render_source.py:464-488), in the same
+        brick order as a volume render."""
+        if self.vr_ds_id is None:
+            raise ValueError("volume was not uploaded with vertex-centred bricks")
+        return _lib.RequestSpec(_lib.INTERPOLATED_PROJECTION, params, self.vr_ds_id,
+                                self.vr_order(params["viewpoint"]), n_samples=n_samples,
+                                record=record, zbuffer=zbuffer)
+
     def free(self):
         for i in (self.proj_ds_id, self.vr_ds_id):
             if i is not None:
