@@ -44,6 +44,10 @@ extern "C" {
 #define YTRC_PROJECTION 0
 #define YTRC_VOLUME_RENDER 1
 #define YTRC_INTERPOLATED_PROJECTION 2
+/* LightSourceRenderSampler (image_samplers.pyx:496-600): the volume render's
+ * tables with a directional light (light_dir, light_rgba of the request),
+ * no mask test (the sampler of old_camera.py:688). */
+#define YTRC_LIGHT_SOURCE 3
 
 /* Lenses: plane-parallel (lenses.pyx:22-55, :179-192), and per-pixel rays
  * (generate_vector_info_null, lenses.pyx:196-207), which is how yt runs the
@@ -183,6 +187,10 @@ typedef struct {
     const double *pix_pos;
     const double *pix_dir;
     const double *camera_data;
+    /* YTRC_LIGHT_SOURCE only: unit light direction and light colour
+     * (image_samplers.pyx:510-531 normalises the direction). */
+    double light_dir[3];
+    double light_rgba[4];
 } ytrc_request;
 
 /* Builds a batch: validates requests, computes the per-brick pixel extents

@@ -63,6 +63,8 @@ def lib():
         L.orc_trilinear.restype = C.c_int
         L.orc_trilinear.argtypes = [I32, D, C.c_int, I32, D, D]
         L.orc_max_threads.restype = C.c_int
+        L.orc_gradients.restype = C.c_int
+        L.orc_gradients.argtypes = [I32, D, C.c_int, I32, D, D]
         _lib = L
     return _lib
 
@@ -92,6 +94,8 @@ class _TFDesc(C.Structure):
         ("weight_table_ids", C.c_int32 * 6),
         ("field_table_ids", C.c_int32 * 6),
         ("grey_opacity", C.c_int32),
+        ("light_dir", C.c_double * 3),
+        ("light_rgba", C.c_double * 4),
     ]
 
 
@@ -134,7 +138,7 @@ def cast(kind, params, bricks, order=None, tf=None, n_samples=10, image=None, zb
         order = np.arange(n, dtype=np.int32)
     order = np.ascontiguousarray(order, dtype=np.int32)
     tfd = None
-    if kind == 1:
+    if kind in (1, 3):
         tfd = _TFDesc()
         tables = tf["tables"]
         tfd.n_fits = len(tables)
@@ -151,6 +155,8 @@ def cast(kind, params, bricks, order=None, tf=None, n_samples=10, image=None, zb
             tfd.weight_table_ids[i] = tf["weight_table_ids"][i]
             tfd.field_table_ids[i] = tf["field_table_ids"][i]
         tfd.grey_opacity = int(tf.get("grey_opacity", 0))
+        tfd.light_dir[:] = [float(x) for x in tf.get("light_dir", (0.0, 0.0, 0.0))]
+        tfd.light_rgba[:] = [float(x) for x in tf.get("light_rgba", (0.0, 0.0, 0.0, 0.0))]
     zb = None
     if zbuffer is not None:
         zb = np.ascontiguousarray(zbuffer, dtype=np.float64)
@@ -254,6 +260,33 @@ def trilinear(dims, vdata, cells, dps):
     p = np.ascontiguousarray(dps, dtype=np.float64)
     out = np.zeros(len(c), dtype=np.float64)
     L.orc_trilinear(_i(d), _d(vd), len(c), _i(c), _d(p), _d(out))
+    return out
+
+
+def gradients(dims, vdata, cells, dps):
+    """eval_gradient (fixed_interpolator.cpp:116-148) at each (cell, dp)."""
+    L = lib()
+    d = np.ascontiguousarray(dims, dtype=np.int32)
+    vd = np.ascontiguousarray(vdata, dtype=np.float64)
+    c = np.ascontiguousarray(cells, dtype=np.int32)
+    p = np.ascontiguousarray(dps, dtype=np.float64)
+    out = np.zeros((len(c), 3), dtype=np.float64)
+    L.orc_gradients(_i(d), _d(vd), len(c), _i(c), _d(p), _d(out))
+    return out
+
+
+def ref_eval_gradient(dims, vdata, cells, dps):
+    """yt's eval_gradient (fixed_interpolator.cpp:116-148) from oracle/_ref."""
+    L = ref_lib()
+    ds = (C.c_int * 3)(*[int(x) for x in dims])
+    vd = np.ascontiguousarray(vdata, dtype=np.float64).ravel()
+    out = np.zeros((len(cells), 3))
+    for n, (c, dp) in enumerate(zip(cells, dps)):
+        off = (int(c[0]) * (dims[1] + 1) + int(c[1])) * (dims[2] + 1) + int(c[2])
+        p = (C.c_double * 3)(*[float(x) for x in dp])
+        g = (C.c_double * 3)()
+        L._Z13eval_gradientPiPdS0_S0_(ds, p, _d(vd[off:]), g)
+        out[n] = list(g)
     return out
 
 

@@ -273,6 +273,74 @@ static inline double orc_offset_interpolate(const int ds[3], const double dp[3],
     return vz[0];
 }
 
+/* eval_gradient: fixed_interpolator.cpp:116-148.  Finite differences of
+ * offset_interpolate at +-0.05 cell widths (one-sided within 0.05 of a
+ * face), normalised and negated; 0 where the difference vector is 0.
+ * Pinned bit-for-bit against the reference's own compiled function
+ * (tests/test_oracle_ref.py). */
+static inline void orc_eval_gradient(const int ds[3], const double dp_in[3], const double *data,
+                                     double grad[3])
+{
+    double dp[3] = {dp_in[0], dp_in[1], dp_in[2]};
+    double denom, plus, minus, backup, normval = 0.0;
+    for (int i = 0; i < 3; i++) {
+        backup = dp[i];
+        grad[i] = 0.0;
+        if (dp[i] >= 0.95) {
+            plus = dp[i];
+            minus = dp[i] - 0.05;
+        } else if (dp[i] <= 0.05) {
+            plus = dp[i] + 0.05;
+            minus = 0.0;
+        } else {
+            plus = dp[i] + 0.05;
+            minus = dp[i] - 0.05;
+        }
+        denom = plus - minus;
+        dp[i] = plus;
+        grad[i] += orc_offset_interpolate(ds, dp, data) / denom;
+        dp[i] = minus;
+        grad[i] -= orc_offset_interpolate(ds, dp, data) / denom;
+        dp[i] = backup;
+        normval += grad[i] * grad[i];
+    }
+    if (normval != 0.0) {
+        normval = sqrt(normval);
+        for (int i = 0; i < 3; i++) grad[i] /= -normval;
+    } else {
+        grad[0] = grad[1] = grad[2] = 0.0;
+    }
+}
+
+/* FIT_eval_transfer_with_light: field_interpolation_tables.pxd:115-143. */
+static inline void orc_fit_eval_transfer_with_light(double dt, const double *dvs, const double *grad,
+                                                    const double *l_dir, const double *l_rgba,
+                                                    double *rgba, int n_fits, const orc_fit *fits,
+                                                    const int field_table_ids[6], int grey_opacity)
+{
+    int i, fid;
+    double ta, dot_prod = 0.0;
+    double istorage[6] = {0, 0, 0, 0, 0, 0};
+    double trgba[6];
+    for (i = 0; i < 3; i++) dot_prod += l_dir[i] * grad[i];
+    for (i = 0; i < n_fits; i++) istorage[i] = orc_fit_get_value(&fits[i], dvs);
+    for (i = 0; i < n_fits; i++) {
+        fid = fits[i].weight_table_id;
+        if (fid != -1) istorage[i] *= istorage[fid];
+    }
+    for (i = 0; i < 6; i++) trgba[i] = istorage[field_table_ids[i]];
+    if (grey_opacity == 1) {
+        ta = y_fmax(1.0 - dt * (trgba[0] + trgba[1] + trgba[2]), 0.0);
+        for (i = 0; i < 3; i++)
+            rgba[i] = (1. - ta) * trgba[i] * (1. + dot_prod * l_rgba[i]) + ta * rgba[i];
+    } else {
+        for (i = 0; i < 3; i++) {
+            ta = y_fmax(1.0 - dt * trgba[i], 0.0);
+            rgba[i] = (1. - ta) * trgba[i] * (1. + dot_prod * l_rgba[i]) + ta * rgba[i];
+        }
+    }
+}
+
 /* ---------------------------------------------------------------------- */
 /* Accumulator: image_samplers.pxd:34-36 (ImageAccumulator) plus the
  * VolumeRenderAccumulator of image_samplers.pyx:45-55, and optional
@@ -283,6 +351,8 @@ typedef struct {
     const orc_fit *fits;
     int field_table_ids[6];
     int grey_opacity;
+    double light_dir[3];  /* LightSourceRenderSampler: normalised direction */
+    double light_rgba[4];
 } orc_vra;
 
 typedef struct {
@@ -375,6 +445,37 @@ static void orc_interpolated_projection_sample(const orc_vc *vc, const double v_
     }
 }
 
+/* LightSourceRenderSampler.sample: image_samplers.pyx:548-590.  As the
+ * volume renderer but without the mask test, with a gradient of field 0 per
+ * sub-sample and FIT_eval_transfer_with_light. */
+static void orc_light_source_sample(const orc_vc *vc, const double v_pos[3], const double v_dir[3],
+                                    double enter_t, double exit_t, const int index[3], void *data)
+{
+    orc_acc *im = (orc_acc *)data;
+    const orc_vra *vri = im->vra;
+    if (!orc_in_range(vc, index)) return;
+    int offset = index[0] * (vc->dims[1] + 1) * (vc->dims[2] + 1) + index[1] * (vc->dims[2] + 1) +
+                 index[2];
+    double dp[3], ds[3], grad[3];
+    double dt = (exit_t - enter_t) / (double)vri->n_samples;
+    double dvs[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 3; i++) {
+        dp[i] = (enter_t + 0.5 * dt) * v_dir[i] + v_pos[i];
+        dp[i] -= (double)index[i] * vc->dds[i] + vc->left_edge[i];
+        dp[i] *= vc->idds[i];
+        ds[i] = v_dir[i] * vc->idds[i] * dt;
+    }
+    for (int s = 0; s < vri->n_samples; s++) {
+        for (int j = 0; j < vc->n_fields; j++)
+            dvs[j] = orc_offset_interpolate(vc->dims, dp, vc->data[j] + offset);
+        orc_eval_gradient(vc->dims, dp, vc->data[0] + offset, grad);
+        orc_fit_eval_transfer_with_light(dt, dvs, grad, vri->light_dir, vri->light_rgba, im->rgba,
+                                         vri->n_fits, vri->fits, vri->field_table_ids,
+                                         vri->grey_opacity);
+        for (int j = 0; j < 3; j++) dp[j] += ds[j];
+    }
+}
+
 /* Visit instrumentation: a 64-bit order-sensitive hash over
  * (brick id, i, j, k, bits(enter_t), bits(exit_t)) of every sampler call. */
 static inline uint64_t orc_mix(uint64_t h, uint64_t w)
@@ -412,6 +513,13 @@ static void orc_interpolated_projection_sample_rec(const orc_vc *vc, const doubl
 {
     orc_record((orc_acc *)data, index, enter_t, exit_t);
     orc_interpolated_projection_sample(vc, v_pos, v_dir, enter_t, exit_t, index, data);
+}
+static void orc_light_source_sample_rec(const orc_vc *vc, const double v_pos[3],
+                                        const double v_dir[3], double enter_t, double exit_t,
+                                        const int index[3], void *data)
+{
+    orc_record((orc_acc *)data, index, enter_t, exit_t);
+    orc_light_source_sample(vc, v_pos, v_dir, enter_t, exit_t, index, data);
 }
 static void orc_volume_render_sample_rec(const orc_vc *vc, const double v_pos[3],
                                          const double v_dir[3], double enter_t, double exit_t,
@@ -578,13 +686,15 @@ typedef struct {
     int32_t weight_table_ids[6];
     int32_t field_table_ids[6];
     int32_t grey_opacity;
+    double light_dir[3];  /* LightSourceRenderSampler only */
+    double light_rgba[4];
 } orc_tf_desc;
 
 /*
  * orc_cast_lens: ImageSampler.__call__ -> cast_through_kdtree
  * (image_samplers.pyx:132-197) for each brick of `order` in sequence.
  * kind 0 = ProjectionSampler, 1 = VolumeRenderSampler,
- * 2 = InterpolatedProjectionSampler.
+ * 2 = InterpolatedProjectionSampler, 3 = LightSourceRenderSampler.
  * lens 0: plane-parallel (vp_pos/vp_dir/center/bounds/x_vec/y_vec/width);
  * lens 1: per-pixel rays pix_pos/pix_dir (nv0, nv1, 3), with the perspective
  * extent when camera_data (5 x 3) is given, else the whole image.  OpenMP: static schedule, chunk 100, as the
@@ -604,7 +714,7 @@ int orc_cast_lens(int kind, int lens, const double *pix_pos, const double *pix_d
 {
     if (nv0 <= 0 || nv1 <= 0 || n_fields <= 0 || n_fields > 6) return -1;
     if (lens != 0 && (pix_pos == NULL || pix_dir == NULL)) return -1;
-    if (kind == 1 && (tf == NULL || tf->n_fits > 6 || n_samples <= 0)) return -1;
+    if ((kind == 1 || kind == 3) && (tf == NULL || tf->n_fits > 6 || n_samples <= 0)) return -1;
     orc_cam cam;
     memcpy(cam.vp_pos, vp_pos, sizeof(cam.vp_pos));
     memcpy(cam.vp_dir, vp_dir, sizeof(cam.vp_dir));
@@ -622,7 +732,9 @@ int orc_cast_lens(int kind, int lens, const double *pix_pos, const double *pix_d
     orc_fit fits[6];
     orc_vra vra;
     memset(&vra, 0, sizeof(vra));
-    if (kind == 1) {
+    if (kind == 1 || kind == 3) {
+        for (int i = 0; i < 3; i++) vra.light_dir[i] = tf->light_dir[i];
+        for (int i = 0; i < 4; i++) vra.light_rgba[i] = tf->light_rgba[i];
         vra.n_fits = tf->n_fits;
         vra.n_samples = n_samples;
         vra.grey_opacity = tf->grey_opacity;
@@ -638,6 +750,8 @@ int orc_cast_lens(int kind, int lens, const double *pix_pos, const double *pix_d
         fn = rec ? orc_projection_sample_rec : orc_projection_sample;
     else if (kind == 2)
         fn = rec ? orc_interpolated_projection_sample_rec : orc_interpolated_projection_sample;
+    else if (kind == 3)
+        fn = rec ? orc_light_source_sample_rec : orc_light_source_sample;
     else
         fn = rec ? orc_volume_render_sample_rec : orc_volume_render_sample;
     if (kind == 2) {
@@ -707,7 +821,7 @@ int orc_cast_lens(int kind, int lens, const double *pix_pos, const double *pix_d
             }
         }
     }
-    if (kind == 1)
+    if (kind == 1 || kind == 3)
         for (int i = 0; i < tf->n_fits; i++) {
             free(fits[i].d0);
             free(fits[i].dy);
@@ -839,6 +953,19 @@ int orc_trilinear(const int32_t *dims, const double *vdata, int n, const int32_t
         const int32_t *c = cells + 3 * p;
         int offset = c[0] * (ds[1] + 1) * (ds[2] + 1) + c[1] * (ds[2] + 1) + c[2];
         out[p] = orc_offset_interpolate(ds, dps + 3 * p, vdata + offset);
+    }
+    return 0;
+}
+
+/* eval_gradient at n positions of cells of one vertex-centred brick. */
+int orc_gradients(const int32_t *dims, const double *vdata, int n, const int32_t *cells,
+                  const double *dps, double *out)
+{
+    int ds[3] = {dims[0], dims[1], dims[2]};
+    for (int p = 0; p < n; p++) {
+        const int32_t *c = cells + 3 * p;
+        int offset = c[0] * (ds[1] + 1) * (ds[2] + 1) + c[1] * (ds[2] + 1) + c[2];
+        orc_eval_gradient(ds, dps + 3 * p, vdata + offset, out + 3 * p);
     }
     return 0;
 }

@@ -25,3 +25,22 @@ def test_trilinear_bit_equal_to_reference(dims, seed):
     got = oracle.trilinear(dims, v, cells, dps)
     want = oracle.ref_offset_interpolate(dims, v, cells, dps)
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+
+
+@pytest.mark.parametrize("dims,seed", [((5, 7, 3), 11), ((8, 8, 8), 12)])
+def test_gradient_bit_equal_to_reference(dims, seed):
+    """eval_gradient (fixed_interpolator.cpp:116-148), including the
+    one-sided differences within 0.05 of a face and the thresholds
+    themselves."""
+    rng = np.random.default_rng(seed)
+    v = np.exp(rng.normal(size=tuple(d + 1 for d in dims)))
+    n = 300
+    cells = np.stack([rng.integers(0, d, n) for d in dims], axis=1).astype(np.int32)
+    dps = rng.random((n, 3))
+    dps[:30] = rng.choice([0.0, 0.05, 0.95, 1.0, 0.04999999, 0.95000001], size=(30, 3))
+    v[0, 0, 0] = v[0, 0, 1]  # a few flat corners
+    got = oracle.gradients(dims, v, cells, dps)
+    want = oracle.ref_eval_gradient(dims, v, cells, dps)
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+    flat = np.ones(tuple(d + 1 for d in dims))
+    assert np.all(oracle.gradients(dims, flat, cells[:10], dps[:10]) == 0.0)

@@ -265,3 +265,44 @@ def test_interpolated_projection_ignores_mask_and_uses_three_channels():
     assert np.all(a[..., 3] == 0.0)
     hit = a[..., 0] > 0
     assert np.allclose(a[..., 1][hit], 2 * a[..., 0][hit]) and np.allclose(a[..., 2][hit], 3 * a[..., 0][hit])
+
+
+@pytest.mark.parametrize("field", ["constant", "linear"])
+def test_light_source_closed_form(field):
+    """LightSourceRenderSampler (image_samplers.pyx:548-590) with a constant
+    transfer function.  On a linear field the finite-difference gradient is
+    minus the unit gradient everywhere, so dot = -l . g_hat.  On a constant
+    field the differences are rounding noise that eval_gradient normalises
+    to an arbitrary unit vector (as the reference does), so that case uses
+    l_rgba = 0, which makes the light term 1.  Each channel then follows
+    rgba' = (1 - ta) e (1 + dot l_c) + ta rgba with ta = max(1 - dt e, 0)
+    over the visit log's sub-sample dt (field_interpolation_tables.pxd:115-143)."""
+    from tests.helpers import tf_dict
+
+    n = 5
+    rgba_tab = (0.6, 0.3, 0.8, 0.5)
+    tf = _ctf_const(-10.0, 10.0, 16, rgba_tab)
+    g = np.linspace(0.0, 1.0, n + 1)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    b = np.array([0.8, -0.5, 0.3])
+    vdata = np.full((n + 1,) * 3, 0.4) if field == "constant" else 0.4 + b[0] * X + b[1] * Y + b[2] * Z
+    ldir = np.array([1.0, 1.0, 1.0]) / np.sqrt(3.0)
+    lrgba = (0.0, 0.0, 0.0, 0.0) if field == "constant" else (1.0, 0.5, 0.25, 1.0)
+    td = dict(tf_dict(tf), light_dir=ldir, light_rgba=lrgba)
+    p = _params(7, (0.3, 0.45, 0.84), width=1.1)
+    img = oracle.cast(3, p, [_brick(n, [vdata], mask=np.zeros((n,) * 3, np.uint8))], tf=td,
+                      n_samples=4)
+    a = rgba_tab[3]
+    e = [rgba_tab[c] * a for c in range(3)]
+    dot = 0.0 if field == "constant" else float(-ldir @ (b / np.linalg.norm(b)))
+    for vi in range(7):
+        for vj in range(7):
+            pos, d = _ray(p, vi, vj)
+            _, _, t = oracle.walk_log((0, 0, 0), (1, 1, 1), [n] * 3, pos, d)
+            dts = np.repeat((t[:, 1] - t[:, 0]) / 4.0, 4)
+            for c in range(3):
+                acc = 0.0
+                for dt in dts:
+                    ta = max(1.0 - dt * e[c], 0.0)
+                    acc = (1.0 - ta) * e[c] * (1.0 + dot * lrgba[c]) + ta * acc
+                assert abs(img[vi, vj, c] - acc) <= 1e-11, (vi, vj, c)

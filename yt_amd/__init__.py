@@ -12,8 +12,8 @@ from . import _lib
 from .amr_kdtree import AMRKDTree
 from .camera import Camera, Orientation, PerspectiveLens, PlaneParallelLens, off_axis_camera
 from .data import load_amr_grids, load_uniform_grid
-from .image_samplers import (ImageSampler, InterpolatedProjectionSampler, ProjectionSampler,
-                             VolumeRenderSampler)
+from .image_samplers import (ImageSampler, InterpolatedProjectionSampler,
+                             LightSourceRenderSampler, ProjectionSampler, VolumeRenderSampler)
 from .off_axis_projection import off_axis_projection
 from .partitioned_grid import PartitionedGrid
 from .transfer_functions import (
@@ -29,6 +29,7 @@ __all__ = [
     "ColorTransferFunction",
     "ImageSampler",
     "InterpolatedProjectionSampler",
+    "LightSourceRenderSampler",
     "MultiVariateTransferFunction",
     "Orientation",
     "PartitionedGrid",

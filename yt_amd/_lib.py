@@ -24,6 +24,7 @@ YTRC_ECANCELLED = -5
 PROJECTION = 0
 VOLUME_RENDER = 1
 INTERPOLATED_PROJECTION = 2
+LIGHT_SOURCE = 3
 LENS_PLANE_PARALLEL = 0
 LENS_PER_PIXEL = 1
 
@@ -149,6 +150,8 @@ class Request(C.Structure):
         ("pix_pos", C.POINTER(C.c_double)),
         ("pix_dir", C.POINTER(C.c_double)),
         ("camera_data", C.POINTER(C.c_double)),
+        ("light_dir", C.c_double * 3),
+        ("light_rgba", C.c_double * 4),
     ]
 
 
@@ -435,8 +438,14 @@ class RequestSpec:
     """One image: sampler constructor parameters + brick order."""
 
     def __init__(self, kind, params, ds_id, order, n_samples=10, tf=None, zbuffer=None,
-                 image_in=None, record=False, ert=False):
+                 image_in=None, record=False, ert=False, light_dir=(1.0, 1.0, 1.0),
+                 light_rgba=(1.0, 1.0, 1.0, 1.0)):
         self.kind = kind
+        # LightSourceRenderSampler normalises the light direction
+        # (image_samplers.pyx:526-528).
+        ld = np.asarray(light_dir, dtype=np.float64).ravel()[:3]
+        self.light_dir = ld / np.sqrt(ld[0] * ld[0] + ld[1] * ld[1] + ld[2] * ld[2])
+        self.light_rgba = np.asarray(light_rgba, dtype=np.float64).ravel()[:4]
         self.params = params
         self.ds_id = ds_id
         self.order = np.ascontiguousarray(order, dtype=np.int32)
@@ -500,6 +509,8 @@ class RequestSpec:
         if self.tf is not None:
             r.tf = C.pointer(self.tf.desc)
         r.flags = (FLAG_RECORD if self.record else 0) | (FLAG_ERT if self.ert else 0)
+        r.light_dir[:] = [float(x) for x in self.light_dir]
+        r.light_rgba[:] = [float(x) for x in self.light_rgba]
         return r
 
 

@@ -126,6 +126,9 @@ namespace {
 // MODE (VR: bit 0 ColorTransferFunction path, bit 1 n_samples == 10, bit 2
 // early ray termination, bit 3 per-pixel lens; projection: bit 3).  NFP 4
 // and 8 variants keep only bits 0 and 3.
+// Sampler kinds that read a transfer function.
+inline bool has_tf(int kind) { return kind == YTRC_VOLUME_RENDER || kind == YTRC_LIGHT_SOURCE; }
+
 int variant_mode(int kind, int nfp, int mode)
 {
     if (nfp >= 4) return (mode & 4) ? -1 : (mode & 9);
@@ -139,6 +142,7 @@ pfn_t pick_persist(int kind, int nfp, bool rec, int mode)
     if (m < 0) return nullptr;
     if (kind == 0) return pick_persist_proj(nfp, rec, m);
     if (kind == 2) return pick_persist_iproj(nfp, rec, m);
+    if (kind == 3) return pick_persist_light(nfp, rec, m);
     if (m < 2) return pick_persist_vr_a(nfp, rec, m);
     if (m < 4) return pick_persist_vr_b(nfp, rec, m);
     if (m < 8) return pick_persist_vr_c(nfp, rec, m);
@@ -151,6 +155,7 @@ kfn_t pick_kernel(int kind, int nfp, bool rec, int mode)
     if (m < 0) return nullptr;
     if (kind == 0) return pick_kernel_proj(nfp, rec, m);
     if (kind == 2) return pick_kernel_iproj(nfp, rec, m);
+    if (kind == 3) return pick_kernel_light(nfp, rec, m);
     if (m < 2) return pick_kernel_vr_a(nfp, rec, m);
     if (m < 4) return pick_kernel_vr_b(nfp, rec, m);
     if (m < 8) return pick_kernel_vr_c(nfp, rec, m);
@@ -757,7 +762,7 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
     ReqHost &q = P.q;
     q.r = r;
     if (r.kind != YTRC_PROJECTION && r.kind != YTRC_VOLUME_RENDER &&
-        r.kind != YTRC_INTERPOLATED_PROJECTION)
+        r.kind != YTRC_INTERPOLATED_PROJECTION && r.kind != YTRC_LIGHT_SOURCE)
         return fail(YTRC_ENOTIMPL, tag + "unknown sampler kind");
     if (r.lens != YTRC_LENS_PLANE_PARALLEL && r.lens != YTRC_LENS_PER_PIXEL)
         return fail(YTRC_ENOTIMPL, tag + "unknown lens");
@@ -781,8 +786,8 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
             return fail(YTRC_EINVAL, tag + "ProjectionSampler needs cell-centred bricks");
         if (r.kind == YTRC_VOLUME_RENDER && !hb.vertex)
             return fail(YTRC_EINVAL, tag + "VolumeRenderSampler needs vertex-centred bricks");
-        if (r.kind == YTRC_INTERPOLATED_PROJECTION && !hb.vertex)
-            return fail(YTRC_EINVAL, tag + "InterpolatedProjectionSampler needs vertex-centred bricks");
+        if ((r.kind == YTRC_INTERPOLATED_PROJECTION || r.kind == YTRC_LIGHT_SOURCE) && !hb.vertex)
+            return fail(YTRC_EINVAL, tag + "this sampler needs vertex-centred bricks");
     }
     q.npix = (int64_t)r.nv[0] * r.nv[1];
     const int th = 64 >> tsh, tw = 1 << tsh;
@@ -800,7 +805,9 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
             return fail(YTRC_ENOTIMPL, tag + "early ray termination applies to volume rendering only");
         q.nch_out = std::min(3, ds.nf);  // image_samplers.pyx:399
     } else {
-        if (r.tf == nullptr) return fail(YTRC_EINVAL, tag + "VolumeRenderSampler needs a transfer function");
+        if (r.tf == nullptr) return fail(YTRC_EINVAL, tag + "this sampler needs a transfer function");
+        if (r.kind == YTRC_LIGHT_SOURCE && (r.flags & YTRC_FLAG_ERT))
+            return fail(YTRC_ENOTIMPL, tag + "early ray termination is implemented for VolumeRenderSampler only");
         const ytrc_tf_desc &tf = *r.tf;
         if (tf.n_fits < 0 || tf.n_fits > 6)
             return fail(YTRC_EINVAL, tag + "n_fits must be <= 6 (image_samplers.pyx:430)");
@@ -930,7 +937,7 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
         // ColorTransferFunction pattern (transfer_functions.py:385-408) in
         // transmittance form gets its own kernel path (MODE bit 0); grey or
         // unbounded CTFs take the general per-table path.
-        ctf = ctf && q.tform;
+        ctf = ctf && q.tform && r.kind == YTRC_VOLUME_RENDER;
         if (ctf) {
             // Transmittance form: per bin, the three emissions e_c(f) =
             // (y_c + f dy_c)(y_a + f dy_a) as quadratics in the in-bin
@@ -973,9 +980,10 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
         T.total_bins = tot;
         q.lds = (size_t)tot * sizeof(float4);
         if (q.lds > 64 * 1024) return fail(YTRC_EINVAL, tag + "transfer-function tables exceed 64 KiB of LDS");
-        q.nch_out = T.grey ? 4 : 3;
+        q.nch_out = (T.grey && r.kind == YTRC_VOLUME_RENDER) ? 4 : 3;
     }
     if (r.kind != YTRC_VOLUME_RENDER) q.mode = pp ? 8 : 0;
+    if (r.kind == YTRC_LIGHT_SOURCE) q.tform = 0;
     q.kfn = pick_kernel(r.kind, q.nfp, rec, q.mode);
     if (q.kfn == nullptr) return fail(YTRC_ENOTIMPL, tag + "no kernel for this field count");
     if (r.image_in != nullptr) q.image_in.assign(r.image_in, r.image_in + q.npix * 4);
@@ -1016,7 +1024,7 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
         off = align_up(off + 8 * q.npix, 256);
     }
     P.o_tf = off;
-    if (r.kind == YTRC_VOLUME_RENDER) off = align_up(off + sizeof(DTF), 256);
+    if (has_tf(r.kind)) off = align_up(off + sizeof(DTF), 256);
     P.o_ppos = P.o_pdir = off;
     if (pp) {
         P.o_ppos = off;
@@ -1053,7 +1061,9 @@ int stage_request(ytrc_ctx *ctx, PrepReq &P, char *A, const float4 *tf_table, in
     d.rec_visits = rec ? (int64_t *)(A + P.o_rv) : nullptr;
     d.rec_hash = rec ? (uint64_t *)(A + P.o_rh) : nullptr;
     d.rec_samples = rec ? (int64_t *)(A + P.o_rs) : nullptr;
-    d.tf = (r.kind == YTRC_VOLUME_RENDER) ? (const DTF *)(A + P.o_tf) : nullptr;
+    d.tf = has_tf(r.kind) ? (const DTF *)(A + P.o_tf) : nullptr;
+    for (int i = 0; i < 3; i++) d.ldir[i] = r.light_dir[i];
+    for (int i = 0; i < 4; i++) d.lrgba[i] = r.light_rgba[i];
     d.nv0 = r.nv[0];
     d.nv1 = r.nv[1];
     d.kind = r.kind;
@@ -1086,7 +1096,7 @@ int stage_request(ytrc_ctx *ctx, PrepReq &P, char *A, const float4 *tf_table, in
             for (int c = 0; c < 4; c++) stage[(size_t)c * q.npix + p] = (float)r.image_in[p * 4 + c];
         HIPCHK(hipMemcpy(A + P.o_init, stage.data(), sizeof(float) * 4 * q.npix, hipMemcpyHostToDevice));
     }
-    if (r.kind == YTRC_VOLUME_RENDER) {
+    if (has_tf(r.kind)) {
         DTF T = P.T;
         T.table = tf_table + tf_slot;
         d.tf_slot = tf_slot;
@@ -1157,7 +1167,7 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
     std::map<std::string, int> tf_slot_of;
     std::vector<int> slot(n_req, 0);
     for (int i = 0; i < n_req; i++) {
-        if (reqs[i].kind != YTRC_VOLUME_RENDER) continue;
+        if (!has_tf(reqs[i].kind)) continue;
         std::string key((const char *)P[i].tft.data(), P[i].tft.size() * sizeof(float4));
         auto it = tf_slot_of.find(key);
         if (it == tf_slot_of.end()) {
@@ -1193,7 +1203,7 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
         g.fn = pick_persist(g.kind, g.nfp, g.rec, g.mode);
         g.n_items = (int)items.size();
         g.counter = (int)B->groups.size();
-        g.lds = (g.kind == YTRC_VOLUME_RENDER) ? sizeof(float4) * tf_all.size() : 0;
+        g.lds = has_tf(g.kind) ? sizeof(float4) * tf_all.size() : 0;
         if (g.lds > 64 * 1024)
             return bad(YTRC_EINVAL, "ytrc_batch_create: the batch's distinct transfer functions exceed 64 KiB of LDS");
         B->groups.push_back(g);
@@ -1699,7 +1709,7 @@ int srv_pump(ytrc_server *S)
         HIPCHK(hipGetLastError());
         for (int v = 0; v < nv; v++) {
             if (vcnt[v] == 0) continue;
-            const bool vr = std::get<0>(S->vkeys[v]) == YTRC_VOLUME_RENDER;
+            const bool vr = has_tf(std::get<0>(S->vkeys[v]));
             const size_t lds = vr ? sizeof(float4) * S->tf_hi : 0;
             int nb = 0;
             HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, S->vfns[v], 256, lds));
@@ -1825,7 +1835,7 @@ int ytrc_server_submit(ytrc_server *S, const ytrc_request *r, int64_t *id_out)
     R.off = off;
     S->used += align_up(R.P.bytes, 256);
     int tf_base = 0;
-    if (r->kind == YTRC_VOLUME_RENDER) {
+    if (has_tf(r->kind)) {
         const int rc = srv_tf_acquire(S, R.P.tft, R.tf_entry);
         if (rc != YTRC_OK) {
             srv_release(S, R);

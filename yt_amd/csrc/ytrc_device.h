@@ -65,6 +65,8 @@ struct alignas(16) DImg {
     uint64_t *rec_hash;
     int64_t *rec_samples;
     const DTF *tf;
+    double ldir[3];         // light-source sampler: unit light direction
+    double lrgba[4];        //   and light colour
     const double *pix_pos;  // per-pixel lens: nv0*nv1*3 origins, else null
     const double *pix_dir;  // per-pixel lens: nv0*nv1*3 directions
     int32_t nv0, nv1;

@@ -581,6 +581,114 @@ struct IProjSampler {
     }
 };
 
+// LightSourceRenderSampler.sample (image_samplers.pyx:548-590) with
+// FIT_eval_transfer_with_light (field_interpolation_tables.pxd:115-143): per
+// sub-sample the tables' values, the gradient of field 0 normalised and
+// negated (eval_gradient, fixed_interpolator.cpp:116-148), dot = light . grad,
+// and rgba_c = (1 - ta) e_c (1 + dot l_c) + ta rgba_c; no mask test.  The
+// reference's one-sided and central differences of the trilinear
+// interpolant along an axis are its exact partial derivative along that axis
+// (the interpolant is linear in each coordinate), so the gradient is formed
+// analytically from the corner differences.  Where the field is flat to
+// within rounding, the reference normalises rounding noise into an
+// arbitrary direction; there the two light terms differ.
+template <int NFP, bool REC>
+struct LightSampler {
+    const float *data;
+    BrickReg b;
+    double pa[3];
+    double pb[3];
+    const TFCache *tf;
+    const float4 *lt;
+    double inv_ns;
+    int n_samples;
+    float ldir[3];
+    double lrgba[3];
+    double st[3];
+    Rec *rec;
+
+    __device__ __forceinline__ void operator()(double en, double ex, int c0, int c1, int c2)
+    {
+        if (REC) rec->visit(en, ex, c0, c1, c2);
+        const int d1 = b.d1, d2 = b.d2;
+        if (!((unsigned)c0 < (unsigned)b.d0 && (unsigned)c1 < (unsigned)d1 &&
+              (unsigned)c2 < (unsigned)d2))
+            return;
+        if (REC) rec->samples += n_samples;
+        const int s1i = d2 + 1;
+        const int s0i = (d1 + 1) * (d2 + 1);
+        const int voff = (int)__umul24(c0, s0i) + (int)__umul24(c1, s1i) + c2;
+        const double dt = (ex - en) * inv_ns;
+        const double tt = en + 0.5 * dt;
+        float dp[3], ds[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int ci = (i == 0) ? c0 : ((i == 1) ? c1 : c2);
+            dp[i] = (float)(__builtin_fma(tt, pa[i], pb[i]) - (double)ci);
+            ds[i] = (float)(pa[i] * dt);
+        }
+        float cv[8][NFP];
+        const float *p0 = data + (size_t)voff * NFP;
+        FieldLoad<NFP>::get(p0, cv[0]);
+        FieldLoad<NFP>::get(p0 + NFP, cv[1]);
+        FieldLoad<NFP>::get(p0 + (size_t)s1i * NFP, cv[2]);
+        FieldLoad<NFP>::get(p0 + (size_t)(s1i + 1) * NFP, cv[3]);
+        FieldLoad<NFP>::get(p0 + (size_t)s0i * NFP, cv[4]);
+        FieldLoad<NFP>::get(p0 + (size_t)(s0i + 1) * NFP, cv[5]);
+        FieldLoad<NFP>::get(p0 + (size_t)(s0i + s1i) * NFP, cv[6]);
+        FieldLoad<NFP>::get(p0 + (size_t)(s0i + s1i + 1) * NFP, cv[7]);
+        float k[NFP][4];
+#pragma unroll
+        for (int f = 0; f < NFP; f++) {
+            const float cf[8] = {cv[0][f], cv[1][f], cv[2][f], cv[3][f],
+                                 cv[4][f], cv[5][f], cv[6][f], cv[7][f]};
+            trilinear_cubic(cf, dp, ds, k[f]);
+        }
+        // Field 0 as A + Bx x + By y + Bz z + Cxy xy + Cxz xz + Cyz yz + D xyz.
+        const float Bz = cv[1][0] - cv[0][0];
+        const float By = cv[2][0] - cv[0][0];
+        const float Bx = cv[4][0] - cv[0][0];
+        const float Cyz = (cv[3][0] - cv[2][0]) - Bz;
+        const float Cxz = (cv[5][0] - cv[4][0]) - Bz;
+        const float Cxy = (cv[6][0] - cv[4][0]) - By;
+        const float D = ((cv[7][0] - cv[6][0]) - (cv[5][0] - cv[4][0])) - Cyz;
+        const TFCache &T = *tf;
+        for (int s = 0; s < n_samples; s++) {
+            const float sf = (float)s;
+            float dvs[NFP], e[4];
+#pragma unroll
+            for (int f = 0; f < NFP; f++)
+                dvs[f] = __builtin_fmaf(
+                    __builtin_fmaf(__builtin_fmaf(k[f][3], sf, k[f][2]), sf, k[f][1]), sf, k[f][0]);
+            const float x = __builtin_fmaf(sf, ds[0], dp[0]);
+            const float y = __builtin_fmaf(sf, ds[1], dp[1]);
+            const float z = __builtin_fmaf(sf, ds[2], dp[2]);
+            const float gx = __builtin_fmaf(D * y, z, __builtin_fmaf(Cxz, z, __builtin_fmaf(Cxy, y, Bx)));
+            const float gy = __builtin_fmaf(D * x, z, __builtin_fmaf(Cyz, z, __builtin_fmaf(Cxy, x, By)));
+            const float gz = __builtin_fmaf(D * x, y, __builtin_fmaf(Cyz, y, __builtin_fmaf(Cxz, x, Bz)));
+            const float n2 = __builtin_fmaf(gx, gx, __builtin_fmaf(gy, gy, gz * gz));
+            double dot = 0.0;
+            if (n2 != 0.0f) {
+                const double lg = (double)__builtin_fmaf(ldir[0], gx, __builtin_fmaf(ldir[1], gy, ldir[2] * gz));
+                dot = -lg / __builtin_sqrt((double)n2);
+            }
+            fit_eval_general<NFP>(T, lt, dvs, e);
+            if (T.grey) {
+                const double ta = y_fmax(1.0 - dt * ((double)e[0] + (double)e[1] + (double)e[2]), 0.0);
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    st[c] = (1.0 - ta) * (double)e[c] * (1.0 + dot * lrgba[c]) + ta * st[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const double ta = y_fmax(1.0 - dt * (double)e[c], 0.0);
+                    st[c] = (1.0 - ta) * (double)e[c] * (1.0 + dot * lrgba[c]) + ta * st[c];
+                }
+            }
+        }
+    }
+};
+
 // Casts the rays of one 8x8 tile of image `im` through its bricks.  `im`
 // and `tile` are wave-uniform.
 template <int KIND, int NFP, bool REC, int MODE>
@@ -627,7 +735,7 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
 
     TFCache tfc;
     int nb = 0, tform = 0;
-    if (KIND == 1) {
+    if (KIND == 1 || KIND == 3) {
         load_tf(*im.tf, tfc);
         nb = uni(im.tf->nbins[0]);
         tform = uni(im.tform);
@@ -708,6 +816,27 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
                     ps[f] = s.s[f];
                     pc[f] = s.c[f];
                 }
+            } else if constexpr (KIND == 3) {
+                LightSampler<NFP, REC> s;
+                s.data = im.pool + b.data_off;
+                s.b = brick_reg(b);
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    s.pa[i] = PP ? dir[i] * b.idds[i] : uni_d(dir[i] * b.idds[i]);
+                    s.pb[i] = (pos[i] - b.le[i]) * b.idds[i];
+                    s.ldir[i] = (float)im.ldir[i];
+                    s.lrgba[i] = im.lrgba[i];
+                }
+                s.tf = &tfc;
+                s.lt = lt;
+                s.inv_ns = im.inv_ns;
+                s.n_samples = im.n_samples;
+#pragma unroll
+                for (int c = 0; c < 3; c++) s.st[c] = st[c];
+                s.rec = &rec;
+                walk_loop(b, w, max_t, s);
+#pragma unroll
+                for (int c = 0; c < 3; c++) st[c] = s.st[c];
             } else if constexpr (KIND == 2) {
                 IProjSampler<NFP, REC> s;
                 s.data = im.pool + b.data_off;
@@ -812,7 +941,7 @@ __global__ __launch_bounds__(256) void k_render(const DImg *__restrict__ imgs, i
 {
     extern __shared__ float4 s_tf[];
     const DImg &im = imgs[idx];
-    if (KIND == 1) {
+    if (KIND == 1 || KIND == 3) {
         const DTF &T = *im.tf;
         for (int i = threadIdx.x; i < T.total_bins; i += blockDim.x) s_tf[i] = T.table[i];
         __syncthreads();
@@ -848,7 +977,7 @@ void k_persist(const DImg *__restrict__ imgs,
                                                  const float4 *__restrict__ tf_all, int tf_bins)
 {
     extern __shared__ float4 s_tf[];
-    if (KIND == 1) {
+    if (KIND == 1 || KIND == 3) {
         for (int i = threadIdx.x; i < tf_bins; i += blockDim.x) s_tf[i] = tf_all[i];
         __syncthreads();
     }

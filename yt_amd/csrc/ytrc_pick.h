@@ -11,13 +11,15 @@ typedef void (*pfn_t)(const DImg *, const int2 *, int, unsigned int *, const flo
 typedef void (*kfn_t)(const DImg *, int);
 
 // Each returns the variant for (nfp, rec, mode) within its unit's family,
-// or nullptr.  Families: projection and interpolated projection (modes 0,
-// 8); volume rendering modes
+// or nullptr.  Families: projection, interpolated projection and light-source
+// rendering (modes 0, 8); volume rendering modes
 // 0-1, 2-3, 4-7 (early ray termination), 8-11 (per-pixel lens).
 pfn_t pick_persist_proj(int nfp, bool rec, int mode);
 kfn_t pick_kernel_proj(int nfp, bool rec, int mode);
 pfn_t pick_persist_iproj(int nfp, bool rec, int mode);
 kfn_t pick_kernel_iproj(int nfp, bool rec, int mode);
+pfn_t pick_persist_light(int nfp, bool rec, int mode);
+kfn_t pick_kernel_light(int nfp, bool rec, int mode);
 pfn_t pick_persist_vr_a(int nfp, bool rec, int mode);
 kfn_t pick_kernel_vr_a(int nfp, bool rec, int mode);
 pfn_t pick_persist_vr_b(int nfp, bool rec, int mode);

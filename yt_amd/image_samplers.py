@@ -96,6 +96,9 @@ class ImageSampler:
     def _tf(self):
         return None
 
+    def _light(self):
+        return {}
+
     def flush(self, record=False):
         """Casts every recorded brick on the GPU, in call order."""
         if not self._pending:
@@ -115,7 +118,8 @@ class ImageSampler:
             spec = _lib.RequestSpec(
                 self.kind, self._params(), ds_id, np.arange(len(pgs), dtype=np.int32),
                 n_samples=getattr(self, "n_samples", 10), tf=self._tf(), zbuffer=self.azbuffer,
-                image_in=self._image if np.any(self._image) else None, record=record)
+                image_in=self._image if np.any(self._image) else None, record=record,
+                **self._light())
             batch = _lib.Batch(ctx, [spec])
             try:
                 batch.run()
@@ -166,3 +170,21 @@ class VolumeRenderSampler(ImageSampler):
 
     def _tf(self):
         return self._tfc
+
+
+class LightSourceRenderSampler(VolumeRenderSampler):
+    """image_samplers.pyx:496-600: the volume render's tables with a
+    directional light; no mask test."""
+
+    kind = _lib.LIGHT_SOURCE
+
+    def __init__(self, vp_pos, vp_dir, center, bounds, image, x_vec, y_vec, width, volume_method,
+                 tf_obj, n_samples=10, light_dir=(1.0, 1.0, 1.0), light_rgba=(1.0, 1.0, 1.0, 1.0),
+                 **kwargs):
+        super().__init__(vp_pos, vp_dir, center, bounds, image, x_vec, y_vec, width,
+                         volume_method, tf_obj, n_samples, **kwargs)
+        self.light_dir = np.asarray(light_dir, dtype="float64")
+        self.light_rgba = np.asarray(light_rgba, dtype="float64")
+
+    def _light(self):
+        return {"light_dir": self.light_dir, "light_rgba": self.light_rgba}

@@ -115,6 +115,18 @@ class ResidentVolume:
                                 self.vr_order(params["viewpoint"]), n_samples=n_samples,
                                 record=record, zbuffer=zbuffer)
 
+    def light_source_request(self, params, tf, light_dir=(1.0, 1.0, 1.0),
+                             light_rgba=(1.0, 1.0, 1.0, 1.0), n_samples=10, record=False,
+                             zbuffer=None):
+        """LightSourceRenderSampler over the kd bricks (old_camera.py:688)."""
+        if self.vr_ds_id is None:
+            raise ValueError("volume was not uploaded with vertex-centred bricks")
+        tfc = tf if isinstance(tf, _lib.TF) else _lib.TF.from_tf_obj(tf)
+        return _lib.RequestSpec(_lib.LIGHT_SOURCE, params, self.vr_ds_id,
+                                self.vr_order(params["viewpoint"]), n_samples=n_samples, tf=tfc,
+                                record=record, zbuffer=zbuffer, light_dir=light_dir,
+                                light_rgba=light_rgba)
+
     def free(self):
         for i in (self.proj_ds_id, self.vr_ds_id):
             if i is not None:
