@@ -130,3 +130,29 @@ def test_drop_in_sampler_perspective():
     oimg = oracle.cast(0, p, oracle_proj_bricks(ds, ["density"]))
     assert out is image
     assert rel_l2(out[..., 0], oimg[..., 0]) <= TOL
+
+
+@pytest.mark.parametrize("lens", ["fisheye", "spherical", "stereo-spherical", "stereo-perspective"])
+def test_other_lenses_vr(ctx, lens):
+    """Cameras inside the AMR volume (rays start inside bricks; the fisheye
+    has zero-length directions outside its disc) and the stereo
+    perspective: volume renders and their visit sequences equal the oracle's."""
+    ds = small_amr(root_n=32, levels=2, seed=131)
+    tf = vr_tf_for(ds, nbins=64)
+    vol = render.ResidentVolume(ctx, ds, vr_field="density")
+    try:
+        inside = lens != "stereo-perspective"
+        pos = (0.47, 0.52, 0.44) if inside else (1.7, 1.1, 1.3)
+        res = (32, 32) if lens == "fisheye" else (40, 24)
+        p = render.lens_params(lens, pos, (0.2, 0.7, 0.3) if inside else (0.5, 0.5, 0.5), 0.9, res)
+        spec = vol.vr_request(p, tf, record=True)
+        (img, rec), = _run(ctx, [spec], _lib.BATCH_PERSISTENT)
+        oimg, ov, oh = oracle.cast(1, p, oracle_vr_bricks(vol), order=spec.order, tf=tf_dict(tf),
+                                   record=True)
+        _same_visits(rec, ov, oh, f"{lens} VR")
+        for c in range(3):
+            e = rel_l2(img[..., c], oimg[..., c])
+            print(f"{lens} VR ch{c} rel-L2={e:.3e}", flush=True)
+            assert e <= TOL
+    finally:
+        vol.free()

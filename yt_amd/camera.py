@@ -7,6 +7,7 @@ Restates, in numpy and without units (all lengths in code units):
   PlaneParallelLens._get_sampler_params   volume_rendering/lens.py:83-112
   PlaneParallelLens.set_viewpoint         volume_rendering/lens.py:114-121
   PerspectiveLens                         volume_rendering/lens.py:145-294
+  StereoPerspective/Fisheye/Spherical/StereoSpherical lenses  lens.py:295-836
 The sampler parameters (vp_pos, vp_dir, center, bounds, x_vec, y_vec, width)
 are exactly what ImageSampler's constructor receives (utils.py:115-139).
 """
@@ -180,7 +181,242 @@ class PerspectiveLens(PlaneParallelLens):
         }
 
 
-LENSES = {"plane-parallel": PlaneParallelLens, "perspective": PerspectiveLens}
+def get_rotation_matrix(theta, rot_vector):
+    """math_utils.py:1140-1206: rotation by theta radians about rot_vector
+    (Rodrigues' formula, in the reference's term order)."""
+    ux, uy, uz = rot_vector[0], rot_vector[1], rot_vector[2]
+    cost = np.cos(theta)
+    sint = np.sin(theta)
+    return np.array([
+        [cost + ux**2 * (1 - cost), ux * uy * (1 - cost) - uz * sint, ux * uz * (1 - cost) + uy * sint],
+        [uy * ux * (1 - cost) + uz * sint, cost + uy**2 * (1 - cost), uy * uz * (1 - cost) - ux * sint],
+        [uz * ux * (1 - cost) - uy * sint, uz * uy * (1 - cost) + ux * sint, cost + uz**2 * (1 - cost)],
+    ])
+
+
+def arr_fisheye_vectors(resolution, fov, nimx=1, nimy=1, nimi=0, nimj=0, off_theta=0.0,
+                        off_phi=0.0):
+    """grid_traversal.pyx:322-350 (Bourke's dome fisheye): a unit vector per
+    pixel at angle r * fov / 2 from +z, zero outside radius 1.01."""
+    fov_rad = fov * np.pi / 180.0
+    nx = resolution // nimx
+    ny = resolution // nimy
+    px = (2.0 * (nimi * nx + np.arange(nx))) / resolution - 1.0
+    py = (2.0 * (nimj * ny + np.arange(ny))) / resolution - 1.0
+    PX, PY = np.meshgrid(px, py, indexing="ij")
+    r = np.sqrt(PX * PX + PY * PY)
+    phi = np.arctan2(PY, PX) + off_phi
+    theta = r * fov_rad / 2.0 + off_theta
+    vp = np.stack([np.sin(theta) * np.cos(phi), np.sin(theta) * np.sin(phi), np.cos(theta)], axis=-1)
+    vp[r > 1.01] = 0.0
+    return vp
+
+
+def _max_length(camera):
+    return np.linalg.norm(camera.position - camera._domain_center) + 0.5 * np.linalg.norm(
+        camera._domain_width)
+
+
+class StereoPerspectiveLens(PerspectiveLens):
+    """lens.py:295-426: left and right perspective images side by side
+    along the first image axis; disparity 3 pixels by default."""
+
+    lens_type = "stereo-perspective"
+
+    def __init__(self):
+        self.disparity = None
+
+    def _get_sampler_params(self, camera, render_source=None):
+        camera.width[1] = camera.width[0] * (2.0 * camera.resolution[1] / camera.resolution[0])
+        if self.disparity is None:
+            self.disparity = 3.0 * camera.width[0] / camera.resolution[0]
+        vl, pl = self._get_positions_vectors(camera, -self.disparity)
+        vr, pr = self._get_positions_vectors(camera, self.disparity)
+        uv = np.ones(3, dtype="float64")
+        return {
+            "vp_pos": np.vstack([pl, pr]).reshape(*camera.resolution, 3),
+            "vp_dir": np.vstack([vl, vr]).reshape(*camera.resolution, 3),
+            "center": self.back_center,
+            "bounds": (0.0, 1.0, 0.0, 1.0),
+            "x_vec": uv,
+            "y_vec": uv,
+            "width": np.zeros(3, dtype="float64"),
+            "image": self.new_image(camera),
+            "lens_type": "stereo-perspective",
+        }
+
+    def _get_positions_vectors(self, camera, disparity):
+        """lens.py:368-426."""
+        nx = int(np.floor(camera.resolution[0]) / 2)
+        ny = camera.resolution[1]
+        east_vec, north_vec, normal_vec = camera.unit_vectors
+        angle_disparity = -np.arctan2(disparity, camera.width[2])
+        R = get_rotation_matrix(angle_disparity, north_vec)
+        east_vec_rot = np.dot(R, east_vec)
+        normal_vec_rot = np.dot(R, normal_vec)
+        px = np.linspace(-0.5, 0.5, nx)[np.newaxis, :]
+        py = np.linspace(-0.5, 0.5, ny)[np.newaxis, :]
+        sample_x = (camera.width[0] * np.array(east_vec_rot.reshape(3, 1) * px)).transpose()
+        sample_y = (camera.width[1] * np.array(north_vec.reshape(3, 1) * py)).transpose()
+        sample_x = np.repeat(sample_x.reshape(nx, 1, 3), ny, axis=1)
+        sample_y = np.repeat(sample_y.reshape(1, ny, 3), nx, axis=0)
+        normal_vecs = np.tile(normal_vec_rot, nx * ny).reshape(nx, ny, 3)
+        east_vecs = np.tile(east_vec_rot, nx * ny).reshape(nx, ny, 3)
+        max_length = _max_length(camera) + np.abs(self.disparity)
+        vectors = (sample_x + sample_y + normal_vecs * camera.width[2]) * (
+            max_length / camera.width[2])
+        positions = np.tile(camera.position, nx * ny).reshape(nx, ny, 3)
+        positions = positions + east_vecs * disparity
+        return vectors, positions
+
+
+class FisheyeLens(PlaneParallelLens):
+    """lens.py:515-574: dome fisheye, field of view `fov` degrees, square
+    image of resolution[0] pixels; rays from the camera position."""
+
+    lens_type = "fisheye"
+
+    def __init__(self):
+        self.fov = 180.0
+        self.radius = 1.0
+        self.center = None
+        self.rotation_matrix = np.eye(3)
+
+    def setup_box_properties(self, camera):
+        self.radius = camera.width.max()
+        super().setup_box_properties(camera)
+        self.set_viewpoint(camera)
+
+    def set_viewpoint(self, camera):
+        self.viewpoint = camera.position
+
+    def new_image(self, camera):
+        return np.zeros((camera.resolution[0], camera.resolution[0], 4), dtype="float64")
+
+    def _get_sampler_params(self, camera, render_source=None):
+        shape = (camera.resolution[0], camera.resolution[0], 3)
+        vp = -arr_fisheye_vectors(camera.resolution[0], self.fov).reshape(shape)
+        vp = vp.dot(np.linalg.inv(self.rotation_matrix))
+        vp *= self.radius
+        uv = np.ones(3, dtype="float64")
+        return {
+            "vp_pos": np.full(shape, camera.position, dtype="float64"),
+            "vp_dir": vp,
+            "center": self.center if self.center is not None else np.zeros(3),
+            "bounds": (0.0, 1.0, 0.0, 1.0),
+            "x_vec": uv,
+            "y_vec": uv,
+            "width": np.zeros(3, dtype="float64"),
+            "image": self.new_image(camera),
+            "lens_type": "fisheye",
+        }
+
+
+class SphericalLens(PlaneParallelLens):
+    """lens.py:626-688: equirectangular (longitude along image axis 0,
+    latitude along axis 1) rays from the camera position."""
+
+    lens_type = "spherical"
+
+    def __init__(self):
+        self.radius = 1.0
+        self.center = None
+        self.rotation_matrix = np.eye(3)
+
+    def setup_box_properties(self, camera):
+        self.radius = camera.width.max()
+        super().setup_box_properties(camera)
+        self.set_viewpoint(camera)
+
+    def set_viewpoint(self, camera):
+        self.viewpoint = camera.position
+
+    def _uv(self, camera):
+        R1 = get_rotation_matrix(0.5 * np.pi, [1, 0, 0])
+        R2 = get_rotation_matrix(0.5 * np.pi, [0, 0, 1])
+        return np.dot(R2, np.dot(R1, camera.unit_vectors))
+
+    def _get_sampler_params(self, camera, render_source=None):
+        px = np.linspace(-np.pi, np.pi, camera.resolution[0], endpoint=True)[:, None]
+        py = np.linspace(-np.pi / 2.0, np.pi / 2.0, camera.resolution[1], endpoint=True)[None, :]
+        positions = np.tile(camera.position, np.prod(camera.resolution)).reshape(
+            *camera.resolution, 3)
+        uv = self._uv(camera)
+        vectors = np.empty((*camera.resolution, 3), dtype="float64", order="C")
+        vectors[:, :, 0] = np.cos(px) * np.cos(py)
+        vectors[:, :, 1] = np.sin(px) * np.cos(py)
+        vectors[:, :, 2] = np.sin(py)
+        vectors = np.dot((vectors * _max_length(camera)), uv).reshape(*camera.resolution, 3)
+        dummy = np.ones(3, dtype="float64")
+        return {
+            "vp_pos": positions,
+            "vp_dir": vectors,
+            "center": self.back_center,
+            "bounds": (0.0, 1.0, 0.0, 1.0),
+            "x_vec": dummy,
+            "y_vec": dummy,
+            "width": np.zeros(3, dtype="float64"),
+            "image": self.new_image(camera).reshape(*camera.resolution, 4),
+            "lens_type": "spherical",
+        }
+
+
+class StereoSphericalLens(SphericalLens):
+    """lens.py:731-836: two spherical images (left, right eye) stacked along
+    image axis 1, eye offset `disparity` along the local east direction."""
+
+    lens_type = "stereo-spherical"
+
+    def __init__(self):
+        super().__init__()
+        self.disparity = None
+
+    def _get_sampler_params(self, camera, render_source=None):
+        if self.disparity is None:
+            self.disparity = camera.width[0] / 1000.0
+        ny = int(np.floor(camera.resolution[1]) / 2)
+        nx = camera.resolution[0]
+        px = np.linspace(-np.pi, np.pi, nx, endpoint=True)[:, None]
+        py = np.linspace(-np.pi / 2.0, np.pi / 2.0, ny, endpoint=True)[None, :]
+        vectors = np.zeros((nx, ny, 3), dtype="float64", order="C")
+        vectors[:, :, 0] = np.cos(px) * np.cos(py)
+        vectors[:, :, 1] = np.sin(px) * np.cos(py)
+        vectors[:, :, 2] = np.sin(py)
+        max_length = _max_length(camera) + np.abs(self.disparity)
+        vectors = vectors * max_length
+        uv = self._uv(camera)
+        vectors = np.dot(vectors, uv)
+        vectors2 = np.zeros((nx, ny, 3), dtype="float64", order="C")
+        vectors2[:, :, 0] = -np.sin(px) * np.ones((1, ny))
+        vectors2[:, :, 1] = np.cos(px) * np.ones((1, ny))
+        vectors2[:, :, 2] = 0
+        vectors2 = np.dot(vectors2, uv)
+        positions = np.tile(camera.position, nx * ny).reshape(nx, ny, 3)
+        # The left and right are switched here since VR is in LHS (lens.py:808).
+        positions_left = positions + vectors2 * self.disparity
+        positions_right = positions + vectors2 * (-self.disparity)
+        dummy = np.ones(3, dtype="float64")
+        return {
+            "vp_pos": np.hstack([positions_left, positions_right]).reshape(*camera.resolution, 3),
+            "vp_dir": np.hstack([vectors, vectors]).reshape(*camera.resolution, 3),
+            "center": self.back_center,
+            "bounds": (0.0, 1.0, 0.0, 1.0),
+            "x_vec": dummy,
+            "y_vec": dummy,
+            "width": np.zeros(3, dtype="float64"),
+            "image": self.new_image(camera).reshape(*camera.resolution, 4),
+            "lens_type": "stereo-spherical",
+        }
+
+
+LENSES = {
+    "plane-parallel": PlaneParallelLens,
+    "perspective": PerspectiveLens,
+    "stereo-perspective": StereoPerspectiveLens,
+    "fisheye": FisheyeLens,
+    "spherical": SphericalLens,
+    "stereo-spherical": StereoSphericalLens,
+}
 
 
 class Camera(Orientation):

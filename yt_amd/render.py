@@ -142,25 +142,38 @@ def camera_params(center, normal, width, resolution, north=None):
     return p
 
 
-def perspective_params(position, focus, width, resolution, north=None,
-                       domain_left_edge=(0.0, 0.0, 0.0), domain_right_edge=(1.0, 1.0, 1.0)):
-    """Sampler parameters (+ kd viewpoint, the front centre) of a perspective
-    camera at `position` looking at `focus` (lens.py:145-240, camera.py:275-281)."""
+def lens_params(lens_type, position, focus, width, resolution, north=None,
+                domain_left_edge=(0.0, 0.0, 0.0), domain_right_edge=(1.0, 1.0, 1.0)):
+    """Sampler parameters (+ the lens's kd viewpoint) of a camera at
+    `position` looking at `focus` through any of yt's lenses (lens.py,
+    camera.py:275-281)."""
     from .camera import Camera
 
-    cam = Camera(lens_type="perspective", domain_left_edge=domain_left_edge,
+    cam = Camera(lens_type=lens_type, domain_left_edge=domain_left_edge,
                  domain_right_edge=domain_right_edge)
     if np.ndim(resolution) == 0:
         resolution = (resolution, resolution)
     cam.resolution = resolution
-    cam.position = position
-    cam.focus = focus
     cam.width = width
+    if np.array_equal(np.asarray(position, dtype="float64"), cam.focus):
+        cam.focus = focus
+        cam.position = position
+    else:
+        cam.position = position
+        cam.focus = focus
     if north is not None:
         cam.switch_orientation(north_vector=north)
     p = cam._get_sampler_params()
-    p["viewpoint"] = cam.lens.viewpoint.copy()
+    p["viewpoint"] = np.array(cam.lens.viewpoint, dtype="float64").copy()
     return p
+
+
+def perspective_params(position, focus, width, resolution, north=None,
+                       domain_left_edge=(0.0, 0.0, 0.0), domain_right_edge=(1.0, 1.0, 1.0)):
+    """Sampler parameters (+ kd viewpoint, the front centre) of a perspective
+    camera at `position` looking at `focus` (lens.py:145-240, camera.py:275-281)."""
+    return lens_params("perspective", position, focus, width, resolution, north,
+                       domain_left_edge, domain_right_edge)
 
 
 def finalize_projection(image, width, weighted, method="integrate"):
