@@ -122,9 +122,27 @@ def cpu_baseline(ds, vol, tf, cams, samples, budget_s):
         tot_s += dt
         tot_samp += samples[i]
         done.append(f"{'proj' if kind == 0 else 'vr'}{p['nv'][0]}")
-        log(f"cpu oracle {done[-1]}: {dt:.2f}s")
         if tot_s >= budget_s:
             break
+    log(f"cpu oracle: {len(done)} requests, {tot_samp} samples in {tot_s:.1f}s on {threads} threads")
+    # One thread, as off_axis_projection runs by default
+    # (off_axis_projection.py:34, num_threads=1): the smallest requests for
+    # about a fifth of the budget.
+    one_s, one_samp, one_n = 0.0, 0, 0
+    for i in order:
+        kind, p = cams[i]
+        t = time.perf_counter()
+        if kind == 0:
+            oracle.cast(0, p, pbricks, num_threads=1)
+        else:
+            oracle.cast(1, p, vbricks, order=vol.vr_order(p["viewpoint"]), tf=tfd, n_samples=10,
+                        num_threads=1)
+        one_s += time.perf_counter() - t
+        one_samp += samples[i]
+        one_n += 1
+        if one_s >= 0.2 * budget_s:
+            break
+    log(f"cpu oracle, 1 thread: {one_n} requests, {one_samp / one_s / 1e9:.4f} Gsamples/s")
     return {
         "value": tot_samp / tot_s / 1e9,
         "unit": "Gsamples/s",
@@ -132,6 +150,8 @@ def cpu_baseline(ds, vol, tf, cams, samples, budget_s):
         "kind": "port",
         "sample": f"{len(done)} of the 64 requests, smallest first ({', '.join(done)}), "
                   f"FP64 C oracle with OpenMP static/100, {tot_samp} samples in {tot_s:.1f}s",
+        "single_thread": {"value": one_samp / one_s / 1e9, "requests": one_n,
+                          "seconds": round(one_s, 2)},
     }
 
 
@@ -192,6 +212,8 @@ def main():
     ap.add_argument("--serve-rates", default="10,50,200",
                     help="Poisson arrival rates (req/s) of the serving leg")
     ap.add_argument("--serve-n", type=int, default=48, help="requests per rate")
+    ap.add_argument("--no-alt", action="store_true",
+                    help="skip the 64-brick (128^3 bricks) variant of the batched step")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the launch-shape autotuner (8x8 tiles, 4 waves/group, occupancy grid)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_k_persist.json"),
@@ -219,7 +241,6 @@ def main():
     log(f"HBM copy probe: {copy_gbps:.0f} GB/s")
     t = time.time()
     vol = render.ResidentVolume(ctx, ds, proj_fields="density", vr_field="density")
-    del rho
     log(f"resident: {vol.n_proj_bricks} projection blocks, {vol.n_vr_bricks} VR bricks, "
         f"{vol.device_bytes() / 2**30:.2f} GiB in {time.time() - t:.1f}s")
 
@@ -315,6 +336,33 @@ def main():
         lat[name] = dict(p50_ms=float(np.percentile(xs, 50)), p99_ms=float(np.percentile(xs, 99)))
     log(f"latency {lat}")
 
+    # The same field in 4x4x4 bricks of 128^3 (64 projection blocks and kd
+    # bricks): the batched step again, with its own sample count.
+    alt = None
+    if not args.no_alt and args.psize != 4:
+        from yt_amd import data as ydata
+
+        ds4 = ydata.load_uniform_grid({"density": rho}, psize=(4, 4, 4))
+        vol4 = render.ResidentVolume(ctx, ds4, proj_fields="density", vr_field="density")
+        rec4 = _lib.Batch(ctx, make_specs(vol4, tf, cams, record=True), _lib.BATCH_PERSISTENT)
+        rec4.run()
+        s4 = sum(int(rec4.read_record(i)[2].sum()) for i in range(len(cams)))
+        rec4.close()
+        b4 = autotune.make_batch(ctx, make_specs(vol4, tf, cams), cfg, flags)
+        for _ in range(max(1, args.warmup)):
+            b4.run()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b4.launch()
+        b4.sync()
+        w4 = (time.perf_counter() - t0) / args.steps
+        b4.close()
+        vol4.free()
+        alt = dict(bricks="4x4x4 of 128^3", samples_per_step=s4, ms_per_step=round(w4 * 1e3, 3),
+                   gsamples_per_s=round(s4 / w4 / 1e9, 3))
+        log(f"64 bricks of 128^3: {w4 * 1e3:.2f} ms/step, {s4 / w4 / 1e9:.1f} Gsamples/s")
+    del rho
+
     serving = None
     if not args.no_serve:
         serving = serve_leg(ctx, vol, tf, [float(x) for x in args.serve_rates.split(",")],
@@ -354,6 +402,7 @@ def main():
             "parallelism": "1 GPU",
         },
         "images_per_s": round(imgs_s, 2),
+        "bricks_64": alt,
         "plain_images_per_s": None if plain is None else round(plain["images_per_s"], 2),
         "latency_ms": lat,
         "serving": serving,

@@ -117,3 +117,32 @@ def test_server_cancel(ctx, volumes):
         assert srv.stats()["used"] == 0
     finally:
         srv.close()
+
+
+def test_server_all_sampler_kinds_and_lenses(ctx, volumes):
+    """Requests of every sampler kind and lens family, mixed in the server's
+    chunks, equal each request run alone."""
+    vols, tfs = volumes
+    va = vols[0]
+    tf = tfs[id(va)]
+    pp = c0_params(48, normal=(0.3, -0.4, 0.87))
+    pv = render.perspective_params((1.6, 0.4, 1.2), (0.5, 0.5, 0.5), 0.8, (40, 36))
+    pf = render.lens_params("fisheye", (0.45, 0.5, 0.55), (0.1, 0.7, 0.3), 0.9, (24, 24))
+    specs = [va.projection_request(pv), va.vr_request(pf, tf), va.interpolated_projection_request(pp),
+             va.light_source_request(pv, tf, light_dir=(0.3, 1.0, -0.2)),
+             va.interpolated_projection_request(pf), va.vr_request(pp, tf)]
+    alone = [_alone(ctx, s) for s in specs]
+    srv = _lib.Server(ctx, max_requests=8, budget_bytes=128 << 20, chunk_tiles=64)
+    try:
+        ids = {srv.submit(s): i for i, s in enumerate(specs)}
+        assert None not in ids
+        got = {}
+        while len(got) < len(specs):
+            for rid in srv.wait():
+                got[ids[rid]] = srv.read(rid)
+        for i in range(len(specs)):
+            a, b = got[i][..., :3], alone[i][..., :3]
+            assert np.array_equal(np.isnan(a), np.isnan(b)) and np.array_equal(a[~np.isnan(a)],
+                                                                               b[~np.isnan(b)]), i
+    finally:
+        srv.close()
