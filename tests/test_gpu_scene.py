@@ -1,0 +1,89 @@
+"""Scene / VolumeSource / create_scene on the GPU path against the oracle:
+the rendered image is the oracle's VolumeRenderSampler (or interpolated
+projection) image with alpha set to 1 (render_source.py:522-537) and
+rotated by 180 degrees (scene.py:625); a second transparent source starts
+from the first one's image (scene.py:612-619)."""
+
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from yt_amd import scene
+from yt_amd.transfer_functions import gaussian_ctf
+
+from .conftest import rel_l2
+from .helpers import oracle_vr_bricks, small_amr, tf_dict, vr_tf_for
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _setup(ds, sc):
+    cam = sc.camera
+    cam.resolution = (48, 40)
+    cam.width = 0.9
+    cam.position = (1.4, 1.1, 1.6)
+    cam.focus = (0.5, 0.5, 0.5)
+    p = cam._get_sampler_params(sc.sources["source_00"])
+    p["viewpoint"] = np.array(cam.lens.viewpoint)
+    return p
+
+
+@pytest.mark.parametrize("lens", ["plane-parallel", "perspective"])
+def test_create_scene_render(ctx, lens, tmp_path):
+    ds = small_amr(root_n=32, levels=2, seed=141)
+    tf = vr_tf_for(ds, nbins=128)
+    sc = scene.create_scene(ds, "density", lens_type=lens, transfer_function=tf, ctx=ctx)
+    src = sc.sources["source_00"]
+    try:
+        p = _setup(ds, sc)
+        im = sc.render()
+        order = src.volume.vr_order(p["viewpoint"])
+        oimg = oracle.cast(1, p, oracle_vr_bricks(src.volume), order=order, tf=tf_dict(tf))
+        oimg[:, :, 3] = 1.0
+        want = np.rot90(oimg, k=2)
+        assert im.shape == want.shape
+        for c in range(4):
+            assert rel_l2(im[..., c], want[..., c]) <= TOL, c
+        fn = sc.save(str(tmp_path / "render.png"), sigma_clip=4.0, render_scene=False)
+        with open(fn, "rb") as f:
+            assert f.read(8) == b"\x89PNG\r\n\x1a\n"
+        assert os.path.getsize(fn) > 100
+    finally:
+        src.free()
+
+
+def test_two_transparent_sources_and_projection_source(ctx):
+    ds = small_amr(root_n=32, levels=2, seed=143)
+    tf1 = vr_tf_for(ds, nbins=64)
+    rho = np.concatenate([np.ravel(g["density"]) for g in ds.grids])
+    tf2 = gaussian_ctf((float(np.log10(rho.min())), float(np.log10(rho.max()))), n_gauss=3, nbins=64)
+    sc = scene.Scene()
+    s1 = scene.VolumeSource(ds, "density", transfer_function=tf1, ctx=ctx)
+    s2 = scene.VolumeSource(ds, "density", transfer_function=tf2, ctx=ctx)
+    sc.add_source(s1).add_source(s2)
+    sc.add_camera(ds)
+    try:
+        p = _setup(ds, sc)
+        im = sc.render()
+        bricks = oracle_vr_bricks(s1.volume)
+        order = s1.volume.vr_order(p["viewpoint"])
+        first = oracle.cast(1, p, bricks, order=order, tf=tf_dict(tf1))
+        first[:, :, 3] = 1.0
+        second = oracle.cast(1, p, bricks, order=order, tf=tf_dict(tf2), image=first.copy())
+        second[:, :, 3] = 1.0
+        want = np.rot90(second, k=2)
+        for c in range(3):
+            assert rel_l2(im[..., c], want[..., c]) <= TOL, c
+        # a "projection" source: InterpolatedProjectionSampler
+        s3 = scene.VolumeSource(ds, "density", log_field=False, sampler_type="projection", ctx=ctx)
+        pim = s3.render(sc.camera)
+        oproj = oracle.cast(2, p, oracle_vr_bricks(s3.volume), order=s3.volume.vr_order(p["viewpoint"]))
+        assert rel_l2(pim[..., 0], oproj[..., 0]) <= TOL
+        s3.free()
+    finally:
+        s1.free()
+        s2.free()

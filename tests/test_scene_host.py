@@ -1,0 +1,70 @@
+"""Host-side composition and image output (no GPU): ZBuffer merging
+(zbuffer_array.py), ImageArray rescale / background / sigma clip
+(image_array.py) and the PNG writer (image_writer.py write_bitmap)."""
+
+import struct
+import zlib
+
+import numpy as np
+
+from yt_amd.scene import ZBuffer, add_background_color, clipping_value, rescale, write_bitmap
+
+
+def test_zbuffer_keeps_nearer():
+    """zbuffer_array.py's docstring example and a mixed-depth case."""
+    shape = (16, 16)
+    rng = np.random.default_rng(3)
+    b1 = ZBuffer(rng.random(shape), np.ones(shape))
+    b2 = ZBuffer(rng.random(shape), np.zeros(shape))
+    c = b1 + b2
+    assert np.all(c.rgba == b2.rgba) and np.all(c.z == b2.z) and c == b2
+    z1 = rng.random(shape)
+    z2 = rng.random(shape)
+    r1, r2 = rng.random(shape + (4,)), rng.random(shape + (4,))
+    d = ZBuffer(r1, z1) + ZBuffer(r2, z2)
+    near1 = z1 < z2
+    assert np.array_equal(d.rgba[near1], r1[near1]) and np.array_equal(d.rgba[~near1], r2[~near1])
+    assert np.array_equal(d.z, np.minimum(z1, z2))
+
+
+def test_rescale_background_clip():
+    im = np.zeros((8, 10, 4))
+    for k in range(4):
+        im[:, :, k] = np.linspace(0.0, 0.3 * (k + 1), 10)[None, :]
+    r = rescale(im)
+    assert np.isclose(r[:, :, :3].sum(axis=2).max(), 1.0 * r[:, :, :3].sum(axis=2).max())
+    assert r[:, :, 3].max() == 1.0 and r.min() >= 0.0
+    assert np.allclose(r[:, :, :3], np.clip(im[:, :, :3] / im[:, :, :3].sum(axis=2).max(), 0, 1))
+    b = add_background_color(r, "black")
+    assert np.allclose(b[:, :, 3], 1.0)
+    assert np.allclose(b[:, :, 0], r[:, :, 0] * r[:, :, 3])
+    nz = r[:, :, :3][r[:, :, :3].nonzero()]
+    assert np.isclose(clipping_value(r, 2.0), nz.mean() + 2.0 * nz.std())
+
+
+def _decode_png(data):
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, chunks = 8, {}
+    while pos < len(data):
+        (n,) = struct.unpack(">I", data[pos:pos + 4])
+        tag = data[pos + 4:pos + 8]
+        payload = data[pos + 8:pos + 8 + n]
+        (crc,) = struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])
+        assert crc == zlib.crc32(tag + payload) & 0xFFFFFFFF
+        chunks[tag] = chunks.get(tag, b"") + payload
+        pos += 12 + n
+    w, h, depth, ctype = struct.unpack(">IIBB", chunks[b"IHDR"][:10])
+    raw = zlib.decompress(chunks[b"IDAT"])
+    rows = [raw[y * (4 * w + 1) + 1:(y + 1) * (4 * w + 1)] for y in range(h)]
+    return np.frombuffer(b"".join(rows), np.uint8).reshape(h, w, 4), depth, ctype
+
+
+def test_png_round_trip():
+    rng = np.random.default_rng(5)
+    im = rng.random((12, 7, 4))
+    arr = write_bitmap(im, None)
+    dec, depth, ctype = _decode_png(arr)
+    assert depth == 8 and ctype == 6
+    want = np.concatenate([(np.clip(im[:, :, :3] / im[:, :, :3].max(), 0, 1) * 255).astype("uint8"),
+                           (255 * im[:, :, 3]).astype("uint8")[..., None]], axis=-1)
+    assert np.array_equal(dec, want)

@@ -16,6 +16,7 @@ from .image_samplers import (ImageSampler, InterpolatedProjectionSampler,
                              LightSourceRenderSampler, ProjectionSampler, VolumeRenderSampler)
 from .off_axis_projection import off_axis_projection
 from .partitioned_grid import PartitionedGrid
+from .scene import Scene, VolumeSource, ZBuffer, create_scene
 from .transfer_functions import (
     ColorTransferFunction,
     MultiVariateTransferFunction,
@@ -37,6 +38,10 @@ __all__ = [
     "PlaneParallelLens",
     "ProjectionSampler",
     "ProjectionTransferFunction",
+    "Scene",
+    "VolumeSource",
+    "ZBuffer",
+    "create_scene",
     "TransferFunction",
     "VolumeRenderSampler",
     "load_amr_grids",
