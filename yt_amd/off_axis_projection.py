@@ -17,7 +17,7 @@ from .render import finalize_projection
 
 def off_axis_projection(data_source, center, normal_vector, width, resolution, item, weight=None,
                         volume=None, no_ghost=False, interpolated=False, north_vector=None,
-                        num_threads=1, method="integrate"):
+                        depth=None, num_threads=1, method="integrate"):
     if method not in ("integrate", "sum"):
         raise NotImplementedError("Only 'integrate' or 'sum' methods are valid for off-axis-projections")
     if interpolated:
