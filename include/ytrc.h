@@ -59,6 +59,12 @@ extern "C" {
 /* Request flags. */
 #define YTRC_FLAG_RECORD 1u /* per-pixel visit count, visit hash and sample count */
 #define YTRC_FLAG_ERT 2u    /* early ray termination (VR only; changes op order)  */
+/* VolumeRenderSampler only, plane-parallel lens, no image_in: instead of an
+ * image, the request yields its per-pixel, per-channel affine map
+ * rgba -> C + T rgba over its bricks (read with ytrc_batch_read_partial).
+ * The maps of consecutive runs of a brick order compose to the map of the
+ * whole order, for every transfer function (grey or not). */
+#define YTRC_FLAG_PARTIAL 4u
 
 /* Batch flags. */
 #define YTRC_BATCH_PLAIN 0u      /* one kernel launch per request                    */
@@ -206,6 +212,14 @@ int ytrc_batch_sync(ytrc_batch *b, float *ms_out);
 /* Copies request i's image to image_out (float64, nv0*nv1*4, the reference's
  * aimage layout); channels the sampler does not write keep image_in. */
 int ytrc_batch_read_image(ytrc_batch *b, int32_t i, double *image_out);
+/* YTRC_FLAG_PARTIAL only: request i's map to map_out (float64,
+ * nv0*nv1*8: C[0..3] then T[0..3] per pixel).  Applied in call order, the
+ * maps of the parts of a split brick order give the image of the whole
+ * order.  Replaces the image exchange of yt's parallel volume render,
+ * receive_and_reduce (yt/utilities/amr_kdtree/amr_kdtools.py:6-39, driven by
+ * AMRKDTree.reduce_tree_images, amr_kdtree.py:297-323), which composites
+ * images with the front alpha (grey opacity) or adds them (otherwise). */
+int ytrc_batch_read_partial(ytrc_batch *b, int32_t i, double *map_out);
 /* Record mode only: per-pixel visit count, visit hash, sample count. */
 int ytrc_batch_read_record(ytrc_batch *b, int32_t i, int64_t *visits, uint64_t *hashes,
                            int64_t *samples);

@@ -191,3 +191,26 @@ def test_leaf_slice_matches_brick_data():
         assert np.array_equal(nle, pg.LeftEdge) and np.array_equal(nre, pg.RightEdge)
         assert np.array_equal(mask, pg.source_mask)
         node.data = None
+
+
+def test_compute_stddev_image():
+    """math_utils.py:1546-1583: sqrt(<v^2> - <v>^2); round-off negatives
+    (a constant field) become 0, significant ones raise."""
+    from yt_amd import compute_stddev_image
+
+    rng = np.random.default_rng(12)
+    # two values a, b with weights w, 1-w: variance w(1-w)(a-b)^2
+    a, b, w = rng.uniform(1, 3, (3, 5, 4)), rng.uniform(1, 3, (3, 5, 4)), rng.uniform(0, 1, (5, 4))
+    a, b = a[0], b[0]
+    mean = w * a + (1 - w) * b
+    mean2 = w * a * a + (1 - w) * b * b
+    sd = compute_stddev_image(mean2, mean)
+    assert np.allclose(sd, np.sqrt(w * (1 - w)) * np.abs(a - b), rtol=1e-12, atol=1e-12)
+    # constant field: <v^2> and <v>^2 differ by round-off only
+    c = np.full((6, 6), 0.1) * 3.0
+    c2 = np.full((6, 6), 0.3 * 0.3)
+    c2[0, 0] = np.nextafter(c[0, 0] * c[0, 0], 0.0)
+    out = compute_stddev_image(c2, c)
+    assert out[0, 0] == 0.0 and np.all(np.isfinite(out))
+    with pytest.raises(ValueError):
+        compute_stddev_image(np.full((2, 2), 1.0), np.full((2, 2), 2.0))

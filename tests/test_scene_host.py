@@ -68,3 +68,30 @@ def test_png_round_trip():
     want = np.concatenate([(np.clip(im[:, :, :3] / im[:, :, :3].max(), 0, 1) * 255).astype("uint8"),
                            (255 * im[:, :, 3]).astype("uint8")[..., None]], axis=-1)
     assert np.array_equal(dec, want)
+
+
+def test_volume_source_transfer_function_rules():
+    """render_source.py:104-134, 280-302: a ProjectionTransferFunction makes
+    the source a linear "projection" source; other types are refused; a
+    weight field is the second kd field with the same log flag."""
+    import pytest
+
+    from yt_amd.scene import VolumeSource
+    from yt_amd.transfer_functions import (
+        ColorTransferFunction,
+        MultiVariateTransferFunction,
+        ProjectionTransferFunction,
+    )
+
+    s = VolumeSource(None, "density", transfer_function=ColorTransferFunction((0.0, 1.0)))
+    assert s.sampler_type == "volume-render" and s.log_field is True
+    assert s._fields() == (["density"], [True])
+    s.set_weight_field("cell_mass")
+    assert s._fields() == (["density", "cell_mass"], [True, True])
+    s.transfer_function = ProjectionTransferFunction(n_fields=2)
+    assert s.sampler_type == "projection" and s.log_field is False
+    assert s._fields() == (["density", "cell_mass"], [False, False])
+    with pytest.raises(RuntimeError):
+        s.transfer_function = MultiVariateTransferFunction()
+    with pytest.raises(NotImplementedError):
+        ProjectionTransferFunction(n_fields=4)

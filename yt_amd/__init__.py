@@ -14,7 +14,11 @@ from .camera import Camera, Orientation, PerspectiveLens, PlaneParallelLens, off
 from .data import load_amr_grids, load_uniform_grid
 from .image_samplers import (ImageSampler, InterpolatedProjectionSampler,
                              LightSourceRenderSampler, ProjectionSampler, VolumeRenderSampler)
-from .off_axis_projection import off_axis_projection
+from .off_axis_projection import (
+    compute_stddev_image,
+    off_axis_projection,
+    off_axis_projection_buffer,
+)
 from .partitioned_grid import PartitionedGrid
 from .scene import Scene, VolumeSource, ZBuffer, create_scene
 from .transfer_functions import (
@@ -48,4 +52,6 @@ __all__ = [
     "load_uniform_grid",
     "off_axis_camera",
     "off_axis_projection",
+    "off_axis_projection_buffer",
+    "compute_stddev_image",
 ]

@@ -30,6 +30,7 @@ LENS_PER_PIXEL = 1
 
 FLAG_RECORD = 1
 FLAG_ERT = 2
+FLAG_PARTIAL = 4
 
 BATCH_PLAIN = 0
 BATCH_PERSISTENT = 1
@@ -50,6 +51,7 @@ EXPORTED = [
     "ytrc_batch_sync",
     "ytrc_batch_read_image",
     "ytrc_batch_read_record",
+    "ytrc_batch_read_partial",
     "ytrc_batch_set_launch",
     "ytrc_batch_bytes",
     "ytrc_batch_destroy",
@@ -182,6 +184,7 @@ def _declare(lib):
     lib.ytrc_batch_launch.argtypes = [P]
     lib.ytrc_batch_sync.argtypes = [P, C.POINTER(C.c_float)]
     lib.ytrc_batch_read_image.argtypes = [P, C.c_int32, C.POINTER(C.c_double)]
+    lib.ytrc_batch_read_partial.argtypes = [P, C.c_int32, C.POINTER(C.c_double)]
     lib.ytrc_batch_read_record.argtypes = [
         P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_uint64), C.POINTER(C.c_int64)]
     lib.ytrc_batch_set_launch.argtypes = [P, C.c_int32, C.c_int32]
@@ -439,7 +442,7 @@ class RequestSpec:
 
     def __init__(self, kind, params, ds_id, order, n_samples=10, tf=None, zbuffer=None,
                  image_in=None, record=False, ert=False, light_dir=(1.0, 1.0, 1.0),
-                 light_rgba=(1.0, 1.0, 1.0, 1.0)):
+                 light_rgba=(1.0, 1.0, 1.0, 1.0), partial=False):
         self.kind = kind
         # LightSourceRenderSampler normalises the light direction
         # (image_samplers.pyx:526-528).
@@ -456,6 +459,7 @@ class RequestSpec:
         self.image_in = None if image_in is None else np.ascontiguousarray(image_in, dtype=np.float64)
         self.record = record
         self.ert = ert
+        self.partial = partial
         # Per-pixel lenses (perspective, fisheye, ...): vp_pos / vp_dir are
         # (nv0, nv1, 3) arrays (image_samplers.pyx:89-96).
         self.lens_type = params.get("lens_type", "plane-parallel")
@@ -508,7 +512,8 @@ class RequestSpec:
             r.image_in = _dptr(self.image_in)
         if self.tf is not None:
             r.tf = C.pointer(self.tf.desc)
-        r.flags = (FLAG_RECORD if self.record else 0) | (FLAG_ERT if self.ert else 0)
+        r.flags = ((FLAG_RECORD if self.record else 0) | (FLAG_ERT if self.ert else 0) |
+                   (FLAG_PARTIAL if self.partial else 0))
         r.light_dir[:] = [float(x) for x in self.light_dir]
         r.light_rgba[:] = [float(x) for x in self.light_rgba]
         return r
@@ -565,6 +570,13 @@ class Batch:
         nv = self.specs[i].nv
         out = np.empty((nv[0], nv[1], 4), dtype=np.float64)
         _check(lib().ytrc_batch_read_image(self._h, int(i), _dptr(out)))
+        return out
+
+    def read_partial(self, i):
+        """(nv0, nv1, 8) float64: C[0..3], T[0..3] of a partial-map request."""
+        nv = self.specs[i].nv
+        out = np.empty((nv[0], nv[1], 8), dtype=np.float64)
+        _check(lib().ytrc_batch_read_partial(self._h, int(i), _dptr(out)))
         return out
 
     def read_record(self, i):

@@ -184,6 +184,45 @@ class AMRKDTree:
     def leaves(self, viewpoint=None):
         return list(self.trunk.kd_traverse(viewpoint=viewpoint))
 
+    def data_root(self):
+        """The smallest subtree holding every leaf that carries a grid: the
+        trunk spans all space, so its first splits, at the domain faces,
+        leave one side without grids."""
+        node = self.trunk
+        while not node.is_leaf():
+            has_l = next(node.left.kd_traverse(), None) is not None
+            has_r = next(node.right.kd_traverse(), None) is not None
+            if has_l and has_r:
+                break
+            node = node.left if has_l else node.right
+        return node
+
+    def subtrees(self, depth, viewpoint=None):
+        """The kd subtrees `depth` levels below data_root() (a branch that
+        ends higher up is a part of its own): the shares of 2^depth GPUs
+        when kd subtrees are split over GPUs, as yt's parallel volume
+        render gives one subtree to each rank (nodes nprocs .. 2 nprocs - 1,
+        amr_kdtree.py:283-323; yt builds that top of the tree by geometric
+        halving, amr_kdtools.pyx:460-500, this splits the serial tree).
+        With a viewpoint, in the order its traversal reaches them, so that
+        their leaves, each in viewpoint order, concatenate to the whole
+        tree's viewpoint order."""
+        parts = []
+
+        def rec(n, d):
+            if d == depth or n.is_leaf():
+                parts.append(n)
+                return
+            rec(n.left, d + 1)
+            rec(n.right, d + 1)
+
+        root = self.data_root()
+        rec(root, 0)
+        if viewpoint is not None:
+            rank = {id(n): i for i, n in enumerate(root.viewpoint_traverse(viewpoint))}
+            parts.sort(key=lambda n: rank[id(n)])
+        return parts
+
     def traverse(self, viewpoint=None):
         for node in self.trunk.kd_traverse(viewpoint=viewpoint):
             yield self.get_brick_data(node)

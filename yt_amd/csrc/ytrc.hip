@@ -772,6 +772,14 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
                                        "(image_samplers.pyx:89-96)");
     if (pp && (r.flags & YTRC_FLAG_ERT))
         return fail(YTRC_ENOTIMPL, tag + "early ray termination is implemented for the plane-parallel lens only");
+    const bool partial = (r.flags & YTRC_FLAG_PARTIAL) != 0;
+    if (partial && r.kind != YTRC_VOLUME_RENDER)
+        return fail(YTRC_ENOTIMPL, tag + "partial maps are implemented for VolumeRenderSampler "
+                                         "(projection partials are plain images, which add)");
+    if (partial && pp)
+        return fail(YTRC_ENOTIMPL, tag + "partial maps are implemented for the plane-parallel lens only");
+    if (partial && r.image_in != nullptr)
+        return fail(YTRC_EINVAL, tag + "a partial map starts from the identity: no image_in");
     if (r.nv[0] <= 0 || r.nv[1] <= 0) return fail(YTRC_EINVAL, tag + "image size must be > 0");
     auto dsit = ctx->datasets.find(r.ds_id);
     if (dsit == ctx->datasets.end()) return fail(YTRC_EINVAL, tag + "unknown dataset");
@@ -976,11 +984,13 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
             tot = (int)P.tft.size();
             T.mode = 1;
         }
-        q.mode = T.mode | (r.n_samples == 10 ? 2 : 0) | ((r.flags & YTRC_FLAG_ERT) ? 4 : 0) | (pp ? 8 : 0);
+        q.mode = T.mode | (r.n_samples == 10 ? 2 : 0) |
+                 ((r.flags & (YTRC_FLAG_ERT | YTRC_FLAG_PARTIAL)) ? 4 : 0) | (pp ? 8 : 0);
         T.total_bins = tot;
         q.lds = (size_t)tot * sizeof(float4);
         if (q.lds > 64 * 1024) return fail(YTRC_EINVAL, tag + "transfer-function tables exceed 64 KiB of LDS");
         q.nch_out = (T.grey && r.kind == YTRC_VOLUME_RENDER) ? 4 : 3;
+        if (partial) q.nch_out = 8;  // C[0..3], T[0..3]
     }
     if (r.kind != YTRC_VOLUME_RENDER) q.mode = pp ? 8 : 0;
     if (r.kind == YTRC_LIGHT_SOURCE) q.tform = 0;
@@ -1013,7 +1023,7 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
     P.o_init = off;
     if (r.image_in) off = align_up(off + sizeof(float) * 4 * q.npix, 256);
     P.o_out = off;
-    off = align_up(off + sizeof(float) * 4 * q.npix, 256);
+    off = align_up(off + sizeof(float) * std::max(4, q.nch_out) * q.npix, 256);
     P.o_rv = P.o_rh = P.o_rs = off;
     if (rec) {
         P.o_rv = off;
@@ -1109,6 +1119,9 @@ int stage_request(ytrc_ctx *ctx, PrepReq &P, char *A, const float4 *tf_table, in
 // not write) into the caller's float64 (nv0, nv1, 4) image.
 int read_request_image(const ReqHost &q, const DImg &d, double *image_out)
 {
+    if (q.r.flags & YTRC_FLAG_PARTIAL)
+        return fail(YTRC_EINVAL, "a YTRC_FLAG_PARTIAL request has a map, not an image: "
+                                 "read it with ytrc_batch_read_partial");
     std::vector<float> h((size_t)q.npix * q.nch_out);
     HIPCHK(hipMemcpy(h.data(), d.out, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
     for (int64_t p = 0; p < q.npix; p++) {
@@ -1390,6 +1403,22 @@ int ytrc_batch_read_image(ytrc_batch *b, int32_t i, double *image_out)
     HIPCHK(hipSetDevice(b->ctx->device));
     HIPCHK(hipStreamSynchronize(b->ctx->stream));
     return read_request_image(b->reqs[i], b->h_imgs[i], image_out);
+}
+
+int ytrc_batch_read_partial(ytrc_batch *b, int32_t i, double *map_out)
+{
+    if (b == nullptr || map_out == nullptr) return fail(YTRC_EINVAL, "ytrc_batch_read_partial: NULL");
+    if (i < 0 || i >= (int)b->reqs.size()) return fail(YTRC_EINVAL, "ytrc_batch_read_partial: bad index");
+    const ReqHost &q = b->reqs[i];
+    if (!(q.r.flags & YTRC_FLAG_PARTIAL))
+        return fail(YTRC_EINVAL, "ytrc_batch_read_partial: request was not built with YTRC_FLAG_PARTIAL");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    std::vector<float> h((size_t)q.npix * 8);
+    HIPCHK(hipMemcpy(h.data(), b->h_imgs[i].out, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
+    for (int64_t p = 0; p < q.npix; p++)
+        for (int c = 0; c < 8; c++) map_out[(size_t)p * 8 + c] = (double)h[(size_t)c * q.npix + p];
+    return YTRC_OK;
 }
 
 int ytrc_batch_read_record(ytrc_batch *b, int32_t i, int64_t *visits, uint64_t *hashes,
@@ -1816,8 +1845,8 @@ int ytrc_server_create(ytrc_ctx *ctx, int32_t max_requests, uint64_t budget_byte
 int ytrc_server_submit(ytrc_server *S, const ytrc_request *r, int64_t *id_out)
 {
     if (S == nullptr || r == nullptr || id_out == nullptr) return fail(YTRC_EINVAL, "ytrc_server_submit: NULL");
-    if (r->flags & YTRC_FLAG_RECORD)
-        return fail(YTRC_EINVAL, "ytrc_server_submit: record mode is for batches");
+    if (r->flags & (YTRC_FLAG_RECORD | YTRC_FLAG_PARTIAL))
+        return fail(YTRC_EINVAL, "ytrc_server_submit: record mode and partial maps are for batches");
     std::lock_guard<std::mutex> lk(S->mu);
     HIPCHK(hipSetDevice(S->ctx->device));
     SrvReq R;

@@ -9,6 +9,7 @@
 
 #include <cstdint>
 
+#include "ytrc.h"
 #include "ytrc_device.h"
 
 namespace {
@@ -363,6 +364,7 @@ struct VRSampler {
     const uint8_t *mask;
     BrickReg b;
     double pa[3];  // v_dir * idds (wave-uniform)
+    float paf[3];  // (float)pa
     double pb[3];  // (v_pos - left_edge) * idds
     const TFCache *tf;
     const float4 *lt;
@@ -482,12 +484,15 @@ struct VRSampler {
         const double tt = en + 0.5 * dt;
         // Cell-relative position (tt*v_dir + v_pos - (idx*dds + LE)) * idds,
         // as tt*pa + pb - idx with pa = v_dir*idds, pb = (v_pos - LE)*idds.
+        // The sub-sample step v_dir * idds * dt in FP32 (paf = (float)pa,
+        // per brick): it only spaces the sub-samples within the cell.
+        const float dtf = (float)dt;
         float dp[3], ds[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const int ci = (i == 0) ? c0 : ((i == 1) ? c1 : c2);
             dp[i] = (float)(__builtin_fma(tt, pa[i], pb[i]) - (double)ci);
-            ds[i] = (float)(pa[i] * dt);
+            ds[i] = paf[i] * dtf;
         }
         float cv[8][NFP];
         {
@@ -770,9 +775,12 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
     // so its eye is at t -> +inf.  Front to back is therefore the reverse brick
     // order; each brick's cells are still walked forward (the same DDA) to form
     // its per-channel affine map rgba -> C_b + T_b rgba, maps are composed as
-    // C += T C_b, T *= T_b, and a wave stops when every lane has T < ERT_EPS.
-    constexpr double ERT_EPS = 1e-4;
+    // C += T C_b, T *= T_b, and a wave stops when every lane has T < 1e-4.
+    // A partial-map request (YTRC_FLAG_PARTIAL) composes the same maps over
+    // all of its bricks, without termination unless YTRC_FLAG_ERT is also
+    // set, and stores (C, T) instead of C + T rgba_in.
     constexpr bool ert = (KIND == 1) && ((MODE & 4) != 0);
+    const double ert_eps = (im.flags & YTRC_FLAG_ERT) ? 1e-4 : -1.0;
     double rgba0[4], Cacc[4], Tacc[4];
 #pragma unroll
     for (int c = 0; c < 4; c++) {
@@ -868,6 +876,7 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
                 for (int i = 0; i < 3; i++) {
                     s.pa[i] = PP ? dir[i] * b.idds[i] : uni_d(dir[i] * b.idds[i]);
                     s.pb[i] = (pos[i] - b.le[i]) * b.idds[i];
+                    s.paf[i] = (float)s.pa[i];
                 }
                 s.tf = &tfc;
                 s.lt = lt;
@@ -907,7 +916,7 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
         }
         if constexpr (ert) {
             const double tm = y_fmax(y_fmax(Tacc[0], Tacc[1]), y_fmax(Tacc[2], tfc.grey ? Tacc[3] : 0.0));
-            if (__all(!active || tm < ERT_EPS)) break;
+            if (__all(!active || tm < ert_eps)) break;
         }
     }
     if (!active) return;
@@ -916,9 +925,17 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
         for (int f = 0; f < NFP; f++)
             if (f < im.nch_out) im.out[f * npix + pix] = ps[f] - pc[f];
     } else if constexpr (ert) {
+        if (im.flags & YTRC_FLAG_PARTIAL) {
 #pragma unroll
-        for (int c = 0; c < 4; c++)
-            if (c < im.nch_out) im.out[c * npix + pix] = (float)__builtin_fma(Tacc[c], rgba0[c], Cacc[c]);
+            for (int c = 0; c < 4; c++) {
+                im.out[c * npix + pix] = (float)Cacc[c];
+                im.out[(4 + c) * npix + pix] = (float)Tacc[c];
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                if (c < im.nch_out) im.out[c * npix + pix] = (float)__builtin_fma(Tacc[c], rgba0[c], Cacc[c]);
+        }
     } else {
         if (tform) {
 #pragma unroll

@@ -107,11 +107,24 @@ class ImageSampler:
         ctx = _lib.default_context()
         vc = self.kind != _lib.PROJECTION
         bricks = [(pg.LeftEdge, pg.RightEdge, pg.dims, vc) for pg in pgs]
-        fields = [pg.my_data[:pg.n_fields] for pg in pgs]
+        fields = []
         for pg in pgs:
-            if pg.vertex_centred != vc:
-                raise ValueError("ProjectionSampler needs cell-centred bricks; VolumeRenderSampler "
-                                 "and InterpolatedProjectionSampler vertex-centred ones")
+            if pg.vertex_centred == vc:
+                fields.append(pg.my_data[:pg.n_fields])
+            elif not vc:
+                # ProjectionSampler handed a vertex-centred kd brick (a
+                # "projection" VolumeSource after set_sampler(camera,
+                # interpolated=False), render_source.py:464-488): it indexes
+                # the (d+1)^3 array with the cell strides of d
+                # (image_samplers.pyx:341), reading its first d0*d1*d2 values
+                # in C order as the cell array.  Reproduced as is.
+                d = tuple(int(x) for x in pg.dims)
+                n = d[0] * d[1] * d[2]
+                fields.append([np.ascontiguousarray(a).ravel()[:n].reshape(d)
+                               for a in pg.my_data[:pg.n_fields]])
+            else:
+                raise ValueError("VolumeRenderSampler and InterpolatedProjectionSampler need "
+                                 "vertex-centred bricks")
         masks = [pg.source_mask for pg in pgs] if self.kind == _lib.VOLUME_RENDER else None
         ds_id = ctx.upload(bricks, fields, masks)
         try:

@@ -42,3 +42,49 @@ def off_axis_projection(data_source, center, normal_vector, width, resolution, i
                              grid.ActiveDimensions.astype("int64"))
         sampler(pg, num_threads=num_threads)
     return finalize_projection(sampler.aimage, cam.width, weight is not None, method)
+
+
+def compute_stddev_image(buff2, buff):
+    """math_utils.py:1546-1583: sigma = sqrt(<v^2> - <v>^2) from the
+    weighted projections of v^2 and v; negative variances are set to 0 when
+    every one of them is round-off (np.isclose of the two terms), otherwise
+    ValueError."""
+    buff1 = buff * buff
+    y = buff2 - buff1
+    neg = y < 0.0
+    if np.isclose(np.asarray(buff2), np.asarray(buff1))[neg].all():
+        y[neg] = 0.0
+    else:
+        raise ValueError("Something went wrong, there are significant negative "
+                         "variances in the standard deviation image!")
+    return np.sqrt(y)
+
+
+def off_axis_projection_buffer(data_source, center, normal_vector, width, buff_size, item,
+                               weight=None, volume=None, no_ghost=False, interpolated=False,
+                               north_vector=None, depth=None, method="integrate", moment=1):
+    """OffAxisProjectionFixedResolutionBuffer._generate_image_and_mask
+    (fixed_resolution.py:622-692): the off-axis projection at the buffer's
+    size; for moment 2 the weighted standard deviation, from a second
+    projection of item**2 with the same weight (compute_stddev_image).
+    Returned as the buffer stores it, transposed (swapaxes(0, 1)): shape
+    (buff_size[1], buff_size[0]), rows along the north vector."""
+    if moment not in (1, 2):
+        raise ValueError("moment must be 1 or 2")
+    kw = dict(weight=weight, volume=volume, no_ghost=no_ghost, interpolated=interpolated,
+              north_vector=north_vector, depth=depth, method=method)
+    buff = off_axis_projection(data_source, center, normal_vector, width, buff_size, item, **kw)
+    if moment == 2:
+        ds = getattr(data_source, "ds", data_source)
+        item_sq = f"tmp_{item}_squared"
+        grids = list(ds.grids)
+        for g in grids:
+            g.fields[item_sq] = np.asarray(g[item], dtype="float64") ** 2
+        try:
+            buff2 = off_axis_projection(data_source, center, normal_vector, width, buff_size,
+                                        item_sq, **kw)
+        finally:
+            for g in grids:
+                g.fields.pop(item_sq, None)
+        buff = compute_stddev_image(buff2, buff)
+    return buff.swapaxes(0, 1)

@@ -23,6 +23,11 @@ import numpy as np
 
 from . import _lib, render
 from .camera import Camera
+from .transfer_functions import (
+    ColorTransferFunction,
+    ProjectionTransferFunction,
+    TransferFunction,
+)
 
 
 class ZBuffer:
@@ -169,35 +174,124 @@ class VolumeSource:
     """KDTreeVolumeSource (render_source.py:563-616) on the GPU path.
 
     sampler_type "volume-render" casts with VolumeRenderSampler,
-    "projection" with InterpolatedProjectionSampler (the `interpolated=True`
-    default of set_sampler, render_source.py:464-488)."""
+    "projection" with InterpolatedProjectionSampler (render() calls
+    set_sampler with its `interpolated=True` default, render_source.py:
+    464-488, 581).  A ProjectionTransferFunction switches the source to
+    "projection" with linear fields (render_source.py:104-116, 295-302).  A
+    weight field becomes the kd bricks' second field, with the same log
+    flag (render_source.py:121-134)."""
 
     volume_method = "KDTree"
 
     def __init__(self, ds, field="density", log_field=True, transfer_function=None,
-                 sampler_type="volume-render", n_samples=10, ctx=None):
+                 sampler_type="volume-render", n_samples=10, ctx=None, weight_field=None):
         self.ds = ds
         self.field = field
         self.log_field = log_field
-        self.transfer_function = transfer_function
+        self.weight_field = weight_field
         self.sampler_type = sampler_type
+        self._transfer_function = None
+        self.transfer_function = transfer_function
         self.num_samples = n_samples
-        self.ctx = ctx or _lib.default_context()
+        self._ctx = ctx
         self.zbuffer = None
         self.current_image = None
+        self.sampler = None
         self._volume = None
+        self._volume_key = None
+
+    @property
+    def ctx(self):
+        if self._ctx is None:
+            self._ctx = _lib.default_context()
+        return self._ctx
+
+    @property
+    def transfer_function(self):
+        return self._transfer_function
+
+    @transfer_function.setter
+    def transfer_function(self, value):
+        """render_source.py:280-302."""
+        valid_types = (TransferFunction, ColorTransferFunction, ProjectionTransferFunction,
+                       type(None))
+        if not isinstance(value, valid_types):
+            raise RuntimeError("transfer_function not a valid type, "
+                               f"received object of type {type(value)}")
+        if isinstance(value, ProjectionTransferFunction):
+            self.sampler_type = "projection"
+            self.log_field = False
+        self._transfer_function = value
+
+    def set_transfer_function(self, transfer_function):
+        self.transfer_function = transfer_function
+        return self
+
+    def set_log(self, log_field):
+        self.log_field = log_field
+        return self
+
+    def set_weight_field(self, weight_field):
+        self.weight_field = weight_field
+        return self
+
+    def _fields(self):
+        """validate_volume (render_source.py:121-134): [field] or
+        [field, weight_field], one log flag for both."""
+        fields = [self.field]
+        if self.weight_field is not None:
+            fields.append(self.weight_field)
+        return fields, [bool(self.log_field)] * len(fields)
 
     @property
     def volume(self):
+        fields, logs = self._fields()
+        key = (tuple(fields), tuple(logs))
+        if self._volume is not None and self._volume_key != key:
+            self._volume.free()
+            self._volume = None
         if self._volume is None:
-            self._volume = render.ResidentVolume(self.ctx, self.ds, vr_field=self.field,
-                                                 log_field=self.log_field)
+            self._volume = render.ResidentVolume(self.ctx, self.ds, vr_field=fields,
+                                                 log_field=logs)
+            self._volume_key = key
         return self._volume
 
     def _params(self, camera):
         p = camera._get_sampler_params(self)
         p["viewpoint"] = np.array(camera.lens.viewpoint, dtype="float64")
         return p
+
+    def set_sampler(self, camera, interpolated=True):
+        """render_source.py:464-488 with utils.py:55-139: the drop-in sampler
+        for this source and camera, to be called with the bricks of
+        `self.volume.kd.traverse(camera.lens.viewpoint)`.  "projection" with
+        interpolated=False is ProjectionSampler on vertex-centred bricks,
+        which reads each brick's vertex array with cell strides (see
+        image_samplers.ImageSampler.flush)."""
+        from . import image_samplers as ims
+
+        p = camera._get_sampler_params(self)
+        image = p["image"]
+        args = (np.atleast_3d(p["vp_pos"]), np.atleast_3d(p["vp_dir"]), p["center"],
+                p["bounds"], image, p["x_vec"], p["y_vec"], p["width"], self.volume_method)
+        kw = {"lens_type": p["lens_type"]}
+        kw["zbuffer"] = (self.zbuffer.z if self.zbuffer is not None
+                         else np.ones(image.shape[:2], "float64"))
+        if self.sampler_type == "volume-render":
+            if "camera_data" in p:
+                kw["camera_data"] = p["camera_data"]
+            if self.zbuffer is not None:
+                image[:] = np.reshape(self.zbuffer.rgba, (*camera.resolution, 4))
+            sampler = ims.VolumeRenderSampler(*args, self.transfer_function, self.num_samples,
+                                              **kw)
+        elif self.sampler_type == "projection" and interpolated:
+            sampler = ims.InterpolatedProjectionSampler(*args, self.num_samples, **kw)
+        elif self.sampler_type == "projection":
+            sampler = ims.ProjectionSampler(*args, self.num_samples, **kw)
+        else:
+            raise NotImplementedError(f"{self.sampler_type} not implemented yet")
+        self.sampler = sampler
+        return sampler
 
     def render(self, camera, zbuffer=None):
         """render_source.py:563-603: cast every kd brick in viewpoint order,
