@@ -28,6 +28,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Gsamples/s off-axis proj+VR at 512^3 (1 GPU); p50/p99 render latency"
 HBM_PEAK_GBS = 8000.0
+# MI355X vector FP64 peak (TFLOP/s), for the DDA's arithmetic floor.
+FP64_VECTOR_TFLOPS = 78.6
+# FP64 operations of one DDA step: the min-tmax selection (2 compares),
+# exit_t = fmin(tmax, max_t), enter_t - exit_t, tmax += tdelta and the
+# enter_t >= max_t test, plus the per-visit sample position (2).
+FP64_OPS_PER_VISIT = 8
 
 
 def log(*a):
@@ -155,7 +161,7 @@ def cpu_baseline(ds, vol, tf, cams, samples, budget_s):
     }
 
 
-def serve_leg(ctx, vol512, tf512, rates, n):
+def serve_leg(ctx, vol512, tf512, rates, n, budgets=()):
     """Continuous-batching server over three resident datasets: 256^3 and
     512^3 uniform lognormal fields and the seeded AMR hierarchy (128^3 root,
     3 levels); requests of side uniform in [256, 2048], half projection and
@@ -188,11 +194,24 @@ def serve_leg(ctx, vol512, tf512, rates, n):
                            for kk, vv in serve.summarize(res, specs).items()}
         log(f"serving rate {lam:g}/s: {out[f'{lam:g}']}")
     srv.close()
+    # Smaller HBM budgets at the highest rate: admission control holds
+    # requests back until earlier ones are read.
+    by_budget = {}
+    lam = rates[-1] if rates else 200.0
+    for k, gib in enumerate(budgets):
+        srv = _lib.Server(ctx, max_requests=256, budget_bytes=int(gib * 2**30), chunk_tiles=16384)
+        specs = serve.make_requests(vols, n, seed=1106 + k, tfs=tfs, rng_cam_seed=1104 + k)
+        res = serve.run_poisson(srv, specs, lam, seed=1105 + k)
+        by_budget[f"{gib:g}"] = {kk: (round(vv, 3) if isinstance(vv, float) else vv)
+                                 for kk, vv in serve.summarize(res, specs).items()}
+        log(f"serving rate {lam:g}/s, budget {gib:g} GiB: {by_budget[f'{gib:g}']}")
+        srv.close()
     v256.free()
     vamr.free()
     return {"datasets": ["256^3 uniform", "512^3 uniform (8 bricks)",
                          f"AMR 128^3+3 levels ({len(amr.grids)} grids)"],
-            "sides": [256, 2048], "budget_gib": 8, "rates": out}
+            "sides": [256, 2048], "budget_gib": 8, "rates": out,
+            "budget_sweep": {"rate": lam, "budgets_gib": by_budget}}
 
 
 def main():
@@ -209,8 +228,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--lat-reps", type=int, default=20)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-rates", default="10,50,200",
+    ap.add_argument("--serve-rates", default="5,10,20,50,100,200",
                     help="Poisson arrival rates (req/s) of the serving leg")
+    ap.add_argument("--serve-budgets", default="1,0.25",
+                    help="smaller server budgets (GiB) swept at the highest rate")
     ap.add_argument("--serve-n", type=int, default=48, help="requests per rate")
     ap.add_argument("--no-alt", action="store_true",
                     help="skip the 64-brick (128^3 bricks) variant of the batched step")
@@ -243,6 +264,7 @@ def main():
     vol = render.ResidentVolume(ctx, ds, proj_fields="density", vr_field="density")
     log(f"resident: {vol.n_proj_bricks} projection blocks, {vol.n_vr_bricks} VR bricks, "
         f"{vol.device_bytes() / 2**30:.2f} GiB in {time.time() - t:.1f}s")
+    dataset_bytes = vol.device_bytes()
 
     # Sample counts (untimed record pass): visits with dt > 0 for projection,
     # mask-1 visits x n_samples for VR.
@@ -279,6 +301,12 @@ def main():
     wall = time.perf_counter() - t0
     ms_step = wall * 1e3 / args.steps
     gsps = step_samples / (wall / args.steps) / 1e9
+    # Compulsory bytes of a step: every resident brick is in view of every
+    # camera here, so each is read at least once, and every image channel
+    # is written once (1 for projection, 3 for VR).
+    image_bytes = sum(4 * npix[i] * (1 if cams[i][0] == 0 else 3) for i in range(len(cams)))
+    compulsory = dataset_bytes + image_bytes
+    fp64_ms = visits * FP64_OPS_PER_VISIT / (FP64_VECTOR_TFLOPS * 1e12) * 1e3
     imgs_s = len(cams) / (wall / args.steps)
     log(f"batched: {ms_step:.2f} ms/step, {gsps:.1f} Gsamples/s, {imgs_s:.1f} images/s")
     batch.close()
@@ -366,7 +394,8 @@ def main():
     serving = None
     if not args.no_serve:
         serving = serve_leg(ctx, vol, tf, [float(x) for x in args.serve_rates.split(",")],
-                            args.serve_n)
+                            args.serve_n,
+                            [float(x) for x in args.serve_budgets.split(",") if x.strip()])
 
     cpu = None
     if not args.no_cpu:
@@ -402,6 +431,9 @@ def main():
             "parallelism": "1 GPU",
         },
         "images_per_s": round(imgs_s, 2),
+        "compulsory": {"bytes": int(compulsory), "ms_at_peak": round(compulsory / HBM_PEAK_GBS / 1e6, 4),
+                       "note": "resident bricks read once + image written once, at 8 TB/s"},
+        "fp64_dda_ceiling_ms": round(fp64_ms, 3),
         "bricks_64": alt,
         "plain_images_per_s": None if plain is None else round(plain["images_per_s"], 2),
         "latency_ms": lat,

@@ -153,6 +153,38 @@ def test_vr_grey_opacity(ctx):
         vol.free()
 
 
+def test_vr_zero_and_nan_vertex_values(ctx):
+    """FIT_get_value returns 0 for values that are not normal
+    (field_interpolation_tables.pxd:74): a block of density exactly 1
+    (log10 -> 0, inside the table range) and cells <= 0 (log10 -> NaN)
+    contribute nothing, in the ColorTransferFunction path (10 sub-samples)."""
+    ds = uniform_lognormal(32, seed=29)
+    rho = ds.grids[0]["density"]
+    rho[4:14, 6:20, 3:17] = 1.0
+    rho[20:23, 20:24, 8:11] = -1.0
+    tf = ColorTransferFunction((-1.0, 1.0), nbins=128)
+    for j, x in enumerate(np.linspace(-0.9, 0.9, 6)):
+        tf.add_gaussian(x, 0.02, [0.9, 0.3 + 0.1 * j, 0.2, 4.0])
+    vol = render.ResidentVolume(ctx, ds, vr_field="density")
+    try:
+        p = c0_params(96, normal=(0.25, 0.6, -0.4))
+        spec = vol.vr_request(p, tf, n_samples=10, record=True)
+        (img, rec), = _run(ctx, [spec])
+        bricks = oracle_vr_bricks(vol)
+        assert np.isnan(bricks[0]["data"][0]).any()
+        assert (bricks[0]["data"][0] == 0.0).sum() > 1000
+        oimg, ovis, ohash = oracle.cast(1, p, bricks, order=spec.order, tf=tf_dict(tf),
+                                        n_samples=10, record=True)
+        _check_visits(rec, ovis, ohash, "vr zero/NaN values")
+        for c in range(3):
+            assert np.all(np.isfinite(img[..., c]))
+            e = rel_l2(img[..., c], oimg[..., c])
+            print(f"vr zero/NaN ch{c}: rel-L2={e:.3e}", flush=True)
+            assert e <= VR_TOL
+    finally:
+        vol.free()
+
+
 def test_batch_many_images(ctx):
     """Several requests of different sizes and kinds in one batch give the
     same images as each run alone."""

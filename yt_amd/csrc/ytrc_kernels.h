@@ -279,12 +279,12 @@ __device__ __forceinline__ void fit_eval_general(const TFCache &tf, const float4
 // (field_interpolation_tables.pxd:63-82) becomes u in [1, nb) through the
 // zero entries 0 and nb, nb + 1 and a clamp of u to [0, nb + 1]; the test
 // is thereby made on the FP32 bin coordinate, exact up to its rounding.
-// Values that are not normal (0, subnormal, inf, NaN) select entry 0.
+// FIT_get_value's test for values that are not normal is made once per
+// visit by the caller (quad_visit_ok), so v here is finite.
 __device__ __forceinline__ void fit_eval_quad(const TFCache &tf, const float4 *__restrict__ lt,
                                               float v, float e[3])
 {
     float u = __builtin_fmaf(v, tf.idbin, tf.qc1);
-    u = __builtin_isnormal(v) ? u : 0.0f;
     u = __builtin_amdgcn_fmed3f(u, 0.0f, tf.qzb);
     const unsigned bin = (unsigned)u;
     const float f = __builtin_amdgcn_fractf(u);
@@ -349,6 +349,26 @@ __device__ __forceinline__ void trilinear_cubic(const float (&c)[8], const float
     const float dydz = dy * dz;
     k[2] = __builtin_fmaf(fxy, dx * dy, __builtin_fmaf(fxz, dx * dz, fyz * dydz));
     k[3] = D * dx * dydz;
+}
+
+// FIT_get_value returns 0 for values that are not normal
+// (field_interpolation_tables.pxd:74, isnormal: false for 0, subnormals, inf
+// and NaN).  The ColorTransferFunction path makes that test once per visit
+// on the cubic k0 + k1 s + k2 s^2 + k3 s^3 of the sampled field:
+// * a NaN or infinite corner makes every sub-sample value of the visit
+//   non-finite (k0 depends on all 8 corners, and 0 * inf is NaN), and makes
+//   |k0| + |k1| + |k2| + |k3| non-finite;
+// * all coefficients 0 is a value of exactly 0 at every sub-sample.
+// Either way every sub-sample gives 0 and the visit changes nothing, so it
+// is skipped.  Otherwise the values are finite, and a sub-sample value that
+// is 0 or subnormal only through FP32 cancellation is looked up like its
+// neighbours (the FP64 value it approximates is normal).  Coefficients whose
+// magnitudes sum beyond FLT_MAX count as non-finite.
+__device__ __forceinline__ bool quad_visit_ok(const float (&k)[4])
+{
+    const float m = __builtin_fabsf(k[0]) + __builtin_fabsf(k[1]) + __builtin_fabsf(k[2]) +
+                    __builtin_fabsf(k[3]);
+    return __builtin_amdgcn_classf(m, (1 << 7) | (1 << 8));  // +subnormal or +normal
 }
 
 // VolumeRenderSampler.sample (image_samplers.pyx:453-487).
@@ -417,10 +437,14 @@ struct VRSampler {
 #pragma unroll
             for (int s = 0; s < nsq; s++, sf += 1.0f) {
                 float dvs[NFP], e[3];
+                // Sub-sample 0 is k0 (the coefficients are finite here).
 #pragma unroll
                 for (int f = 0; f < NFP; f++)
-                    dvs[f] = __builtin_fmaf(
-                        __builtin_fmaf(__builtin_fmaf(k[f][3], sf, k[f][2]), sf, k[f][1]), sf, k[f][0]);
+                    dvs[f] = (NS > 0 && s == 0)
+                                 ? k[f][0]
+                                 : __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(k[f][3], sf, k[f][2]), sf,
+                                                                 k[f][1]),
+                                                  sf, k[f][0]);
                 fit_eval_quad(T, lt, (NFP == 1) ? dvs[0] : pick(dvs, T.fid), e);
 #pragma unroll
                 for (int ch = 0; ch < 3; ch++) {
@@ -514,6 +538,16 @@ struct VRSampler {
             const float cf[8] = {cv[0][f], cv[1][f], cv[2][f], cv[3][f],
                                  cv[4][f], cv[5][f], cv[6][f], cv[7][f]};
             trilinear_cubic(cf, dp, ds, k[f]);
+        }
+        if constexpr ((MODE & 1) != 0) {
+            float kq[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                kq[j] = k[0][j];
+#pragma unroll
+                for (int f = 1; f < NFP; f++) kq[j] = (tf->fid == f) ? k[f][j] : kq[j];
+            }
+            if (!quad_visit_ok(kq)) return;
         }
         if (MODE & 2)
             samples<10>(k, dt, 10);
