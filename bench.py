@@ -188,7 +188,8 @@ def serve_leg(ctx, vol512, tf512, rates, n, budgets=()):
     srv = _lib.Server(ctx, max_requests=256, budget_bytes=8 << 30, chunk_tiles=16384)
     out = {}
     for k, lam in enumerate(rates):
-        specs = serve.make_requests(vols, n, seed=1006 + k, tfs=tfs, rng_cam_seed=1004 + k)
+        specs = serve.make_requests(vols, n, seed=1006 + k, tfs=tfs, rng_cam_seed=1004 + k,
+                                     deferred=True)
         res = serve.run_poisson(srv, specs, lam, seed=1005 + k)
         out[f"{lam:g}"] = {kk: (round(vv, 3) if isinstance(vv, float) else vv)
                            for kk, vv in serve.summarize(res, specs).items()}
@@ -200,7 +201,8 @@ def serve_leg(ctx, vol512, tf512, rates, n, budgets=()):
     lam = rates[-1] if rates else 200.0
     for k, gib in enumerate(budgets):
         srv = _lib.Server(ctx, max_requests=256, budget_bytes=int(gib * 2**30), chunk_tiles=16384)
-        specs = serve.make_requests(vols, n, seed=1106 + k, tfs=tfs, rng_cam_seed=1104 + k)
+        specs = serve.make_requests(vols, n, seed=1106 + k, tfs=tfs, rng_cam_seed=1104 + k,
+                                     deferred=True)
         res = serve.run_poisson(srv, specs, lam, seed=1105 + k)
         by_budget[f"{gib:g}"] = {kk: (round(vv, 3) if isinstance(vv, float) else vv)
                                  for kk, vv in serve.summarize(res, specs).items()}

@@ -271,6 +271,37 @@ int ytrc_hbm_probe(ytrc_ctx *ctx, uint64_t bytes, int32_t iters, float *gbps, fl
  * batch creation, from HIP events on the launch stream. */
 int ytrc_batch_kernel_ms(const ytrc_batch *b, float *avg_ms, int32_t *n_launches);
 
+/* kd-tree of an AMR grid hierarchy (host code): the brick set and brick
+ * order of the volume render.  Replaces AMRKDTree's build, Tree.build
+ * (yt/utilities/amr_kdtree/amr_kdtree.py:63-118) over Node.add_grids /
+ * insert_grids / split_grids / divide and kdtree_get_choices
+ * (yt/utilities/lib/amr_kdtools.pyx:193-532, :760-823), and its viewpoint
+ * traversal kd_traverse(viewpoint) (step_viewpoint, amr_kdtools.pyx:
+ * 612-622, 708-755).  Grids are added level by level, lowest first; within
+ * a level in the order given (the reference adds data_source.blocks in
+ * order). */
+typedef struct ytrc_kdtree ytrc_kdtree;
+typedef struct {
+    double left_edge[3];
+    double right_edge[3];
+    double split_pos; /* NaN at a leaf */
+    int64_t grid;     /* a leaf's grid id; -1 where the node carries none */
+    int64_t node_id;  /* 1 at the root, 2 id and 2 id + 1 for the children (mod 2^64) */
+    int32_t parent, left, right; /* node indices, -1 where none */
+    int32_t split_dim;           /* -1 at a leaf */
+} ytrc_kd_node;
+int ytrc_kdtree_build(int32_t n_grids, const double *left_edges, const double *right_edges,
+                      const int64_t *grid_ids, const int32_t *levels, ytrc_kdtree **out);
+/* Node count, and the count of leaves that carry a grid (the bricks). */
+int ytrc_kdtree_size(const ytrc_kdtree *t, int32_t *n_nodes, int32_t *n_leaves);
+/* All nodes; index 0 is the root, links are indices into this array. */
+int ytrc_kdtree_read_nodes(const ytrc_kdtree *t, ytrc_kd_node *nodes);
+/* The bricks farthest from `viewpoint` first, as indices into the
+ * depth-first (left first) list of leaves that carry a grid: the call order
+ * of VolumeSource.render (render_source.py:594-597).  order: n_leaves. */
+int ytrc_kdtree_order(const ytrc_kdtree *t, const double *viewpoint, int32_t *order);
+void ytrc_kdtree_destroy(ytrc_kdtree *t);
+
 #ifdef __cplusplus
 }
 #endif

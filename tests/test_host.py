@@ -125,6 +125,92 @@ def test_kdtree_viewpoint_order_is_front_to_back_along_rays():
         assert all(a <= b + 1e-12 for a, b in zip(tin, tin[1:]))
 
 
+def _same_tree(a, b):
+    na, nb = list(a.depth_traverse()), list(b.depth_traverse())
+    assert len(na) == len(nb)
+    for x, y in zip(na, nb):
+        assert x.split_dim == y.split_dim and x.grid == y.grid and x.node_id == y.node_id
+        assert (x.split_pos == y.split_pos) or (np.isnan(x.split_pos) and np.isnan(y.split_pos))
+        assert np.array_equal(x.left_edge, y.left_edge) and np.array_equal(x.right_edge, y.right_edge)
+        assert x.is_leaf() == y.is_leaf()
+
+
+@pytest.mark.parametrize("case", ["amr16", "amr32", "amr64x3", "uniform8", "psize343"])
+def test_native_kdtree_equals_oracle(case):
+    """The native build (ytrc_kdtree_*) against the Python restatement in
+    oracle/kdtree.py: the same nodes, splits, grids and node ids, and the
+    same brick order from 24 viewpoints, inside and outside the domain."""
+    from oracle import kdtree as okd
+    from tests.helpers import small_amr, uniform_lognormal
+
+    ds = {"amr16": lambda: small_amr(root_n=16, levels=2, seed=11),
+          "amr32": lambda: small_amr(root_n=32, levels=2, seed=7),
+          "amr64x3": lambda: small_amr(root_n=64, levels=3, seed=3),
+          "uniform8": lambda: uniform_lognormal(16, nprocs=8),
+          "psize343": lambda: uniform_lognormal(21, psize=(3, 4, 3))}[case]()
+    kd = amr_kdtree.AMRKDTree(ds)
+    ot = okd.build([(g.LeftEdge, g.RightEdge, g.id, g.Level) for g, _ in ds.blocks()])
+    _same_tree(kd.trunk, ot)
+    rng = np.random.default_rng(17)
+    vps = list(rng.uniform(-1.0, 2.0, size=(20, 3))) + [
+        np.array([0.5, 0.5, 0.5]), np.array([0.25, 0.5, 0.75]),
+        np.array([1e6, -1e6, 3.0]), np.array([0.0, 0.0, 0.0])]
+    for vp in vps:
+        assert np.array_equal(kd.order(vp), okd.order(ot, vp))
+    # the mirrored Python traversal gives the same order
+    idx = {id(n): i for i, n in enumerate(kd.leaves())}
+    assert np.array_equal(kd.order(vps[0]),
+                          [idx[id(n)] for n in kd.trunk.kd_traverse(viewpoint=vps[0])])
+
+
+def test_oracle_kdtree_invariants():
+    """oracle/kdtree.py is pinned by what a kd-tree of the hierarchy must
+    satisfy: the leaves tile the domain; each leaf's grid covers it and no
+    finer grid overlaps it; painter's order along lines of sight."""
+    from oracle import kdtree as okd
+
+    ds = _small_amr()
+    ot = okd.build([(g.LeftEdge, g.RightEdge, g.id, g.Level) for g, _ in ds.blocks()])
+    leaves = list(ot.kd_traverse())
+    vol = sum(float(np.prod(n.right_edge - n.left_edge)) for n in leaves)
+    assert abs(vol - 1.0) < 1e-12
+    by_id = {g.id: g for g in ds.grids}
+    for node in leaves:
+        g = by_id[node.grid]
+        assert np.all(g.LeftEdge <= node.left_edge) and np.all(g.RightEdge >= node.right_edge)
+        for h in ds.grids:
+            if h.Level > g.Level:
+                lo = np.maximum(h.LeftEdge, node.left_edge)
+                hi = np.minimum(h.RightEdge, node.right_edge)
+                assert np.any(hi <= lo)
+    rng = np.random.default_rng(9)
+    for _ in range(10):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        vp = 0.5 + n * 1e6
+        pos = rng.uniform(0.2, 0.8, 3) - n
+        tin = []
+        for node in ot.kd_traverse(viewpoint=vp):
+            t0, t1 = -np.inf, np.inf
+            for a in range(3):
+                ta = (node.left_edge[a] - pos[a]) / n[a]
+                tb = (node.right_edge[a] - pos[a]) / n[a]
+                t0, t1 = max(t0, min(ta, tb)), min(t1, max(ta, tb))
+            if t1 > t0:
+                tin.append(t0)
+        assert len(tin) > 1
+        assert all(a <= b + 1e-12 for a, b in zip(tin, tin[1:]))
+
+
+def test_native_kdtree_errors():
+    with pytest.raises(ValueError):
+        _lib.KdTree([[0, 0, 0]], [[1, 0, 1]], [0], [0])   # empty extent
+    with pytest.raises(ValueError):
+        _lib.KdTree([[0, 0, 0]], [[1, 1, 1]], [-1], [0])  # -1 means "no grid"
+    t = _lib.KdTree(np.zeros((0, 3)), np.zeros((0, 3)), [], [])
+    assert t.n_leaves == 0 and t.n_nodes == 1
+
+
 def test_tf_from_ctf_object():
     from yt_amd.transfer_functions import gaussian_ctf
 

@@ -58,6 +58,11 @@ EXPORTED = [
     "ytrc_batch_kernel_ms",
     "ytrc_batch_request_ms",
     "ytrc_hbm_probe",
+    "ytrc_kdtree_build",
+    "ytrc_kdtree_size",
+    "ytrc_kdtree_read_nodes",
+    "ytrc_kdtree_order",
+    "ytrc_kdtree_destroy",
     "ytrc_server_create",
     "ytrc_server_submit",
     "ytrc_server_progress",
@@ -112,6 +117,21 @@ class LeafDesc(C.Structure):
         ("pad", C.c_int32),
         ("left_edge", C.c_double * 3),
         ("right_edge", C.c_double * 3),
+    ]
+
+
+class KdNode(C.Structure):
+    """ytrc_kd_node."""
+    _fields_ = [
+        ("left_edge", C.c_double * 3),
+        ("right_edge", C.c_double * 3),
+        ("split_pos", C.c_double),
+        ("grid", C.c_int64),
+        ("node_id", C.c_int64),
+        ("parent", C.c_int32),
+        ("left", C.c_int32),
+        ("right", C.c_int32),
+        ("split_dim", C.c_int32),
     ]
 
 
@@ -185,6 +205,13 @@ def _declare(lib):
     lib.ytrc_batch_sync.argtypes = [P, C.POINTER(C.c_float)]
     lib.ytrc_batch_read_image.argtypes = [P, C.c_int32, C.POINTER(C.c_double)]
     lib.ytrc_batch_read_partial.argtypes = [P, C.c_int32, C.POINTER(C.c_double)]
+    lib.ytrc_kdtree_build.argtypes = [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(P)]
+    lib.ytrc_kdtree_size.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    lib.ytrc_kdtree_read_nodes.argtypes = [P, C.POINTER(KdNode)]
+    lib.ytrc_kdtree_order.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+    lib.ytrc_kdtree_destroy.argtypes = [P]
+    lib.ytrc_kdtree_destroy.restype = None
     lib.ytrc_batch_read_record.argtypes = [
         P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_uint64), C.POINTER(C.c_int64)]
     lib.ytrc_batch_set_launch.argtypes = [P, C.c_int32, C.c_int32]
@@ -246,6 +273,51 @@ def _check(rc):
 
 def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class KdTree:
+    """The native kd-tree of a grid hierarchy (ytrc_kdtree_*): host code,
+    no device needed.  left_edges / right_edges: (n, 3); ids: grid ids
+    (>= 0); levels: grid levels."""
+
+    def __init__(self, left_edges, right_edges, ids, levels):
+        le = np.ascontiguousarray(left_edges, dtype=np.float64).reshape(-1, 3)
+        re = np.ascontiguousarray(right_edges, dtype=np.float64).reshape(-1, 3)
+        ids = np.ascontiguousarray(ids, dtype=np.int64)
+        levels = np.ascontiguousarray(levels, dtype=np.int32)
+        n = len(ids)
+        if not (len(le) == len(re) == len(levels) == n):
+            raise ValueError("KdTree: one edge pair, id and level per grid")
+        self._h = C.c_void_p()
+        _check(lib().ytrc_kdtree_build(
+            n, _dptr(le), _dptr(re), ids.ctypes.data_as(C.POINTER(C.c_int64)),
+            levels.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(self._h)))
+        nn, nl = C.c_int32(), C.c_int32()
+        _check(lib().ytrc_kdtree_size(self._h, C.byref(nn), C.byref(nl)))
+        self.n_nodes, self.n_leaves = nn.value, nl.value
+
+    def nodes(self):
+        out = (KdNode * self.n_nodes)()
+        _check(lib().ytrc_kdtree_read_nodes(self._h, out))
+        return list(out)
+
+    def order(self, viewpoint):
+        vp = np.ascontiguousarray(viewpoint, dtype=np.float64).ravel()[:3]
+        out = np.empty(self.n_leaves, dtype=np.int32)
+        _check(lib().ytrc_kdtree_order(self._h, _dptr(vp),
+                                       out.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().ytrc_kdtree_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Context:

@@ -1,17 +1,17 @@
 """AMRKDTree: the brick set and brick order of VolumeRenderSampler.
 
-Restates:
-  Node, add_grids, insert_grids, split_grids, divide  amr_kdtools.pyx:21-532
-  kdtree_get_choices                                  amr_kdtools.pyx:760-823
-  viewpoint_traverse / step_viewpoint (farthest first) amr_kdtools.pyx:612-622,708-755
-  Tree.build (levels coarse -> fine, blocks order)    amr_kdtree.py:63-118
-  AMRKDTree.set_fields / traverse / get_brick_data    amr_kdtree.py:161-374
-for one process (rank 0 of 1: should_i_build is always true and
-should_i_split never fires).
+The tree is built, and ordered for each viewpoint, by the library's native
+kd-tree (ytrc_kdtree_*, restating Tree.build, amr_kdtree.py:63-118, over
+Node.add_grids / insert_grids / split_grids / divide and kdtree_get_choices,
+amr_kdtools.pyx:193-532, 760-823, and step_viewpoint, amr_kdtools.pyx:
+708-755).  This module mirrors its nodes as Node objects for the Python
+side: traversal, subtrees, and the bricks of AMRKDTree.set_fields /
+traverse / get_brick_data (amr_kdtree.py:161-374), single process.
 """
 
 import numpy as np
 
+from . import _lib
 from .partitioned_grid import PartitionedGrid
 
 
@@ -40,58 +40,6 @@ class Node:
 
     def get_right_edge(self):
         return self.right_edge.copy()
-
-    # amr_kdtools.pyx:193-262
-    def add_grids(self, gles, gres, gids):
-        if self.is_leaf():
-            self.insert_grids(gles, gres, gids)
-            return
-        d, p = self.split_dim, self.split_pos
-        less = gles[:, d] < p
-        greater = gres[:, d] > p
-        if less.any():
-            self.left.add_grids(gles[less], gres[less], gids[less])
-        if greater.any():
-            self.right.add_grids(gles[greater], gres[greater], gids[greater])
-
-    # amr_kdtools.pyx:275-313
-    def insert_grids(self, gles, gres, gids):
-        n = len(gids)
-        if n == 0:
-            return
-        if n == 1:
-            if np.all(gles[0] <= self.left_edge) and np.all(gres[0] >= self.right_edge):
-                self.grid = int(gids[0])
-                return
-        if self.split_grids(gles, gres, gids) == -1:
-            self.grid = -1
-
-    # amr_kdtools.pyx:369-458
-    def split_grids(self, gles, gres, gids):
-        if len(gids) == 0:
-            return 0
-        best_dim, split, less_ids, greater_ids = kdtree_get_choices(
-            gles, gres, self.left_edge, self.right_edge)
-        if best_dim == -1:
-            return -1
-        self.divide(best_dim, split)
-        if less_ids.any():
-            self.left.insert_grids(gles[less_ids], gres[less_ids], gids[less_ids])
-        if greater_ids.any():
-            self.right.insert_grids(gles[greater_ids], gres[greater_ids], gids[greater_ids])
-        return 0
-
-    # amr_kdtools.pyx:509-532
-    def divide(self, dim, pos):
-        self.split_dim = int(dim)
-        self.split_pos = float(pos)
-        le = self.left_edge.copy()
-        re = self.right_edge.copy()
-        re[dim] = pos
-        self.left = Node(self, None, None, le, re, self.grid, self.node_id << 1)
-        re[dim] = self.right_edge[dim]
-        le[dim] = pos
-        self.right = Node(self, None, None, le, re, self.grid, (self.node_id << 1) + 1)
 
     def depth_traverse(self):
         yield self
@@ -126,29 +74,11 @@ class Node:
         return self.left.kd_sum_volume() + self.right.kd_sum_volume()
 
 
-def kdtree_get_choices(gles, gres, l_corner, r_corner):
-    """amr_kdtools.pyx:760-823: per dimension, the unique grid faces strictly
-    inside the node; the dimension with the most wins (first on ties); the
-    split is the ((n-1)//2)-th smallest."""
-    best_dim, my_max, my_split = -1, 0, 0
-    uniq_best = None
-    for dim in range(3):
-        v = np.concatenate([gles[:, dim], gres[:, dim]])
-        inside = (l_corner[dim] < v) & (v < r_corner[dim])
-        u = np.unique(v[inside])
-        if len(u) > my_max:
-            best_dim, my_max, my_split = dim, len(u), (len(u) - 1) // 2
-            uniq_best = u
-    if best_dim == -1:
-        return -1, 0.0, None, None
-    split = np.sort(uniq_best)[my_split]
-    less_ids = gles[:, best_dim] < split
-    greater_ids = gres[:, best_dim] > split
-    return best_dim, split, less_ids, greater_ids
-
-
 class AMRKDTree:
-    """amr_kdtree.py:161-374 (single process)."""
+    """amr_kdtree.py:161-374 (single process).  The grids of
+    data_source.blocks() between min_level and max_level go to the native
+    build level by level, in blocks order; the tree's nodes are mirrored as
+    Node objects (self.trunk), and order(viewpoint) asks the native tree."""
 
     def __init__(self, ds, min_level=None, max_level=None, data_source=None):
         self.ds = ds
@@ -157,19 +87,30 @@ class AMRKDTree:
         self.log_fields = None
         self.no_ghost = True
         self._vcd_cache = {}
-        self.trunk = Node(None, None, None, [-np.inf] * 3, [np.inf] * 3, -1, 1)
         min_level = 0 if min_level is None else min_level
         max_level = ds.max_level if max_level is None else max_level
-        blocks = list(self.data_source.blocks())
-        for lvl in range(min_level, max_level + 1):
-            grids = [b for b, _ in blocks if b.Level == lvl]
-            if not grids:
-                continue
-            gles = np.array([g.LeftEdge for g in grids])
-            gres = np.array([g.RightEdge for g in grids])
-            gids = np.array([g.id for g in grids], dtype="int64")
-            self.trunk.add_grids(gles, gres, gids)
+        grids = [g for g, _ in self.data_source.blocks() if min_level <= g.Level <= max_level]
+        self._native = _lib.KdTree(
+            np.array([g.LeftEdge for g in grids], dtype="float64").reshape(-1, 3),
+            np.array([g.RightEdge for g in grids], dtype="float64").reshape(-1, 3),
+            np.array([g.id for g in grids], dtype="int64"),
+            np.array([g.Level for g in grids], dtype="int32"))
+        nodes = self._native.nodes()
+        mirror = [Node(None, None, None, n.left_edge, n.right_edge, n.grid, n.node_id)
+                  for n in nodes]
+        for m, n in zip(mirror, nodes):
+            m.split_dim = int(n.split_dim)
+            m.split_pos = float(n.split_pos)
+            m.parent = mirror[n.parent] if n.parent >= 0 else None
+            m.left = mirror[n.left] if n.left >= 0 else None
+            m.right = mirror[n.right] if n.right >= 0 else None
+        self.trunk = mirror[0]
         self._grid_by_id = {g.id: g for g in ds.grids}
+
+    def order(self, viewpoint):
+        """The bricks farthest from the viewpoint first, as indices into
+        leaves() (kd_traverse(viewpoint), amr_kdtools.pyx:612-622)."""
+        return self._native.order(viewpoint)
 
     def set_fields(self, fields, log_fields, no_ghost=True, force=False):
         if not np.iterable(log_fields) or isinstance(log_fields, (str, bytes)):

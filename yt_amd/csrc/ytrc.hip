@@ -28,6 +28,7 @@
 
 #include "../../include/ytrc.h"
 #include "ytrc_device.h"
+#include "ytrc_host.h"
 #include "ytrc_kernels.h"
 #include "ytrc_pick.h"
 
@@ -176,6 +177,12 @@ int fail(int code, const std::string &msg)
     g_err = msg;
     return code;
 }
+
+}  // namespace
+
+int ytrc::set_error(int code, const std::string &msg) { return fail(code, msg); }
+
+namespace {
 
 #define HIPCHK(expr)                                                                        \
     do {                                                                                    \

@@ -100,8 +100,9 @@ class ResidentVolume:
         return b
 
     def vr_order(self, viewpoint):
-        return np.array([self._leaf_index[id(n)] for n in
-                         self.kd.trunk.kd_traverse(viewpoint=viewpoint)], dtype=np.int32)
+        """The kd bricks farthest from the viewpoint first (the native
+        tree's order; indices into self.leaves)."""
+        return self.kd.order(viewpoint)
 
     def projection_request(self, params, record=False, zbuffer=None):
         if self.proj_ds_id is None:

@@ -5,7 +5,9 @@ seeded Poisson times.  Each is offered to the continuous-batching server
 (ytrc_server_*) when it arrives; a request the server cannot admit yet (no
 slot or HBM budget) waits in arrival order and is offered again as finished
 requests are read back and release their memory.  Latency is measured from
-arrival to the image being in host memory.
+arrival to the image being in host memory; with deferred requests it
+includes the host's preparation of the request at arrival (camera
+parameters, the kd-tree brick order for the viewpoint).
 """
 
 import time
@@ -17,25 +19,33 @@ from .render import camera_params
 
 
 def make_requests(volumes, n, seed=1006, side_range=(256, 2048), vr_fraction=0.5,
-                  tfs=None, rng_cam_seed=1004):
-    """n request specs spread over the resident volumes: random view
-    direction and north, side uniform in side_range, projection or VR."""
+                  tfs=None, rng_cam_seed=1004, deferred=False):
+    """n requests spread over the resident volumes: random view direction
+    and north, side uniform in side_range, projection or VR.  Returns
+    request specs, or with deferred=True callables that build them (so that
+    run_poisson prepares each request when it arrives)."""
     rng = np.random.default_rng(rng_cam_seed)
     rs = np.random.default_rng(seed)
-    specs = []
+    tfc = {k: (v if isinstance(v, _lib.TF) else _lib.TF.from_tf_obj(v))
+           for k, v in (tfs or {}).items()}
+    out = []
     for i in range(n):
         vol = volumes[i % len(volumes)]
         nrm = rng.normal(size=3)
         nrm /= np.linalg.norm(nrm)
         side = int(rs.integers(side_range[0], side_range[1] + 1))
-        p = camera_params(vol.ds.domain_center, nrm, float(vol.ds.domain_width.max()), side,
-                          rng.normal(size=3))
+        north = rng.normal(size=3)
         vr = (vol.vr_ds_id is not None) and (vol.proj_ds_id is None or rs.random() < vr_fraction)
-        if vr:
-            specs.append(vol.vr_request(p, tfs[id(vol)] if tfs else None))
-        else:
-            specs.append(vol.projection_request(p))
-    return specs
+
+        def build(vol=vol, nrm=nrm, side=side, north=north, vr=vr):
+            p = camera_params(vol.ds.domain_center, nrm, float(vol.ds.domain_width.max()), side,
+                              north)
+            if vr:
+                return vol.vr_request(p, tfc.get(id(vol)))
+            return vol.projection_request(p)
+
+        out.append(build if deferred else build())
+    return out
 
 
 def run_poisson(server, specs, rate, seed=1005, readback="planar", idle_sleep=2e-4):
@@ -57,6 +67,8 @@ def run_poisson(server, specs, rate, seed=1005, readback="planar", idle_sleep=2e
             nxt += 1
         while waiting:
             i = waiting[0]
+            if callable(specs[i]):
+                specs[i] = specs[i]()
             rid = server.submit(specs[i])
             if rid is None:
                 waits += 1
