@@ -146,3 +146,40 @@ def test_server_all_sampler_kinds_and_lenses(ctx, volumes):
                                                                                b[~np.isnan(b)]), i
     finally:
         srv.close()
+
+
+def test_server_reuses_released_transfer_functions(ctx, volumes):
+    """Transfer-function tables stay in the server's table after their
+    requests are read, are found again by content, and are forgotten when a
+    new table is written over them; the launch always copies every table in
+    use into LDS.  Sequence: T1 (256 bins) and T2 (128 bins) placed and
+    released; T3 (64 bins) is written over T1; T2, found again at its old
+    place beyond T3, must still render exactly as alone."""
+    vols, _ = volumes
+    va = vols[0]
+    t1 = vr_tf_for(va.ds, nbins=256)
+    t2 = vr_tf_for(va.ds, nbins=128)
+    t3 = vr_tf_for(va.ds, nbins=64)
+    p = c0_params(72, normal=(0.3, -0.4, 0.87))
+    srv = _lib.Server(ctx, max_requests=8, budget_bytes=64 << 20, chunk_tiles=64)
+
+    def run(specs):
+        ids = [srv.submit(s) for s in specs]
+        assert None not in ids
+        got = {}
+        while len(got) < len(ids):
+            for rid in srv.wait():
+                got[rid] = srv.read(rid)
+        return [got[i] for i in ids]
+
+    try:
+        run([va.vr_request(p, t1), va.vr_request(p, t2)])
+        run([va.vr_request(p, t3)])
+        s2 = va.vr_request(p, t2)
+        img, = run([s2])
+        assert np.array_equal(img[..., :3], _alone(ctx, s2)[..., :3])
+        s1 = va.vr_request(p, t1)
+        img1, = run([s1])
+        assert np.array_equal(img1[..., :3], _alone(ctx, s1)[..., :3])
+    finally:
+        srv.close()

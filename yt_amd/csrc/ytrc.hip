@@ -1651,6 +1651,8 @@ int srv_tf_acquire(ytrc_server *S, const std::vector<float4> &t, int &entry)
     for (size_t i = 0; i < S->tfs.size(); i++)
         if (S->tfs[i].len > 0 && S->tfs[i].key == key) {
             S->tfs[i].refs++;
+            // launches copy the table's first tf_hi entries into LDS
+            S->tf_hi = std::max(S->tf_hi, S->tfs[i].base + S->tfs[i].len);
             entry = (int)i;
             return YTRC_OK;
         }
@@ -1666,9 +1668,20 @@ int srv_tf_acquire(ytrc_server *S, const std::vector<float4> &t, int &entry)
     }
     if (pos + (int)t.size() > SRV_TF_CAP)
         return fail(YTRC_EBUDGET, "server: no room for another transfer function in LDS");
+    // Released tables keep their place (and are found again by key) until a
+    // new table is written over them: forget every released table the new
+    // one overlaps.
+    for (auto &e : S->tfs)
+        if (e.refs == 0 && e.len > 0 && e.base < pos + (int)t.size() && pos < e.base + e.len) {
+            e.len = 0;
+            e.key.clear();
+        }
     HIPCHK(hipMemcpy(S->d_tf + pos, t.data(), t.size() * sizeof(float4), hipMemcpyHostToDevice));
+    // A free entry: one forgotten above, else any released one.
     int slot = -1;
-    for (size_t i = 0; i < S->tfs.size(); i++)
+    for (size_t i = 0; i < S->tfs.size() && slot < 0; i++)
+        if (S->tfs[i].refs == 0 && S->tfs[i].len == 0) slot = (int)i;
+    for (size_t i = 0; i < S->tfs.size() && slot < 0; i++)
         if (S->tfs[i].refs == 0) slot = (int)i;
     if (slot < 0) {
         S->tfs.push_back(TFEntry());
