@@ -34,6 +34,7 @@ extern "C" {
 #define YTRC_EBUDGET (-3)   /* request would exceed the context's HBM budget              */
 #define YTRC_ENOTIMPL (-4)  /* e.g. unknown volume method (image_samplers.pyx:137-141)    */
 #define YTRC_ECANCELLED (-5)
+#define YTRC_EBUSY (-6)     /* the dataset is still used by a batch or a server request    */
 
 /* Sampler kinds: ProjectionSampler (image_samplers.pyx:327-343),
  * VolumeRenderSampler (image_samplers.pyx:405-494) and
@@ -142,6 +143,10 @@ int ytrc_dataset_build_vr(ytrc_ctx *ctx, int32_t n_grids, const ytrc_grid_desc *
  * brick preparation against the host's. */
 int ytrc_dataset_read_points(ytrc_ctx *ctx, int32_t ds_id, int32_t brick, float *out, int64_t cap,
                              int64_t *n_out);
+/* Frees a dataset's device memory.  Refused with YTRC_EBUSY while a live
+ * batch or an unread server request uses it (a batch holds its datasets
+ * until ytrc_batch_destroy; a server request until it is read, cancelled or
+ * the server is destroyed). */
 int ytrc_dataset_free(ytrc_ctx *ctx, int32_t ds_id);
 int ytrc_dataset_bytes(const ytrc_ctx *ctx, int32_t ds_id, uint64_t *bytes);
 

@@ -183,3 +183,33 @@ def test_server_reuses_released_transfer_functions(ctx, volumes):
         assert np.array_equal(img1[..., :3], _alone(ctx, s1)[..., :3])
     finally:
         srv.close()
+
+
+def test_dataset_free_refused_while_in_use(ctx):
+    """ytrc_dataset_free refuses a dataset that a live batch or an unread
+    server request still reads (YTRC_EBUSY), so no kernel can read freed
+    memory; destroying the batch, reading the request or destroying the
+    server gives it back."""
+    ds = uniform_lognormal(16, seed=5)
+    vol = render.ResidentVolume(ctx, ds, proj_fields="density")
+    p = c0_params(24)
+    b = _lib.Batch(ctx, [vol.projection_request(p)])
+    with pytest.raises(_lib.YtrcError):
+        ctx.free(vol.proj_ds_id)
+    b.run()
+    b.close()
+    srv = _lib.Server(ctx, max_requests=4, budget_bytes=16 << 20, chunk_tiles=64)
+    try:
+        rid = srv.submit(vol.projection_request(p))
+        assert rid is not None
+        with pytest.raises(_lib.YtrcError):
+            ctx.free(vol.proj_ds_id)
+        done = []
+        while rid not in done:
+            done += srv.wait()
+        srv.read(rid)
+        assert srv.submit(vol.projection_request(p)) is not None  # left unread
+    finally:
+        srv.close()
+    vol.free()
+    assert vol.proj_ds_id is None

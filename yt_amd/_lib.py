@@ -20,6 +20,7 @@ YTRC_EDEVICE = -2
 YTRC_EBUDGET = -3
 YTRC_ENOTIMPL = -4
 YTRC_ECANCELLED = -5
+YTRC_EBUSY = -6
 
 PROJECTION = 0
 VOLUME_RENDER = 1

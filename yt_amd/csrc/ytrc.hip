@@ -202,6 +202,7 @@ struct Dataset {
     float *d_pool = nullptr;
     uint8_t *d_masks = nullptr;
     uint64_t bytes = 0;
+    int refs = 0;  // live batches and server requests that read it
 };
 
 }  // namespace
@@ -269,6 +270,7 @@ struct ytrc_batch {
     int launched = 0;
     double total_ms = 0.0;
     int timed = 0;
+    std::vector<int> ds_ids;  // datasets this batch holds a reference on
 };
 
 namespace {
@@ -724,6 +726,10 @@ int ytrc_dataset_free(ytrc_ctx *ctx, int32_t ds_id)
     auto it = ctx->datasets.find(ds_id);
     if (it == ctx->datasets.end())
         return fail(YTRC_EINVAL, "ytrc_dataset_free: unknown dataset " + std::to_string(ds_id));
+    if (it->second.refs > 0)
+        return fail(YTRC_EBUSY, "ytrc_dataset_free: dataset " + std::to_string(ds_id) + " is used by " +
+                                    std::to_string(it->second.refs) +
+                                    " live batch(es) or server request(s); destroy or read them first");
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     hipFree(it->second.d_bricks);
@@ -1272,6 +1278,12 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
         B->ev_req.resize(2 * n_req);
         for (auto &ev : B->ev_req) HIPCHK(hipEventCreate(&ev));
     }
+    // Hold the datasets until the batch is destroyed (ytrc_dataset_free
+    // refuses a dataset in use).
+    for (int i = 0; i < n_req; i++)
+        if (std::find(B->ds_ids.begin(), B->ds_ids.end(), reqs[i].ds_id) == B->ds_ids.end())
+            B->ds_ids.push_back(reqs[i].ds_id);
+    for (int id : B->ds_ids) ctx->datasets[id].refs++;
     *out = B;
     return YTRC_OK;
 }
@@ -1496,6 +1508,13 @@ void ytrc_batch_destroy(ytrc_batch *b)
         std::lock_guard<std::mutex> lk(b->ctx->mu);
         b->ctx->used -= b->arena_bytes;
     }
+    {
+        std::lock_guard<std::mutex> lk(b->ctx->mu);
+        for (int id : b->ds_ids) {
+            auto it = b->ctx->datasets.find(id);
+            if (it != b->ctx->datasets.end()) it->second.refs--;
+        }
+    }
     if (b->ev0) hipEventDestroy(b->ev0);
     if (b->ev1) hipEventDestroy(b->ev1);
     for (auto &e : b->ev_req) hipEventDestroy(e);
@@ -1577,6 +1596,7 @@ struct SrvReq {
     int64_t last_chunk = -1;
     int state = 0;  // 0 tiles pending, 1 all tiles launched, 2 done, 3 cancelled
     int tf_entry = -1;
+    bool ds_ref = false;  // holds a reference on its dataset
 };
 
 struct TFEntry {
@@ -1712,6 +1732,12 @@ void srv_release(ytrc_server *S, SrvReq &R)
     if (R.tf_entry >= 0) {
         S->tfs[R.tf_entry].refs--;
         R.tf_entry = -1;
+    }
+    if (R.ds_ref) {
+        std::lock_guard<std::mutex> lc(S->ctx->mu);
+        auto it = S->ctx->datasets.find(R.P.q.r.ds_id);
+        if (it != S->ctx->datasets.end()) it->second.refs--;
+        R.ds_ref = false;
     }
 }
 
@@ -1874,13 +1900,22 @@ int ytrc_server_submit(ytrc_server *S, const ytrc_request *r, int64_t *id_out)
         std::lock_guard<std::mutex> lc(S->ctx->mu);
         const int rc = prep_request(S->ctx, *r, "request: ", R.P);
         if (rc != YTRC_OK) return rc;
+        // Held until the request is read or cancelled (srv_release).
+        S->ctx->datasets[r->ds_id].refs++;
+        R.ds_ref = true;
     }
     // Admission: a descriptor slot and arena room for the request's region.
-    if (S->free_slots.empty()) return fail(YTRC_EBUDGET, "server: all request slots are in use");
+    R.off = (size_t)-1;
+    if (S->free_slots.empty()) {
+        srv_release(S, R);
+        return fail(YTRC_EBUDGET, "server: all request slots are in use");
+    }
     size_t off = 0;
-    if (!S->alloc.alloc(R.P.bytes, off))
+    if (!S->alloc.alloc(R.P.bytes, off)) {
+        srv_release(S, R);
         return fail(YTRC_EBUDGET, "server: request needs " + std::to_string(R.P.bytes) +
                                       " bytes, more than the free HBM budget");
+    }
     R.off = off;
     S->used += align_up(R.P.bytes, 256);
     int tf_base = 0;
@@ -2054,6 +2089,12 @@ void ytrc_server_destroy(ytrc_server *S)
     {
         std::lock_guard<std::mutex> lk(S->ctx->mu);
         S->ctx->used -= S->budget;
+        // Requests never read or cancelled give their datasets back.
+        for (auto &kv : S->reqs) {
+            if (!kv.second.ds_ref) continue;
+            auto it = S->ctx->datasets.find(kv.second.P.q.r.ds_id);
+            if (it != S->ctx->datasets.end()) it->second.refs--;
+        }
     }
     delete S;
 }
