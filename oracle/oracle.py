@@ -47,7 +47,7 @@ def lib():
         L.orc_cast.argtypes = [
             C.c_int, D, D, D, D, D, D, D, C.c_int, C.c_int, C.c_int, D, D, I32, C.c_int,
             C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int, I32, C.c_void_p, C.c_int,
-            D, D, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]
+            D, D, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_uint64), C.POINTER(C.c_int64)]
         L.orc_cast_lens.restype = C.c_int
         L.orc_cast_lens.argtypes = [C.c_int, C.c_int, D, D, D] + L.orc_cast.argtypes[1:]
         L.orc_extents_perspective.restype = C.c_int
@@ -100,7 +100,7 @@ class _TFDesc(C.Structure):
 
 
 def cast(kind, params, bricks, order=None, tf=None, n_samples=10, image=None, zbuffer=None,
-         num_threads=0, record=False):
+         num_threads=0, record=False, with_samples=False):
     """Runs the oracle for one image.
 
     params: sampler parameters (vp_pos, vp_dir, center, bounds, x_vec, y_vec,
@@ -160,10 +160,11 @@ def cast(kind, params, bricks, order=None, tf=None, n_samples=10, image=None, zb
     zb = None
     if zbuffer is not None:
         zb = np.ascontiguousarray(zbuffer, dtype=np.float64)
-    visits = hashes = None
+    visits = hashes = samples = None
     if record:
         visits = np.zeros((nv0, nv1), dtype=np.int64)
         hashes = np.zeros((nv0, nv1), dtype=np.uint64)
+        samples = np.zeros((nv0, nv1), dtype=np.int64)
     lens = 0 if params.get("lens_type", "plane-parallel") == "plane-parallel" else 1
     pix_pos = pix_dir = cam = None
     if lens:
@@ -189,9 +190,12 @@ def cast(kind, params, bricks, order=None, tf=None, n_samples=10, image=None, zb
         _i(order), C.cast(C.pointer(tfd), C.c_void_p) if tfd is not None else None,
         int(n_samples), _d(image), _d(zb) if zb is not None else None, int(num_threads),
         visits.ctypes.data_as(C.POINTER(C.c_int64)) if record else None,
-        hashes.ctypes.data_as(C.POINTER(C.c_uint64)) if record else None)
+        hashes.ctypes.data_as(C.POINTER(C.c_uint64)) if record else None,
+        samples.ctypes.data_as(C.POINTER(C.c_int64)) if record else None)
     if rc != 0:
         raise ValueError(f"orc_cast failed ({rc})")
+    if record and with_samples:
+        return image, visits, hashes, samples
     if record:
         return image, visits, hashes
     return image

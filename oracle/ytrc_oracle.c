@@ -361,6 +361,7 @@ typedef struct {
     int brick_id;
     int64_t visits;
     uint64_t hash;
+    int64_t samples;
 } orc_acc;
 
 static inline int orc_in_range(const orc_vc *vc, const int index[3])
@@ -501,10 +502,23 @@ static inline void orc_record(orc_acc *im, const int index[3], double enter_t, d
     im->hash = h;
     im->visits += 1;
 }
+/* Samples of the measurement: one per projection visit with exit_t >
+ * enter_t; n_samples per visit of an in-range cell for the samplers that
+ * sub-sample, VolumeRenderSampler only where the mask is 1
+ * (image_samplers.pyx:467-470). */
+static void orc_count_subsamples(const orc_vc *vc, const int index[3], orc_acc *im, int masked)
+{
+    if (!orc_in_range(vc, index)) return;
+    if (masked && vc->mask != NULL &&
+        vc->mask[((size_t)index[0] * vc->dims[1] + index[1]) * vc->dims[2] + index[2]] != 1)
+        return;
+    im->samples += im->vra->n_samples;
+}
 static void orc_projection_sample_rec(const orc_vc *vc, const double v_pos[3], const double v_dir[3],
                                       double enter_t, double exit_t, const int index[3], void *data)
 {
     orc_record((orc_acc *)data, index, enter_t, exit_t);
+    if (exit_t > enter_t) ((orc_acc *)data)->samples += 1;
     orc_projection_sample(vc, v_pos, v_dir, enter_t, exit_t, index, data);
 }
 static void orc_interpolated_projection_sample_rec(const orc_vc *vc, const double v_pos[3],
@@ -512,6 +526,7 @@ static void orc_interpolated_projection_sample_rec(const orc_vc *vc, const doubl
                                                    double exit_t, const int index[3], void *data)
 {
     orc_record((orc_acc *)data, index, enter_t, exit_t);
+    orc_count_subsamples(vc, index, (orc_acc *)data, 0);
     orc_interpolated_projection_sample(vc, v_pos, v_dir, enter_t, exit_t, index, data);
 }
 static void orc_light_source_sample_rec(const orc_vc *vc, const double v_pos[3],
@@ -519,6 +534,7 @@ static void orc_light_source_sample_rec(const orc_vc *vc, const double v_pos[3],
                                         const int index[3], void *data)
 {
     orc_record((orc_acc *)data, index, enter_t, exit_t);
+    orc_count_subsamples(vc, index, (orc_acc *)data, 0);
     orc_light_source_sample(vc, v_pos, v_dir, enter_t, exit_t, index, data);
 }
 static void orc_volume_render_sample_rec(const orc_vc *vc, const double v_pos[3],
@@ -526,6 +542,7 @@ static void orc_volume_render_sample_rec(const orc_vc *vc, const double v_pos[3]
                                          const int index[3], void *data)
 {
     orc_record((orc_acc *)data, index, enter_t, exit_t);
+    orc_count_subsamples(vc, index, (orc_acc *)data, 1);
     orc_volume_render_sample(vc, v_pos, v_dir, enter_t, exit_t, index, data);
 }
 
@@ -701,7 +718,8 @@ typedef struct {
  * reference's prange.  kind 0 = ProjectionSampler, 1 = VolumeRenderSampler.
  * image: float64 (nv0, nv1, 4), read-modify-write.  zbuffer: (nv0, nv1) or NULL
  * (NULL means ones, utils.py:136).  visits/hashes: per-pixel instrumentation
- * arrays or NULL.  Returns 0, or -1 on bad arguments.
+ * arrays or NULL; samples: per-pixel sample counts (orc_count_subsamples) or
+ * NULL, filled only with visits/hashes.  Returns 0, or -1 on bad arguments.
  */
 int orc_cast_lens(int kind, int lens, const double *pix_pos, const double *pix_dir,
                   const double *camera_data, const double *vp_pos, const double *vp_dir,
@@ -710,7 +728,8 @@ int orc_cast_lens(int kind, int lens, const double *pix_pos, const double *pix_d
                   const double *les, const double *res, const int32_t *dims, int n_fields,
                   const double *const *field_ptrs, const uint8_t *const *mask_ptrs, int n_order,
                   const int32_t *order, const orc_tf_desc *tf, int n_samples, double *image,
-                  const double *zbuffer, int num_threads, int64_t *visits, uint64_t *hashes)
+                  const double *zbuffer, int num_threads, int64_t *visits, uint64_t *hashes,
+                  int64_t *samples)
 {
     if (nv0 <= 0 || nv1 <= 0 || n_fields <= 0 || n_fields > 6) return -1;
     if (lens != 0 && (pix_pos == NULL || pix_dir == NULL)) return -1;
@@ -812,12 +831,14 @@ int orc_cast_lens(int kind, int lens, const double *pix_pos, const double *pix_d
             acc.brick_id = b;
             acc.visits = rec ? visits[pix] : 0;
             acc.hash = rec ? hashes[pix] : 0;
+            acc.samples = (rec && samples) ? samples[pix] : 0;
             double max_t = y_fclip(zbuffer ? zbuffer[pix] : 1.0, 0.0, 1.0);
             orc_walk_volume(&vc, v_pos, v_dir, fn, &acc, max_t);
             for (int c = 0; c < 4; c++) image[pix * 4 + c] = acc.rgba[c];
             if (rec) {
                 visits[pix] = acc.visits;
                 hashes[pix] = acc.hash;
+                if (samples) samples[pix] = acc.samples;
             }
         }
     }
@@ -835,11 +856,12 @@ int orc_cast(int kind, const double *vp_pos, const double *vp_dir, const double 
              const int32_t *dims, int n_fields, const double *const *field_ptrs,
              const uint8_t *const *mask_ptrs, int n_order, const int32_t *order,
              const orc_tf_desc *tf, int n_samples, double *image, const double *zbuffer,
-             int num_threads, int64_t *visits, uint64_t *hashes)
+             int num_threads, int64_t *visits, uint64_t *hashes, int64_t *samples)
 {
     return orc_cast_lens(kind, 0, NULL, NULL, NULL, vp_pos, vp_dir, center, bounds, x_vec, y_vec,
                          width, nv0, nv1, n_bricks, les, res, dims, n_fields, field_ptrs, mask_ptrs,
-                         n_order, order, tf, n_samples, image, zbuffer, num_threads, visits, hashes);
+                         n_order, order, tf, n_samples, image, zbuffer, num_threads, visits, hashes,
+                         samples);
 }
 
 /* Clipped perspective extents (lenses.pyx:60-161 + image_samplers.pyx:159-166).

@@ -1,5 +1,6 @@
 """GPU parity at the measured workloads' full sizes, against the FP64 oracle
-on the same inputs (bit-exact visit sequences; image rel-L2 <= 1e-5):
+on the same inputs (bit-exact visit sequences and per-pixel sample counts;
+image rel-L2 <= 1e-5):
 
   * 256^3 lognormal density + temperature, density-weighted projection of
     temperature, 1024^2;
@@ -31,11 +32,12 @@ def _one(ctx, spec):
         b.close()
 
 
-def _visits_equal(rec, ov, oh, tag):
-    v, h, _ = rec
-    nd, nh = int(np.sum(v != ov)), int(np.sum(h != oh))
-    print(f"{tag}: visits {int(v.sum())}, count-mismatch={nd} hash-mismatch={nh}", flush=True)
-    assert nd == 0 and nh == 0
+def _visits_equal(rec, ov, oh, os_, tag):
+    v, h, s = rec
+    nd, nh, ns = int(np.sum(v != ov)), int(np.sum(h != oh)), int(np.sum(s != os_))
+    print(f"{tag}: visits {int(v.sum())}, samples {int(s.sum())} (oracle {int(os_.sum())}), "
+          f"count-mismatch={nd} hash-mismatch={nh} sample-mismatch={ns}", flush=True)
+    assert nd == 0 and nh == 0 and ns == 0
 
 
 def test_weighted_projection_256_at_1024(ctx):
@@ -46,9 +48,10 @@ def test_weighted_projection_256_at_1024(ctx):
     try:
         p = c0_params(1024)
         img, rec = _one(ctx, vol.projection_request(p, record=True))
-        oimg, ov, oh = oracle.cast(0, p, oracle_proj_bricks(ds, ["temperature"], weight="density"),
-                                   record=True)
-        _visits_equal(rec, ov, oh, "256^3 weighted 1024^2")
+        oimg, ov, oh, os_ = oracle.cast(
+            0, p, oracle_proj_bricks(ds, ["temperature"], weight="density"), record=True,
+            with_samples=True)
+        _visits_equal(rec, ov, oh, os_, "256^3 weighted 1024^2")
         out = render.finalize_projection(img, p["width"], True)
         ref = render.finalize_projection(oimg, p["width"], True)
         e = rel_l2(out, ref)
@@ -68,9 +71,9 @@ def test_amr_vr_at_1024(ctx):
         p = c0_params(1024, normal=(0.3, 0.5, 0.81))
         spec = vol.vr_request(p, tf, record=True)
         img, rec = _one(ctx, spec)
-        oimg, ov, oh = oracle.cast(1, p, oracle_vr_bricks(vol), order=spec.order, tf=tf_dict(tf),
-                                   record=True)
-        _visits_equal(rec, ov, oh, f"AMR ({len(gd)} grids, {vol.n_vr_bricks} bricks) VR 1024^2")
+        oimg, ov, oh, os_ = oracle.cast(1, p, oracle_vr_bricks(vol), order=spec.order,
+                                        tf=tf_dict(tf), record=True, with_samples=True)
+        _visits_equal(rec, ov, oh, os_, f"AMR ({len(gd)} grids, {vol.n_vr_bricks} bricks) VR 1024^2")
         for c in range(3):
             e = rel_l2(img[..., c], oimg[..., c])
             print(f"AMR VR ch{c}: rel-L2={e:.3e}", flush=True)
@@ -88,17 +91,18 @@ def test_512_bricks_projection_and_vr(ctx):
     try:
         pp = c0_params(256, normal=(-0.62, 0.33, 0.71))
         img, rec = _one(ctx, vol.projection_request(pp, record=True))
-        oimg, ov, oh = oracle.cast(0, pp, oracle_proj_bricks(ds, ["density"]), record=True)
-        _visits_equal(rec, ov, oh, "512^3 projection 256^2")
+        oimg, ov, oh, os_ = oracle.cast(0, pp, oracle_proj_bricks(ds, ["density"]), record=True,
+                                        with_samples=True)
+        _visits_equal(rec, ov, oh, os_, "512^3 projection 256^2")
         e = rel_l2(img[..., 0], oimg[..., 0])
         print(f"512^3 projection: rel-L2={e:.3e}", flush=True)
         assert e <= TOL
         pv = c0_params(192, normal=(0.45, -0.8, 0.39))
         spec = vol.vr_request(pv, tf, record=True)
         img, rec = _one(ctx, spec)
-        oimg, ov, oh = oracle.cast(1, pv, oracle_vr_bricks(vol), order=spec.order, tf=tf_dict(tf),
-                                   record=True)
-        _visits_equal(rec, ov, oh, "512^3 VR 192^2")
+        oimg, ov, oh, os_ = oracle.cast(1, pv, oracle_vr_bricks(vol), order=spec.order,
+                                        tf=tf_dict(tf), record=True, with_samples=True)
+        _visits_equal(rec, ov, oh, os_, "512^3 VR 192^2")
         for c in range(3):
             e = rel_l2(img[..., c], oimg[..., c])
             print(f"512^3 VR ch{c}: rel-L2={e:.3e}", flush=True)

@@ -51,9 +51,10 @@ def test_uniform_grids(ctx, n_samples):
         p = c0_params(72, normal=(0.25, 0.6, -0.76))
         spec = vol.interpolated_projection_request(p, n_samples=n_samples, record=True)
         img, rec = _one(ctx, spec)
-        oimg, ov, oh = oracle.cast(2, p, oracle_vr_bricks(vol), order=spec.order,
-                                   n_samples=n_samples, record=True)
+        oimg, ov, oh, os_ = oracle.cast(2, p, oracle_vr_bricks(vol), order=spec.order,
+                                        n_samples=n_samples, record=True, with_samples=True)
         _check(img, rec, oimg, ov, oh, f"iproj uniform n={n_samples}")
+        assert np.array_equal(rec[2], os_)
     finally:
         vol.free()
 

@@ -48,12 +48,13 @@ def test_light_source_uniform(ctx, grey):
         b = _lib.Batch(ctx, [spec], _lib.BATCH_PERSISTENT)
         try:
             b.run()
-            img, (v, h, _) = b.read_image(0), b.read_record(0)
+            img, (v, h, sm) = b.read_image(0), b.read_record(0)
         finally:
             b.close()
-        oimg, ov, oh = oracle.cast(3, p, oracle_vr_bricks(vol), order=spec.order,
-                                   tf=_td(tf, spec), n_samples=6, record=True)
-        assert np.array_equal(v, ov) and np.array_equal(h, oh)
+        oimg, ov, oh, os_ = oracle.cast(3, p, oracle_vr_bricks(vol), order=spec.order,
+                                        tf=_td(tf, spec), n_samples=6, record=True,
+                                        with_samples=True)
+        assert np.array_equal(v, ov) and np.array_equal(h, oh) and np.array_equal(sm, os_)
         for c in range(3):
             _close(img[..., c], oimg[..., c], f"light grey={grey} ch{c}")
     finally:

@@ -45,13 +45,14 @@ def _run(ctx, specs, flags=_lib.BATCH_PLAIN):
         b.close()
 
 
-def _check_visits(rec, ovis, ohash, tag):
-    v, h, _ = rec
+def _check_visits(rec, ovis, ohash, tag, osamp=None):
+    v, h, s = rec
     nd = int(np.sum(v != ovis))
     nh = int(np.sum(h != ohash))
+    ns = 0 if osamp is None else int(np.sum(s != osamp))
     print(f"{tag}: total visits gpu={int(v.sum())} oracle={int(ovis.sum())} "
-          f"count-mismatch={nd} hash-mismatch={nh}", flush=True)
-    assert nd == 0 and nh == 0
+          f"count-mismatch={nd} hash-mismatch={nh} sample-mismatch={ns}", flush=True)
+    assert nd == 0 and nh == 0 and ns == 0
 
 
 @pytest.mark.parametrize("res", [64, 256])
@@ -62,8 +63,9 @@ def test_c0_projection(ctx, res):
     try:
         p = c0_params(res)
         (img, rec), = _run(ctx, [vol.projection_request(p, record=True)])
-        oimg, ovis, ohash = oracle.cast(0, p, oracle_proj_bricks(ds, ["density"]), record=True)
-        _check_visits(rec, ovis, ohash, f"c0 res={res}")
+        oimg, ovis, ohash, osamp = oracle.cast(0, p, oracle_proj_bricks(ds, ["density"]),
+                                               record=True, with_samples=True)
+        _check_visits(rec, ovis, ohash, f"c0 res={res}", osamp)
         e = rel_l2(img[..., 0], oimg[..., 0])
         m = np.abs(img[..., 0] - oimg[..., 0]).max() / np.abs(oimg[..., 0]).max()
         print(f"c0 res={res}: rel-L2={e:.3e} max-abs/max={m:.3e}", flush=True)
@@ -122,9 +124,9 @@ def test_vr_amr(ctx):
         spec = vol.vr_request(p, tf, n_samples=10, record=True)
         (img, rec), = _run(ctx, [spec])
         bricks = oracle_vr_bricks(vol)
-        oimg, ovis, ohash = oracle.cast(1, p, bricks, order=spec.order, tf=tf_dict(tf),
-                                        n_samples=10, record=True)
-        _check_visits(rec, ovis, ohash, f"vr amr ({len(bricks)} bricks)")
+        oimg, ovis, ohash, osamp = oracle.cast(1, p, bricks, order=spec.order, tf=tf_dict(tf),
+                                               n_samples=10, record=True, with_samples=True)
+        _check_visits(rec, ovis, ohash, f"vr amr ({len(bricks)} bricks)", osamp)
         for c in range(3):
             e = rel_l2(img[..., c], oimg[..., c])
             print(f"vr ch{c}: rel-L2={e:.3e} max={oimg[..., c].max():.3e}", flush=True)
@@ -144,9 +146,10 @@ def test_vr_grey_opacity(ctx):
         p = c0_params(64, normal=(0.6, 0.3, -0.2))
         spec = vol.vr_request(p, tf, n_samples=7, record=True)
         (img, rec), = _run(ctx, [spec])
-        oimg, ovis, ohash = oracle.cast(1, p, oracle_vr_bricks(vol), order=spec.order,
-                                        tf=tf_dict(tf), n_samples=7, record=True)
-        _check_visits(rec, ovis, ohash, "vr grey")
+        oimg, ovis, ohash, osamp = oracle.cast(1, p, oracle_vr_bricks(vol), order=spec.order,
+                                               tf=tf_dict(tf), n_samples=7, record=True,
+                                               with_samples=True)
+        _check_visits(rec, ovis, ohash, "vr grey", osamp)
         for c in range(4):
             assert rel_l2(img[..., c], oimg[..., c]) <= VR_TOL
     finally:
@@ -354,3 +357,41 @@ def test_autotune_picks_a_working_shape(ctx, tmp_path):
             b.close()
     finally:
         vol.free()
+
+
+def test_vr_child_masks_below_max_level(ctx):
+    """AMRKDTree(ds, max_level=...) leaves finer grids out, so its bricks'
+    child masks are not all ones (amr_kdtree.py:346-367): VolumeRenderSampler
+    skips mask-0 cells (image_samplers.pyx:467-470).  Visits, per-pixel
+    sample counts and the image against the oracle."""
+    from yt_amd.amr_kdtree import AMRKDTree
+
+    ds = small_amr(root_n=32, levels=2, seed=61)
+    assert ds.max_level == 2
+    kd = AMRKDTree(ds, max_level=1)
+    kd.set_fields(["density"], [True], no_ghost=True)
+    leaves = kd.leaves()
+    pgs = [kd.get_brick_data(n) for n in leaves]
+    assert any(not pg.source_mask.all() for pg in pgs)
+    tf = vr_tf_for(ds, nbins=128)
+    ds_id = ctx.upload([(pg.LeftEdge, pg.RightEdge, pg.dims, True) for pg in pgs],
+                       [pg.my_data for pg in pgs], [pg.source_mask for pg in pgs])
+    try:
+        p = c0_params(96, normal=(-0.5, 0.4, 0.77))
+        order = kd.order(p["viewpoint"])
+        spec = _lib.RequestSpec(_lib.VOLUME_RENDER, p, ds_id, order, n_samples=10,
+                                tf=_lib.TF.from_tf_obj(tf), record=True)
+        (img, rec), = _run(ctx, [spec], _lib.BATCH_PERSISTENT)
+        bricks = [dict(le=pg.LeftEdge, re=pg.RightEdge, dims=pg.dims,
+                       data=[np.asarray(d, dtype=np.float64) for d in pg.my_data],
+                       mask=pg.source_mask) for pg in pgs]
+        oimg, ovis, ohash, osamp = oracle.cast(1, p, bricks, order=order, tf=tf_dict(tf),
+                                               n_samples=10, record=True, with_samples=True)
+        _check_visits(rec, ovis, ohash, "vr masked bricks", osamp)
+        assert np.any(osamp < 10 * ovis)
+        for c in range(3):
+            e = rel_l2(img[..., c], oimg[..., c])
+            print(f"vr masked ch{c}: rel-L2={e:.3e}", flush=True)
+            assert e <= VR_TOL
+    finally:
+        ctx.free(ds_id)

@@ -306,3 +306,50 @@ def test_light_source_closed_form(field):
                     ta = max(1.0 - dt * e[c], 0.0)
                     acc = (1.0 - ta) * e[c] * (1.0 + dot * lrgba[c]) + ta * acc
                 assert abs(img[vi, vj, c] - acc) <= 1e-11, (vi, vj, c)
+
+
+def test_oracle_sample_counts():
+    """The oracle's per-pixel sample counts against its visit counts: the
+    interpolated projection takes n_samples at every visit; the volume
+    render n_samples at every visit of a mask-1 cell, so with all-ones masks
+    it equals visits x n_samples and with child masks it is less where rays
+    cross covered cells; the projection counts visits with exit_t > enter_t,
+    all of them for these rays (no exact multi-axis ties)."""
+    import numpy as np
+
+    from oracle import oracle
+    from tests.helpers import c0_params, small_amr, tf_dict, vr_tf_for
+    from yt_amd.amr_kdtree import AMRKDTree
+
+    ds = small_amr(root_n=16, levels=2, seed=41)
+    kd = AMRKDTree(ds)
+    kd.set_fields(["density"], [True], no_ghost=True)
+    bricks = []
+    for n in kd.leaves():
+        pg = kd.get_brick_data(n)
+        bricks.append(dict(le=pg.LeftEdge, re=pg.RightEdge, dims=pg.dims,
+                           data=list(pg.my_data), mask=pg.source_mask))
+    # kd leaves lie inside their finest grid, so their child masks are all
+    # ones: mask out a slab of every brick to exercise the mask test
+    unmasked = [dict(b, mask=None) for b in bricks]
+    for b in bricks:
+        m = np.ones(tuple(int(x) for x in b["dims"]), dtype=np.uint8)
+        m[: max(1, m.shape[0] // 2)] = 0
+        b["mask"] = m
+    p = c0_params(24, normal=(0.3, 0.6, -0.7))
+    order = kd.order(p["viewpoint"])
+    tf = tf_dict(vr_tf_for(ds, nbins=32))
+
+    def counts(kind, bs, ns):
+        _, v, _, s = oracle.cast(kind, p, bs, order=order, tf=tf if kind == 1 else None,
+                                 n_samples=ns, record=True, with_samples=True)
+        return v, s
+
+    v, s = counts(2, bricks, 3)
+    assert v.sum() > 0 and np.array_equal(s, 3 * v)
+    v, s = counts(1, unmasked, 7)
+    assert np.array_equal(s, 7 * v)
+    v, s = counts(1, bricks, 7)
+    assert np.all(s % 7 == 0) and np.all(s <= 7 * v) and np.any(s < 7 * v)
+    v, s = counts(0, unmasked, 1)
+    assert np.array_equal(s, v)
