@@ -348,6 +348,36 @@ def main():
         plain = dict(images_per_s=len(cams) / pw, ms_per_step=pw * 1e3)
         log(f"plain one-call-per-image: {pw * 1e3:.2f} ms for {len(cams)} images "
             f"({len(cams) / pw:.1f} images/s); batched speed-up {imgs_s / plain['images_per_s']:.2f}x")
+        # The same comparison per image side: the cameras of one side as a
+        # batch (persistent launch in a hipGraph) and one call per image.
+        by_side = {}
+        for side in sorted({int(p["nv"][0]) for _, p in cams}):
+            idx = [i for i in range(len(cams)) if int(cams[i][1]["nv"][0]) == side]
+            sub = [specs[i] for i in idx]
+            bb = autotune.make_batch(ctx, sub, cfg, flags)
+            bb.run()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                bb.launch()
+            bb.sync()
+            tb = (time.perf_counter() - t0) / args.steps
+            bb.close()
+            singles = [_lib.Batch(ctx, [s], _lib.BATCH_PLAIN) for s in sub]
+            for b in singles:
+                b.run()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                for b in singles:
+                    b.launch()
+                    b.sync()
+            tp = (time.perf_counter() - t0) / args.steps
+            for b in singles:
+                b.close()
+            by_side[str(side)] = dict(images=len(sub), batched_images_per_s=round(len(sub) / tb, 1),
+                                      plain_images_per_s=round(len(sub) / tp, 1),
+                                      speedup=round(tp / tb, 2))
+            log(f"side {side}: {by_side[str(side)]}")
+        plain["by_side"] = by_side
 
     # Latency of single 1024^2 requests (launch -> sync on the host clock).
     lat = {}
@@ -438,6 +468,7 @@ def main():
         "fp64_dda_ceiling_ms": round(fp64_ms, 3),
         "bricks_64": alt,
         "plain_images_per_s": None if plain is None else round(plain["images_per_s"], 2),
+        "batched_vs_plain_by_side": None if plain is None else plain["by_side"],
         "latency_ms": lat,
         "serving": serving,
         "roofline": {

@@ -76,7 +76,9 @@ def test_amr_vr_at_1024(ctx):
         _visits_equal(rec, ov, oh, os_, f"AMR ({len(gd)} grids, {vol.n_vr_bricks} bricks) VR 1024^2")
         for c in range(3):
             e = rel_l2(img[..., c], oimg[..., c])
-            print(f"AMR VR ch{c}: rel-L2={e:.3e}", flush=True)
+            m = float(np.max(np.abs(img[..., c] - oimg[..., c])))
+            print(f"AMR VR ch{c}: rel-L2={e:.3e} max-abs={m:.3e} (max {oimg[..., c].max():.3f})",
+                  flush=True)
             assert e <= TOL
     finally:
         vol.free()
@@ -95,7 +97,8 @@ def test_512_bricks_projection_and_vr(ctx):
                                         with_samples=True)
         _visits_equal(rec, ov, oh, os_, "512^3 projection 256^2")
         e = rel_l2(img[..., 0], oimg[..., 0])
-        print(f"512^3 projection: rel-L2={e:.3e}", flush=True)
+        m = float(np.max(np.abs(img[..., 0] - oimg[..., 0])) / np.max(np.abs(oimg[..., 0])))
+        print(f"512^3 projection: rel-L2={e:.3e} max-abs/max={m:.3e}", flush=True)
         assert e <= TOL
         pv = c0_params(192, normal=(0.45, -0.8, 0.39))
         spec = vol.vr_request(pv, tf, record=True)
@@ -105,7 +108,9 @@ def test_512_bricks_projection_and_vr(ctx):
         _visits_equal(rec, ov, oh, os_, "512^3 VR 192^2")
         for c in range(3):
             e = rel_l2(img[..., c], oimg[..., c])
-            print(f"512^3 VR ch{c}: rel-L2={e:.3e}", flush=True)
+            m = float(np.max(np.abs(img[..., c] - oimg[..., c])))
+            print(f"512^3 VR ch{c}: rel-L2={e:.3e} max-abs={m:.3e} (max {oimg[..., c].max():.3f})",
+                  flush=True)
             assert e <= TOL
     finally:
         vol.free()

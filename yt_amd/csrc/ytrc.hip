@@ -55,8 +55,9 @@ struct VertexJob {
     int gdims[3];  // the grid's cells per axis
     int li[3];     // the leaf's first cell in the grid
     int vd[3];     // the leaf's vertices per axis (dims + 1)
+    int nb[3];     // the leaf's vertex blocks per axis
     int nf, nfp, logmask;
-    int64_t n;     // vertices in the leaf
+    int64_t n;     // stored points of the leaf (whole vertex blocks)
     float *out;    // the leaf's points in the pool (nfp floats per point)
 };
 
@@ -74,18 +75,31 @@ __device__ __forceinline__ void vc_axis(int o, int n, int &ii, double &xp, doubl
     xm = 1.0 - xp;
 }
 
-// One thread per leaf vertex: the 8-cell weighted sum in FP64 in the
-// reference's term order, then log10 with v <= 0 -> NaN where the field is a
-// log field (amr_kdtree.py:346-352), stored as float32.
+// One thread per stored point of the leaf (vertex blocks, so a vertex on a
+// block face is computed once per block that stores it): the 8-cell
+// weighted sum in FP64 in the reference's term order, then log10 with
+// v <= 0 -> NaN where the field is a log field (amr_kdtree.py:346-352),
+// stored as float32.  Padding points and points past the leaf's last
+// vertex are stored as 0 (never read).
 template <class T>
 __global__ __launch_bounds__(256) void k_vertex_centre(VertexJob J, const T *__restrict__ cells, int64_t nc)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= J.n) return;
-    const int c = (int)(p % J.vd[2]);
-    const int64_t q = p / J.vd[2];
-    const int b = (int)(q % J.vd[1]);
-    const int a = (int)(q / J.vd[1]);
+    float *o = J.out + p * J.nfp;
+    const int local = (int)(p & (VBP - 1));
+    const int64_t blk = p >> VBP_SHIFT;
+    const int bk = (int)(blk % J.nb[2]);
+    const int64_t bq = blk / J.nb[2];
+    const int bj = (int)(bq % J.nb[1]);
+    const int bi = (int)(bq / J.nb[1]);
+    const int a = bi * VB + local / (VB1 * VB1);
+    const int b = bj * VB + (local / VB1) % VB1;
+    const int c = bk * VB + local % VB1;
+    if (local >= VB1 * VB1 * VB1 || a >= J.vd[0] || b >= J.vd[1] || c >= J.vd[2]) {
+        for (int f = 0; f < J.nfp; f++) o[f] = 0.0f;
+        return;
+    }
     int ii, ij, ik;
     double xp, xm, yp, ym, zp, zm;
     vc_axis(J.li[0] + a, J.gdims[0], ii, xp, xm);
@@ -93,7 +107,6 @@ __global__ __launch_bounds__(256) void k_vertex_centre(VertexJob J, const T *__r
     vc_axis(J.li[2] + c, J.gdims[2], ik, zp, zm);
     const int64_t s1 = J.gdims[2];
     const int64_t s0 = (int64_t)J.gdims[1] * J.gdims[2];
-    float *o = J.out + p * J.nfp;
     for (int f = 0; f < J.nfp; f++) {
         if (f >= J.nf) {
             o[f] = 0.0f;
@@ -563,24 +576,40 @@ int ytrc_dataset_upload(ytrc_ctx *ctx, int32_t n_bricks, const ytrc_brick_desc *
         const int64_t np = (int64_t)(h.dims[0] + h.vertex) * (h.dims[1] + h.vertex) *
                            (h.dims[2] + h.vertex);
         stage.assign((size_t)stored_points(h) * ds.nfp, 0.0f);
-        // Destination point of each source point: C order for vertex-centred
-        // bricks, 4x4x4 blocks for cell-centred ones.
-        std::vector<int64_t> dst;
+        // (stored point, source point) pairs: 4x4x4 cell blocks for
+        // cell-centred bricks; vertex blocks for vertex-centred ones, where a
+        // vertex on a block face is stored in every block that holds it.
+        std::vector<std::pair<int64_t, int64_t>> map;
+        map.reserve((size_t)np);
         if (!h.vertex) {
-            dst.resize((size_t)np);
             const int nbj = (h.dims[1] + 3) >> 2, nbk = (h.dims[2] + 3) >> 2;
             int64_t p = 0;
             for (int i = 0; i < h.dims[0]; i++)
                 for (int j = 0; j < h.dims[1]; j++)
                     for (int k = 0; k < h.dims[2]; k++)
-                        dst[(size_t)p++] = blocked_offset(i, j, k, nbj, nbk);
+                        map.push_back({blocked_offset(i, j, k, nbj, nbk), p++});
+        } else {
+            const int nb0 = vblocks(h.dims[0]), nb1 = vblocks(h.dims[1]), nb2 = vblocks(h.dims[2]);
+            const int64_t s1 = h.dims[2] + 1, s0 = (int64_t)(h.dims[1] + 1) * s1;
+            for (int bi = 0; bi < nb0; bi++)
+                for (int bj = 0; bj < nb1; bj++)
+                    for (int bk = 0; bk < nb2; bk++) {
+                        const int64_t base = (((int64_t)bi * nb1 + bj) * nb2 + bk) << VBP_SHIFT;
+                        for (int a = 0; a < VB1; a++)
+                            for (int c1 = 0; c1 < VB1; c1++)
+                                for (int c2 = 0; c2 < VB1; c2++) {
+                                    const int vi = bi * VB + a, vj = bj * VB + c1, vk = bk * VB + c2;
+                                    if (vi > h.dims[0] || vj > h.dims[1] || vk > h.dims[2]) continue;
+                                    map.push_back({base + vcorner(a, c1, c2), vi * s0 + vj * s1 + vk});
+                                }
+                    }
         }
         for (int f = 0; f < n_fields; f++) {
             const void *src = field_ptrs[(size_t)b * n_fields + f];
-            for (int64_t p = 0; p < np; p++) {
-                const float v = (src_dtype == 0) ? (float)((const double *)src)[p] : ((const float *)src)[p];
-                const int64_t q = h.vertex ? p : dst[(size_t)p];
-                stage[(size_t)q * ds.nfp + f] = v;
+            for (const auto &m : map) {
+                const float v = (src_dtype == 0) ? (float)((const double *)src)[m.second]
+                                                 : ((const float *)src)[m.second];
+                stage[(size_t)m.first * ds.nfp + f] = v;
             }
         }
         const hipError_t e = hipMemcpy(ds.d_pool + h.data_off, stage.data(), stage.size() * 4,
@@ -674,11 +703,12 @@ int ytrc_dataset_build_vr(ytrc_ctx *ctx, int32_t n_grids, const ytrc_grid_desc *
             J.gdims[a] = grids[L.grid].dims[a];
             J.li[a] = L.li[a];
             J.vd[a] = h.dims[a] + 1;
+            J.nb[a] = vblocks(h.dims[a]);
         }
         J.nf = n_fields;
         J.nfp = ds.nfp;
         J.logmask = logmask;
-        J.n = (int64_t)J.vd[0] * J.vd[1] * J.vd[2];
+        J.n = stored_points(h);
         J.out = ds.d_pool + h.data_off;
         const int64_t nc = (int64_t)J.gdims[0] * J.gdims[1] * J.gdims[2];
         const unsigned grid = (unsigned)((J.n + 255) / 256);

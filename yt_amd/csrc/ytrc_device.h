@@ -84,16 +84,49 @@ struct alignas(16) DImg {
 // cells: blocks in C order over the brick padded to whole blocks, cells in C
 // order inside a block (256 B per block and field).  A ray bundle walking
 // through a block touches two 128-B lines instead of up to 16 rows of the C
-// order.  Vertex-centred bricks stay in C order.
+// order.
 __host__ __device__ inline int64_t blocked_offset(int i, int j, int k, int nbj, int nbk)
 {
     const int64_t blk = ((int64_t)(i >> 2) * nbj + (j >> 2)) * nbk + (k >> 2);
     return (blk << 6) | ((i & 3) << 4) | ((j & 3) << 2) | (k & 3);
 }
 
+// Vertex-centred bricks (the volume-render, interpolated-projection and
+// light-source paths) are stored in blocks of VB x VB x VB cells.  Each
+// block holds the (VB+1)^3 vertices its cells read -- vertices on a face
+// shared by two blocks are stored in both -- padded to VBP points; blocks in
+// C order over the brick padded to whole blocks, vertices in C order inside
+// a block.  A cell's 8 corners then lie in one block at fixed offsets
+// (vcorner), so a visit addresses them from one base, and a ray crossing a
+// block reads its four 128-B lines instead of two new C-order rows per step
+// across the rows.  Blocks of 4^3 cells (125 vertices in 512 B) store 1.95x
+// the vertices; against the C order they cut the volume render's time at
+// 512^3 by 20 % (by half where rays are sparser than cells), and beat 2^3
+// blocks (27 vertices in 128 B).
+constexpr int VB_SHIFT = 2;                 // log2 of the cells per block edge
+constexpr int VB = 1 << VB_SHIFT;
+constexpr int VB1 = VB + 1;                 // vertices per block edge
+constexpr int VBP_SHIFT = VB == 2 ? 5 : 7;  // 27 -> 32, 125 -> 128 points
+constexpr int VBP = 1 << VBP_SHIFT;
+static_assert(VB1 * VB1 * VB1 <= VBP, "vertex block padding");
+
+__host__ __device__ inline int vblocks(int cells) { return (cells + VB - 1) >> VB_SHIFT; }
+
+// Point offset of corner (0, 0, 0) of cell (i, j, k).
+__host__ __device__ inline int64_t vertex_block_offset(int i, int j, int k, int nbj, int nbk)
+{
+    const int64_t blk = ((int64_t)(i >> VB_SHIFT) * nbj + (j >> VB_SHIFT)) * nbk + (k >> VB_SHIFT);
+    return (blk << VBP_SHIFT) + ((i & (VB - 1)) * VB1 + (j & (VB - 1))) * VB1 + (k & (VB - 1));
+}
+
+// Offset of the block's vertex (a, b, c), 0 <= a, b, c <= VB, from its
+// vertex (0, 0, 0); for a, b, c in {0, 1}, the cell corner offsets.
+__host__ __device__ constexpr int vcorner(int a, int b, int c) { return (a * VB1 + b) * VB1 + c; }
+
 inline int64_t stored_points(const DBrick &h)
 {
-    if (h.vertex) return (int64_t)(h.dims[0] + 1) * (h.dims[1] + 1) * (h.dims[2] + 1);
+    if (h.vertex)
+        return (int64_t)vblocks(h.dims[0]) * vblocks(h.dims[1]) * vblocks(h.dims[2]) * VBP;
     return (int64_t)((h.dims[0] + 3) >> 2) * ((h.dims[1] + 3) >> 2) * ((h.dims[2] + 3) >> 2) * 64;
 }
 

@@ -303,6 +303,7 @@ struct BrickReg {
     double dds[3];
     double idds[3];
     int d0, d1, d2;
+    int vnbj, vnbk;  // vertex blocks along axes 1 and 2
 };
 
 __device__ __forceinline__ BrickReg brick_reg(const DBrick &b)
@@ -317,7 +318,38 @@ __device__ __forceinline__ BrickReg brick_reg(const DBrick &b)
     r.d0 = uni(b.dims[0]);
     r.d1 = uni(b.dims[1]);
     r.d2 = uni(b.dims[2]);
+    r.vnbj = vblocks(r.d1);
+    r.vnbk = vblocks(r.d2);
     return r;
+}
+
+// Offset (in points) of corner (0, 0, 0) of cell (c0, c1, c2) of a vertex
+// brick; the 24-bit multiplies are exact because a brick has fewer than
+// 2^24 vertex blocks per face (checked at upload).
+__device__ __forceinline__ int vertex_base(const BrickReg &b, int c0, int c1, int c2)
+{
+    const unsigned blk = __umul24(__umul24((unsigned)c0 >> VB_SHIFT, (unsigned)b.vnbj) +
+                                      ((unsigned)c1 >> VB_SHIFT),
+                                  (unsigned)b.vnbk) +
+                         ((unsigned)c2 >> VB_SHIFT);
+    return (int)((blk << VBP_SHIFT) +
+                 __umul24(__umul24((unsigned)c0 & (VB - 1), VB1) + ((unsigned)c1 & (VB - 1)), VB1) +
+                 ((unsigned)c2 & (VB - 1)));
+}
+
+// The 8 corner values of a cell, fields interleaved, from its corner
+// (0, 0, 0) at p0: cv[4a + 2b + c] is corner (a, b, c).
+template <int NFP>
+__device__ __forceinline__ void load_corners(const float *p0, float (&cv)[8][NFP])
+{
+    FieldLoad<NFP>::get(p0, cv[0]);
+    FieldLoad<NFP>::get(p0 + vcorner(0, 0, 1) * NFP, cv[1]);
+    FieldLoad<NFP>::get(p0 + vcorner(0, 1, 0) * NFP, cv[2]);
+    FieldLoad<NFP>::get(p0 + vcorner(0, 1, 1) * NFP, cv[3]);
+    FieldLoad<NFP>::get(p0 + vcorner(1, 0, 0) * NFP, cv[4]);
+    FieldLoad<NFP>::get(p0 + vcorner(1, 0, 1) * NFP, cv[5]);
+    FieldLoad<NFP>::get(p0 + vcorner(1, 1, 0) * NFP, cv[6]);
+    FieldLoad<NFP>::get(p0 + vcorner(1, 1, 1) * NFP, cv[7]);
 }
 
 // offset_interpolate (fixed_interpolator.cpp:34-52) restricted to the
@@ -496,9 +528,7 @@ struct VRSampler {
               (unsigned)c2 < (unsigned)d2))
             return;
         const int cell = (int)__umul24(__umul24(c0, d1) + c1, d2) + c2;
-        const int s1 = d2 + 1;
-        const int s0 = (d1 + 1) * (d2 + 1);
-        const int voff = (int)__umul24(c0, s0) + (int)__umul24(c1, s1) + c2;
+        const int voff = vertex_base(b, c0, c1, c2);
         // The mask byte and the corner values are independent loads: issue
         // both before testing the mask.
         const int mval = (mask != nullptr) ? (int)mask[cell] : 1;
@@ -519,17 +549,7 @@ struct VRSampler {
             ds[i] = paf[i] * dtf;
         }
         float cv[8][NFP];
-        {
-            const float *p0 = data + (size_t)voff * NFP;
-            FieldLoad<NFP>::get(p0, cv[0]);
-            FieldLoad<NFP>::get(p0 + NFP, cv[1]);
-            FieldLoad<NFP>::get(p0 + (size_t)s1 * NFP, cv[2]);
-            FieldLoad<NFP>::get(p0 + (size_t)(s1 + 1) * NFP, cv[3]);
-            FieldLoad<NFP>::get(p0 + (size_t)s0 * NFP, cv[4]);
-            FieldLoad<NFP>::get(p0 + (size_t)(s0 + 1) * NFP, cv[5]);
-            FieldLoad<NFP>::get(p0 + (size_t)(s0 + s1) * NFP, cv[6]);
-            FieldLoad<NFP>::get(p0 + (size_t)(s0 + s1 + 1) * NFP, cv[7]);
-        }
+        load_corners<NFP>(data + (size_t)voff * NFP, cv);
         if (mval != 1) return;
         if (REC) rec->samples += n_samples;
         float k[NFP][4];
@@ -583,9 +603,7 @@ struct IProjSampler {
               (unsigned)c2 < (unsigned)d2))
             return;
         if (REC) rec->samples += n_samples;
-        const int s1i = d2 + 1;
-        const int s0i = (d1 + 1) * (d2 + 1);
-        const int voff = (int)__umul24(c0, s0i) + (int)__umul24(c1, s1i) + c2;
+        const int voff = vertex_base(b, c0, c1, c2);
         const double dt = (ex - en) * inv_ns;
         const double tt = en + 0.5 * dt;
         float dp[3], ds[3];
@@ -596,15 +614,7 @@ struct IProjSampler {
             ds[i] = (float)(pa[i] * dt);
         }
         float cv[8][NFP];
-        const float *p0 = data + (size_t)voff * NFP;
-        FieldLoad<NFP>::get(p0, cv[0]);
-        FieldLoad<NFP>::get(p0 + NFP, cv[1]);
-        FieldLoad<NFP>::get(p0 + (size_t)s1i * NFP, cv[2]);
-        FieldLoad<NFP>::get(p0 + (size_t)(s1i + 1) * NFP, cv[3]);
-        FieldLoad<NFP>::get(p0 + (size_t)s0i * NFP, cv[4]);
-        FieldLoad<NFP>::get(p0 + (size_t)(s0i + 1) * NFP, cv[5]);
-        FieldLoad<NFP>::get(p0 + (size_t)(s0i + s1i) * NFP, cv[6]);
-        FieldLoad<NFP>::get(p0 + (size_t)(s0i + s1i + 1) * NFP, cv[7]);
+        load_corners<NFP>(data + (size_t)voff * NFP, cv);
         const double ns = (double)n_samples;
 #pragma unroll
         for (int f = 0; f < (NFP < 3 ? NFP : 3); f++) {
@@ -654,9 +664,7 @@ struct LightSampler {
               (unsigned)c2 < (unsigned)d2))
             return;
         if (REC) rec->samples += n_samples;
-        const int s1i = d2 + 1;
-        const int s0i = (d1 + 1) * (d2 + 1);
-        const int voff = (int)__umul24(c0, s0i) + (int)__umul24(c1, s1i) + c2;
+        const int voff = vertex_base(b, c0, c1, c2);
         const double dt = (ex - en) * inv_ns;
         const double tt = en + 0.5 * dt;
         float dp[3], ds[3];
@@ -667,15 +675,7 @@ struct LightSampler {
             ds[i] = (float)(pa[i] * dt);
         }
         float cv[8][NFP];
-        const float *p0 = data + (size_t)voff * NFP;
-        FieldLoad<NFP>::get(p0, cv[0]);
-        FieldLoad<NFP>::get(p0 + NFP, cv[1]);
-        FieldLoad<NFP>::get(p0 + (size_t)s1i * NFP, cv[2]);
-        FieldLoad<NFP>::get(p0 + (size_t)(s1i + 1) * NFP, cv[3]);
-        FieldLoad<NFP>::get(p0 + (size_t)s0i * NFP, cv[4]);
-        FieldLoad<NFP>::get(p0 + (size_t)(s0i + 1) * NFP, cv[5]);
-        FieldLoad<NFP>::get(p0 + (size_t)(s0i + s1i) * NFP, cv[6]);
-        FieldLoad<NFP>::get(p0 + (size_t)(s0i + s1i + 1) * NFP, cv[7]);
+        load_corners<NFP>(data + (size_t)voff * NFP, cv);
         float k[NFP][4];
 #pragma unroll
         for (int f = 0; f < NFP; f++) {

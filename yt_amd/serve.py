@@ -95,13 +95,18 @@ def run_poisson(server, specs, rate, seed=1005, readback="planar", idle_sleep=2e
 
 
 def summarize(res, specs):
+    """Latency percentiles over all requests and over those of side <= 1024
+    (the 1024^2-and-smaller requests), images/s and admission waits."""
     lat = res["latency_s"] * 1e3
     span = res["wall_s"] - res["first_arrival"]
+    sides = np.array([max(s.nv) if hasattr(s, "nv") else 0 for s in specs])
+    small = lat[sides <= 1024]
     return dict(
         n=len(specs),
         p50_ms=float(np.percentile(lat, 50)),
         p99_ms=float(np.percentile(lat, 99)),
         mean_ms=float(np.mean(lat)),
+        p99_ms_side_le_1024=float(np.percentile(small, 99)) if small.size else None,
         images_per_s=len(specs) / span,
         admission_waits=int(res["admission_waits"]),
     )
