@@ -103,7 +103,9 @@ typedef struct {
  * (0 = float64, 1 = float32) with dims or dims+1 points per axis.
  * mask_ptrs: NULL, or n_bricks pointers to dims-shaped uint8 masks (a NULL
  * entry means all ones).  Device data is float32, fields interleaved per
- * point.  Replaces the per-call numpy views of PartitionedGrid. */
+ * point; cell-centred bricks in blocks of 4x4x4 cells, vertex-centred ones
+ * in blocks of the 5x5x5 vertices of 4x4x4 cells (face vertices stored in
+ * both blocks).  Replaces the per-call numpy views of PartitionedGrid. */
 int ytrc_dataset_upload(ytrc_ctx *ctx, int32_t n_bricks, const ytrc_brick_desc *bricks,
                         int32_t n_fields, int32_t src_dtype, const void *const *field_ptrs,
                         const uint8_t *const *mask_ptrs, int32_t *ds_id_out);
