@@ -1215,6 +1215,10 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
             hipFree(B->arena);
             ctx->used -= B->arena_bytes;
         }
+        if (B->ev0) hipEventDestroy(B->ev0);
+        if (B->ev1) hipEventDestroy(B->ev1);
+        for (auto &ev : B->ev_req)
+            if (ev) hipEventDestroy(ev);
         delete B;
         return fail(code, m);
     };
@@ -1302,11 +1306,13 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
     B->d_imgs = (DImg *)(A + imgs_off);
     e = hipMemcpy(B->d_imgs, B->h_imgs.data(), sizeof(DImg) * n_req, hipMemcpyHostToDevice);
     if (e != hipSuccess) return bad(YTRC_EDEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
-    HIPCHK(hipEventCreate(&B->ev0));
-    HIPCHK(hipEventCreate(&B->ev1));
+    if ((e = hipEventCreate(&B->ev0)) != hipSuccess || (e = hipEventCreate(&B->ev1)) != hipSuccess)
+        return bad(YTRC_EDEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e));
     if (!(batch_flags & YTRC_BATCH_PERSISTENT)) {
-        B->ev_req.resize(2 * n_req);
-        for (auto &ev : B->ev_req) HIPCHK(hipEventCreate(&ev));
+        B->ev_req.assign(2 * n_req, nullptr);
+        for (auto &ev : B->ev_req)
+            if ((e = hipEventCreate(&ev)) != hipSuccess)
+                return bad(YTRC_EDEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e));
     }
     // Hold the datasets until the batch is destroyed (ytrc_dataset_free
     // refuses a dataset in use).
