@@ -470,6 +470,22 @@ class Camera(Orientation):
         self.switch_orientation(normal_vector=self._focus - self._position,
                                 north_vector=self.north_vector)
 
+    def set_position(self, position, north_vector=None):
+        """camera.py:367-385."""
+        if north_vector is not None:
+            self.north_vector = north_vector
+        self.position = position
+
+    def get_position(self):
+        return self.position
+
+    def set_focus(self, new_focus):
+        """camera.py:391-403."""
+        self.focus = new_focus
+
+    def get_focus(self):
+        return self.focus
+
     @property
     def focus(self):
         return self._focus
