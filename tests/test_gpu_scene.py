@@ -135,3 +135,49 @@ def test_scene_lenses_grey_opacity(ctx, lens, tmp_path):
         assert os.path.getsize(fn) > 100
     finally:
         src.free()
+
+
+def test_opaque_source_zbuffer(ctx):
+    """An opaque source draws a blue plane at depth 0.45 into the scene's
+    z-buffer (Scene.composite renders opaque sources first, scene.py:
+    612-619); the transparent volume source then starts every ray from that
+    plane's colour and stops it at max_t = clip(z, 0, 1) (utils.py:78-86,
+    image_samplers.pyx:173-176): the oracle with that image and z-buffer."""
+
+    class Plane(scene.OpaqueSource):
+        def render(self, camera, zbuffer=None):
+            zbuffer.rgba[:, :, 2] = 1.0
+            zbuffer.rgba[:, :, 3] = 1.0
+            zbuffer.z[:] = 0.45
+            self.zbuffer = zbuffer
+            return zbuffer.rgba
+
+    ds = small_amr(root_n=32, levels=2, seed=177)
+    tf = vr_tf_for(ds, nbins=96)
+    sc = scene.Scene()
+    cam = sc.add_camera(ds)
+    cam.resolution = (40, 40)
+    cam.set_width(1.2)
+    src = scene.VolumeSource(ds, "density", transfer_function=tf, ctx=ctx)
+    sc.add_source(src)
+    sc.add_source(Plane())
+    try:
+        im = sc.render()
+        p = cam._get_sampler_params(None)
+        p["viewpoint"] = np.array(cam.lens.viewpoint)
+        start = np.zeros((40, 40, 4))
+        start[:, :, 2] = 1.0
+        start[:, :, 3] = 1.0
+        oimg = oracle.cast(1, p, oracle_vr_bricks(src.volume), order=src.volume.vr_order(p["viewpoint"]),
+                           tf=tf_dict(tf), image=start, zbuffer=np.full((40, 40), 0.45))
+        oimg[:, :, 3] = 1.0
+        want = np.rot90(oimg, k=2)
+        for c in range(3):
+            e = rel_l2(im[..., c], want[..., c])
+            print(f"opaque plane + VR ch{c}: rel-L2={e:.3e}", flush=True)
+            assert e <= TOL
+        # the plane's blue stays (non-grey compositing of an opaque channel)
+        # and the volume in front of it adds red
+        assert np.allclose(im[..., 2], 1.0) and im[..., 0].max() > 0.0
+    finally:
+        src.free()

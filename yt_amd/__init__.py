@@ -20,7 +20,7 @@ from .off_axis_projection import (
     off_axis_projection_buffer,
 )
 from .partitioned_grid import PartitionedGrid
-from .scene import Scene, VolumeSource, ZBuffer, create_scene
+from .scene import OpaqueSource, Scene, VolumeSource, ZBuffer, create_scene
 from .transfer_functions import (
     ColorTransferFunction,
     MultiVariateTransferFunction,
@@ -42,6 +42,7 @@ __all__ = [
     "PlaneParallelLens",
     "ProjectionSampler",
     "ProjectionTransferFunction",
+    "OpaqueSource",
     "Scene",
     "VolumeSource",
     "ZBuffer",

@@ -170,6 +170,23 @@ def write_png(image, filename, sigma_clip=None, background="black", rescale_imag
 # ---------------------------------------------------------------------------
 # Sources and scenes
 
+class OpaqueSource:
+    """render_source.py:163-175: the base of sources that write into the
+    scene's z-buffer (lines, boxes, ...).  Scene.composite renders them
+    first, each given the shared ZBuffer to draw into."""
+
+    opaque = True
+
+    def __init__(self):
+        self.zbuffer = None
+
+    def set_zbuffer(self, zbuffer):
+        self.zbuffer = zbuffer
+
+    def render(self, camera, zbuffer=None):
+        raise NotImplementedError
+
+
 class VolumeSource:
     """KDTreeVolumeSource (render_source.py:563-616) on the GPU path.
 
