@@ -264,16 +264,19 @@ __device__ __forceinline__ int walk_loop(const DBrick &b, Walk &w, double max_t,
     const int D0 = uni(b.dims[0]), D1 = uni(b.dims[1]), D2 = uni(b.dims[2]);
     while (true) {
         hit += 1;
+        // The axis of the smallest tmax, ties to the later axis as the
+        // reference's comparisons order it (grid_traversal.pyx:120-130), as
+        // three lane masks (no divergent branches, no axis index).
         const bool a01 = t0 < t1;
         const bool a02 = t0 < t2;
         const bool a12 = t1 < t2;
-        const int ax = a01 ? (a02 ? 0 : 2) : (a12 ? 1 : 2);
-        const double tm = (ax == 0) ? t0 : ((ax == 1) ? t1 : t2);
+        const bool s0 = a01 & a02;
+        const bool s1 = !a01 & a12;
+        const bool s2 = !(s0 | s1);
+        const double tm = s0 ? t0 : (s1 ? t1 : t2);
         const double exit_t = y_fmin(tm, max_t);
         sample(enter_t, exit_t, c0, c1, c2);
-        // Step the chosen axis and test only its new index (lane masks, no
-        // divergent branches).
-        const bool s0 = (ax == 0), s1 = (ax == 1), s2 = (ax == 2);
+        // Step the chosen axis and test only its new index.
         c0 += s0 ? w.step[0] : 0;
         c1 += s1 ? w.step[1] : 0;
         c2 += s2 ? w.step[2] : 0;

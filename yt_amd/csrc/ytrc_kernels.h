@@ -108,8 +108,10 @@ __device__ __forceinline__ void walk_projection(const DBrick &b, Walk &w, double
                 const bool a01 = t0 < t1;
                 const bool a02 = t0 < t2;
                 const bool a12 = t1 < t2;
-                const int ax = a01 ? (a02 ? 0 : 2) : (a12 ? 1 : 2);
-                const double tm = (ax == 0) ? t0 : ((ax == 1) ? t1 : t2);
+                const bool s0 = a01 & a02;
+                const bool s1 = !a01 & a12;
+                const bool s2 = !(s0 | s1);
+                const double tm = s0 ? t0 : (s1 ? t1 : t2);
                 const double exit_t = y_fmin(tm, max_t);
                 if (REC) {
                     S.rec->visit(enter_t, exit_t, c0, c1, c2);
@@ -129,7 +131,6 @@ __device__ __forceinline__ void walk_projection(const DBrick &b, Walk &w, double
                 // Step the chosen axis and test only its new index, as the
                 // reference does, with lane masks rather than an axis-indexed
                 // table (which the compiler would place in scratch memory).
-                const bool s0 = (ax == 0), s1 = (ax == 1), s2 = (ax == 2);
                 c0 += s0 ? st0 : 0;
                 c1 += s1 ? st1 : 0;
                 c2 += s2 ? st2 : 0;
