@@ -132,6 +132,46 @@ __global__ void k_zero_counters(unsigned int *c, int n)
     if (i < n) c[i] = 0u;
 }
 
+// Stores field f of one brick, uploaded in C order (src, or none for the
+// padding fields f >= nf), in the device layout: one thread per stored
+// point; 4x4x4 cell blocks for cell-centred bricks, vertex blocks for
+// vertex-centred ones (a face vertex is read once per block that stores
+// it).  Points of no cell or vertex (block padding) are stored as 0.
+template <class T>
+__global__ __launch_bounds__(256) void k_store_field(const T *__restrict__ src, float *__restrict__ dst,
+                                                      int nfp, int f, int d0, int d1, int d2,
+                                                      int vertex, int64_t n)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int64_t s = -1;
+    if (vertex) {
+        const int local = (int)(p & (VBP - 1));
+        const int64_t blk = p >> VBP_SHIFT;
+        const int nb1 = vblocks(d1), nb2 = vblocks(d2);
+        const int bk = (int)(blk % nb2);
+        const int64_t bq = blk / nb2;
+        const int bj = (int)(bq % nb1);
+        const int bi = (int)(bq / nb1);
+        const int a = bi * VB + local / (VB1 * VB1);
+        const int b = bj * VB + (local / VB1) % VB1;
+        const int c = bk * VB + local % VB1;
+        if (local < VB1 * VB1 * VB1 && a <= d0 && b <= d1 && c <= d2)
+            s = ((int64_t)a * (d1 + 1) + b) * (d2 + 1) + c;
+    } else {
+        const int local = (int)(p & 63);
+        const int64_t blk = p >> 6;
+        const int nbj = (d1 + 3) >> 2, nbk = (d2 + 3) >> 2;
+        const int bk = (int)(blk % nbk);
+        const int64_t bq = blk / nbk;
+        const int bj = (int)(bq % nbj);
+        const int bi = (int)(bq / nbj);
+        const int i = 4 * bi + (local >> 4), j = 4 * bj + ((local >> 2) & 3), k = 4 * bk + (local & 3);
+        if (i < d0 && j < d1 && k < d2) s = ((int64_t)i * d1 + j) * d2 + k;
+    }
+    dst[p * nfp + f] = (src != nullptr && s >= 0) ? (float)src[s] : 0.0f;
+}
+
 }  // namespace
 
 namespace {
@@ -569,55 +609,51 @@ int ytrc_dataset_upload(ytrc_ctx *ctx, int32_t n_bricks, const ytrc_brick_desc *
     Dataset ds;
     int rc = dataset_alloc(ctx, "ytrc_dataset_upload", n_bricks, bricks, n_fields, mask_ptrs, ds);
     if (rc != YTRC_OK) return rc;
-    // Interleave fields per point, convert to float32, stage brick by brick.
-    std::vector<float> stage;
+    // Each brick field goes to the device in C order as given (float64 or
+    // float32) and k_store_field converts it to float32 in the device
+    // layout, fields interleaved per point.  One staging buffer holds the
+    // largest brick field; copies and kernels are ordered on the stream.
+    const size_t esz = src_dtype == 0 ? 8 : 4;
+    int64_t max_np = 0;
     for (int b = 0; b < n_bricks; b++) {
+        const DBrick &h = ds.h_bricks[b];
+        max_np = std::max(max_np, (int64_t)(h.dims[0] + h.vertex) * (h.dims[1] + h.vertex) *
+                                      (h.dims[2] + h.vertex));
+    }
+    void *d_src = nullptr;
+    hipError_t e = hipMalloc(&d_src, (size_t)max_np * esz);
+    if (e != hipSuccess) {
+        dataset_release(ctx, ds);
+        return fail(YTRC_EDEVICE, std::string("hipMalloc(staging): ") + hipGetErrorString(e));
+    }
+    for (int b = 0; b < n_bricks && e == hipSuccess; b++) {
         const DBrick &h = ds.h_bricks[b];
         const int64_t np = (int64_t)(h.dims[0] + h.vertex) * (h.dims[1] + h.vertex) *
                            (h.dims[2] + h.vertex);
-        stage.assign((size_t)stored_points(h) * ds.nfp, 0.0f);
-        // (stored point, source point) pairs: 4x4x4 cell blocks for
-        // cell-centred bricks; vertex blocks for vertex-centred ones, where a
-        // vertex on a block face is stored in every block that holds it.
-        std::vector<std::pair<int64_t, int64_t>> map;
-        map.reserve((size_t)np);
-        if (!h.vertex) {
-            const int nbj = (h.dims[1] + 3) >> 2, nbk = (h.dims[2] + 3) >> 2;
-            int64_t p = 0;
-            for (int i = 0; i < h.dims[0]; i++)
-                for (int j = 0; j < h.dims[1]; j++)
-                    for (int k = 0; k < h.dims[2]; k++)
-                        map.push_back({blocked_offset(i, j, k, nbj, nbk), p++});
-        } else {
-            const int nb0 = vblocks(h.dims[0]), nb1 = vblocks(h.dims[1]), nb2 = vblocks(h.dims[2]);
-            const int64_t s1 = h.dims[2] + 1, s0 = (int64_t)(h.dims[1] + 1) * s1;
-            for (int bi = 0; bi < nb0; bi++)
-                for (int bj = 0; bj < nb1; bj++)
-                    for (int bk = 0; bk < nb2; bk++) {
-                        const int64_t base = (((int64_t)bi * nb1 + bj) * nb2 + bk) << VBP_SHIFT;
-                        for (int a = 0; a < VB1; a++)
-                            for (int c1 = 0; c1 < VB1; c1++)
-                                for (int c2 = 0; c2 < VB1; c2++) {
-                                    const int vi = bi * VB + a, vj = bj * VB + c1, vk = bk * VB + c2;
-                                    if (vi > h.dims[0] || vj > h.dims[1] || vk > h.dims[2]) continue;
-                                    map.push_back({base + vcorner(a, c1, c2), vi * s0 + vj * s1 + vk});
-                                }
-                    }
+        const int64_t n = stored_points(h);
+        const unsigned grid = (unsigned)((n + 255) / 256);
+        for (int f = 0; f < ds.nfp && e == hipSuccess; f++) {
+            const bool have = f < n_fields;
+            if (have)
+                e = hipMemcpyAsync(d_src, field_ptrs[(size_t)b * n_fields + f], (size_t)np * esz,
+                                   hipMemcpyHostToDevice, ctx->stream);
+            if (e != hipSuccess) break;
+            if (src_dtype == 0)
+                hipLaunchKernelGGL(k_store_field<double>, dim3(grid), dim3(256), 0, ctx->stream,
+                                   have ? (const double *)d_src : nullptr, ds.d_pool + h.data_off,
+                                   ds.nfp, f, h.dims[0], h.dims[1], h.dims[2], h.vertex, n);
+            else
+                hipLaunchKernelGGL(k_store_field<float>, dim3(grid), dim3(256), 0, ctx->stream,
+                                   have ? (const float *)d_src : nullptr, ds.d_pool + h.data_off,
+                                   ds.nfp, f, h.dims[0], h.dims[1], h.dims[2], h.vertex, n);
+            e = hipGetLastError();
         }
-        for (int f = 0; f < n_fields; f++) {
-            const void *src = field_ptrs[(size_t)b * n_fields + f];
-            for (const auto &m : map) {
-                const float v = (src_dtype == 0) ? (float)((const double *)src)[m.second]
-                                                 : ((const float *)src)[m.second];
-                stage[(size_t)m.first * ds.nfp + f] = v;
-            }
-        }
-        const hipError_t e = hipMemcpy(ds.d_pool + h.data_off, stage.data(), stage.size() * 4,
-                                       hipMemcpyHostToDevice);
-        if (e != hipSuccess) {
-            dataset_release(ctx, ds);
-            return fail(YTRC_EDEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
-        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(d_src);
+    if (e != hipSuccess) {
+        dataset_release(ctx, ds);
+        return fail(YTRC_EDEVICE, std::string("ytrc_dataset_upload: ") + hipGetErrorString(e));
     }
     return dataset_commit(ctx, ds, ds_id_out);
 }

@@ -41,13 +41,25 @@ class ResidentVolume:
                 proj_fields = [proj_fields]
             bricks, fields = [], []
             for grid, mask in ds.blocks():
-                m = mask.astype("float64") if mask.dtype == bool else mask
+                # The child-masked fields (off_axis_projection.py:510-514),
+                # multiplied in the fields' own precision: float32 products
+                # round once, as the float64 product rounded to float32 would
+                # (the device stores float32); an all-ones mask is skipped.
+                arrs = {f: np.asarray(grid[f]) for f in list(proj_fields) + ([weight] if weight else [])}
+                dt = np.result_type(*[a.dtype for a in arrs.values()], np.float32)
+                full = bool(np.all(mask))
+                m = None if full else mask.astype(dt)
+
+                def masked(a):
+                    a = a.astype(dt, copy=False)
+                    return a if m is None else a * m
+
                 if weight is None:
-                    data = [np.asarray(grid[f], dtype="float64") * m for f in proj_fields]
+                    data = [masked(arrs[f]) for f in proj_fields]
                 else:
                     f0 = proj_fields[0]
-                    w = np.asarray(grid[weight], dtype="float64")
-                    data = [np.asarray(grid[f0], dtype="float64") * w * m, w * m]
+                    w = arrs[weight].astype(dt, copy=False)
+                    data = [masked(arrs[f0].astype(dt, copy=False) * w), masked(w)]
                 bricks.append((grid.LeftEdge, grid.RightEdge, grid.ActiveDimensions, False))
                 fields.append(data)
                 self.proj_cells += int(np.prod(grid.ActiveDimensions))
