@@ -149,6 +149,22 @@ def cpu_baseline(ds, vol, tf, cams, samples, budget_s):
         if one_s >= 0.2 * budget_s:
             break
     log(f"cpu oracle, 1 thread: {one_n} requests, {one_samp / one_s / 1e9:.4f} Gsamples/s")
+    # The 64^3 float64 field at 256^2 (the off_axis_projection case of the
+    # drop-in leg), per call, on 1 and on `threads` threads.
+    from yt_amd import data, render
+
+    ds0 = data.load_uniform_grid({"density": data.lognormal_density(64, seed=1001)})
+    p0 = render.camera_params((0.5, 0.5, 0.5), (0.3, 0.5, 0.81), 1.0, 256)
+    b0 = oracle_proj_bricks(ds0, ["density"])
+    c0 = {}
+    for nt in (1, threads):
+        xs = []
+        for _ in range(4):
+            t = time.perf_counter()
+            oracle.cast(0, p0, b0, num_threads=nt)
+            xs.append((time.perf_counter() - t) * 1e3)
+        c0[f"{nt}_threads"] = round(float(np.median(xs[1:])), 2)
+    log(f"cpu oracle, 64^3 projection at 256^2 (ms): {c0}")
     return {
         "value": tot_samp / tot_s / 1e9,
         "unit": "Gsamples/s",
@@ -158,7 +174,48 @@ def cpu_baseline(ds, vol, tf, cams, samples, budget_s):
                   f"FP64 C oracle with OpenMP static/100, {tot_samp} samples in {tot_s:.1f}s",
         "single_thread": {"value": one_samp / one_s / 1e9, "requests": one_n,
                           "seconds": round(one_s, 2)},
+        "c0_projection_64cubed_256_ms": c0,
     }
+
+
+def drop_in_leg(reps=3):
+    """End to end through the drop-in entry points, host arrays in, image
+    out (upload, brick preparation and cast included): off_axis_projection
+    of the 64^3 field at 256^2 and of the 256^3 field, density-weighted
+    temperature, at 1024^2; create_scene(...).render() of the AMR hierarchy
+    (8-Gaussian CTF) at 1024^2.  Median wall ms of `reps` calls after one."""
+    from yt_amd import data, off_axis_projection, scene
+
+    def timed(fn):
+        fn()
+        xs = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            xs.append((time.perf_counter() - t0) * 1e3)
+        return round(float(np.median(xs)), 2)
+
+    out = {}
+    ds0 = data.load_uniform_grid({"density": data.lognormal_density(64, seed=1001)})
+    out["c0_projection_64cubed_256"] = timed(
+        lambda: off_axis_projection(ds0, (0.5,) * 3, (0.3, 0.5, 0.81), 1.0, 256, "density"))
+    rho = data.lognormal_density(256, seed=1001)
+    ds1 = data.load_uniform_grid({"density": rho,
+                                  "temperature": data.temperature_from_density(rho, seed=1002)})
+    out["c1_weighted_projection_256cubed_1024"] = timed(
+        lambda: off_axis_projection(ds1, (0.5,) * 3, (0.3, 0.5, 0.81), 1.0, 1024, "temperature",
+                                    weight="density"))
+    amr = data.load_amr_grids(data.make_amr_hierarchy(128, 3), [128] * 3)
+
+    def render_amr():
+        sc = scene.create_scene(amr, "density")
+        sc.camera.resolution = (1024, 1024)
+        sc.render()
+        sc.sources["source_00"].free()
+
+    out["c2_amr_scene_render_1024"] = timed(render_amr)
+    log(f"drop-in end to end (ms): {out}")
+    return out
 
 
 def serve_leg(ctx, vol512, tf512, rates, n, budgets=()):
@@ -230,6 +287,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--lat-reps", type=int, default=20)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the end-to-end drop-in calls (off_axis_projection, Scene.render)")
     ap.add_argument("--serve-rates", default="5,10,20,50,100,200",
                     help="Poisson arrival rates (req/s) of the serving leg")
     ap.add_argument("--serve-budgets", default="1,0.25",
@@ -429,6 +488,8 @@ def main():
                             args.serve_n,
                             [float(x) for x in args.serve_budgets.split(",") if x.strip()])
 
+    dropin = None if args.no_dropin else drop_in_leg()
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(ds, vol, tf, cams, samples, args.cpu_seconds)
@@ -471,6 +532,7 @@ def main():
         "batched_vs_plain_by_side": None if plain is None else plain["by_side"],
         "latency_ms": lat,
         "serving": serving,
+        "drop_in_ms": dropin,
         "roofline": {
             "bound": "hbm",
             "kernel": "k_persist<VR>" if dom == 1 else "k_persist<projection>",

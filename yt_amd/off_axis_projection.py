@@ -33,12 +33,22 @@ def off_axis_projection(data_source, center, normal_vector, width, resolution, i
         params["bounds"], image, params["x_vec"], params["y_vec"], params["width"], "KDTree", 10,
         lens_type=params["lens_type"], zbuffer=zbuffer)
     for grid, mask in data_source.blocks():
+        # off_axis_projection.py:510-514: the fields times the child mask.
+        # The bricks are stored as float32, so the products are formed in the
+        # fields' own precision (a float32 product rounds once, as the float64
+        # product rounded to float32 does) and an all-ones mask is skipped.
+        full = bool(mask.all())
+        a = np.asarray(grid[item])
         if weight is None:
-            data = [np.asarray(grid[item], dtype="float64") * mask]
+            data = [a if full else a * mask]
         else:
-            w = np.asarray(grid[weight], dtype="float64")
-            data = [np.asarray(grid[item], dtype="float64") * w * mask, w * mask]
-        pg = PartitionedGrid(grid.id, data, mask.astype("uint8"), grid.LeftEdge, grid.RightEdge,
+            w = np.asarray(grid[weight])
+            dt = np.result_type(a.dtype, w.dtype, np.float32)
+            aw = a.astype(dt, copy=False) * w.astype(dt, copy=False)
+            w = w.astype(dt, copy=False)
+            data = [aw, w] if full else [aw * mask, w * mask]
+        m8 = mask.view(np.uint8) if mask.dtype == bool else mask.astype("uint8")
+        pg = PartitionedGrid(grid.id, data, m8, grid.LeftEdge, grid.RightEdge,
                              grid.ActiveDimensions.astype("int64"))
         sampler(pg, num_threads=num_threads)
     return finalize_projection(sampler.aimage, cam.width, weight is not None, method)

@@ -429,10 +429,18 @@ def create_scene(ds, field="density", lens_type="plane-parallel", transfer_funct
 
     sc = Scene()
     if transfer_function is None:
-        vals = np.concatenate([np.ravel(g[field]) for g in ds.grids])
-        vals = np.log10(vals[vals > 0]) if log_field else vals
-        transfer_function = gaussian_ctf((float(vals.min()), float(vals.max())), n_gauss=8,
-                                         nbins=256)
+        # the field's extrema (of its positive values for a log field), in
+        # log10 for a log field
+        lo, hi = np.inf, -np.inf
+        for g in ds.grids:
+            v = np.asarray(g[field])
+            if log_field:
+                v = v[v > 0]
+            if v.size:
+                lo, hi = min(lo, float(v.min())), max(hi, float(v.max()))
+        if log_field:
+            lo, hi = float(np.log10(lo)), float(np.log10(hi))
+        transfer_function = gaussian_ctf((lo, hi), n_gauss=8, nbins=256)
     src = VolumeSource(ds, field=field, log_field=log_field, transfer_function=transfer_function,
                        ctx=ctx)
     sc.add_source(src)
