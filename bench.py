@@ -218,11 +218,14 @@ def drop_in_leg(reps=3):
     return out
 
 
-def serve_leg(ctx, vol512, tf512, rates, n, budgets=()):
+def serve_leg(ctx, vol512, tf512, rates, n, budgets=(), policy_rates=(), age_limit_ms=100.0):
     """Continuous-batching server over three resident datasets: 256^3 and
     512^3 uniform lognormal fields and the seeded AMR hierarchy (128^3 root,
     3 levels); requests of side uniform in [256, 2048], half projection and
-    half VR (8-Gaussian CTF per dataset), Poisson arrivals (seed 1005)."""
+    half VR (8-Gaussian CTF per dataset), Poisson arrivals (seed 1005).
+    The server fills its chunks shortest first (requests older than
+    age_limit_ms first); policy_rates repeat the same request sets and
+    arrivals with chunks filled in arrival order, for comparison."""
     from yt_amd import _lib, data, render, serve
     from yt_amd.transfer_functions import gaussian_ctf
 
@@ -242,7 +245,12 @@ def serve_leg(ctx, vol512, tf512, rates, n, budgets=()):
 
     vols = [v256, vol512, vamr]
     tfs = {id(v256): tf_for(v256.ds), id(vol512): tf512, id(vamr): tf_for(amr)}
-    srv = _lib.Server(ctx, max_requests=256, budget_bytes=8 << 30, chunk_tiles=16384)
+    def server(budget):
+        srv = _lib.Server(ctx, max_requests=256, budget_bytes=budget, chunk_tiles=16384)
+        srv.set_policy("shortest_first", age_limit_ms=age_limit_ms)
+        return srv
+
+    srv = server(8 << 30)
     out = {}
     for k, lam in enumerate(rates):
         specs = serve.make_requests(vols, n, seed=1006 + k, tfs=tfs, rng_cam_seed=1004 + k,
@@ -252,12 +260,25 @@ def serve_leg(ctx, vol512, tf512, rates, n, budgets=()):
                            for kk, vv in serve.summarize(res, specs).items()}
         log(f"serving rate {lam:g}/s: {out[f'{lam:g}']}")
     srv.close()
+    by_policy = {}
+    srv = _lib.Server(ctx, max_requests=256, budget_bytes=8 << 30, chunk_tiles=16384)
+    srv.set_policy("fifo")
+    for k, lam in enumerate(rates):
+        if lam not in policy_rates:
+            continue
+        specs = serve.make_requests(vols, n, seed=1006 + k, tfs=tfs, rng_cam_seed=1004 + k,
+                                     deferred=True)
+        res = serve.run_poisson(srv, specs, lam, seed=1005 + k)
+        by_policy[f"{lam:g}"] = {kk: (round(vv, 3) if isinstance(vv, float) else vv)
+                                 for kk, vv in serve.summarize(res, specs).items()}
+        log(f"serving rate {lam:g}/s, arrival order: {by_policy[f'{lam:g}']}")
+    srv.close()
     # Smaller HBM budgets at the highest rate: admission control holds
     # requests back until earlier ones are read.
     by_budget = {}
     lam = rates[-1] if rates else 200.0
     for k, gib in enumerate(budgets):
-        srv = _lib.Server(ctx, max_requests=256, budget_bytes=int(gib * 2**30), chunk_tiles=16384)
+        srv = server(int(gib * 2**30))
         specs = serve.make_requests(vols, n, seed=1106 + k, tfs=tfs, rng_cam_seed=1104 + k,
                                      deferred=True)
         res = serve.run_poisson(srv, specs, lam, seed=1105 + k)
@@ -269,8 +290,10 @@ def serve_leg(ctx, vol512, tf512, rates, n, budgets=()):
     vamr.free()
     return {"datasets": ["256^3 uniform", "512^3 uniform (8 bricks)",
                          f"AMR 128^3+3 levels ({len(amr.grids)} grids)"],
-            "sides": [256, 2048], "budget_gib": 8, "rates": out,
-            "budget_sweep": {"rate": lam, "budgets_gib": by_budget}}
+            "sides": [256, 2048], "budget_gib": 8,
+            "policy": {"fill": "shortest_first", "age_limit_ms": age_limit_ms}, "rates": out,
+            "budget_sweep": {"rate": lam, "budgets_gib": by_budget},
+            "policy_sweep": {"policy": "fifo", "rates": by_policy}}
 
 
 def main():
@@ -294,6 +317,8 @@ def main():
     ap.add_argument("--serve-budgets", default="1,0.25",
                     help="smaller server budgets (GiB) swept at the highest rate")
     ap.add_argument("--serve-n", type=int, default=48, help="requests per rate")
+    ap.add_argument("--serve-policy-rates", default="50,100,200",
+                    help="rates repeated with chunks filled in arrival order")
     ap.add_argument("--no-alt", action="store_true",
                     help="skip the 64-brick (128^3 bricks) variant of the batched step")
     ap.add_argument("--no-tune", action="store_true",
@@ -486,7 +511,8 @@ def main():
     if not args.no_serve:
         serving = serve_leg(ctx, vol, tf, [float(x) for x in args.serve_rates.split(",")],
                             args.serve_n,
-                            [float(x) for x in args.serve_budgets.split(",") if x.strip()])
+                            [float(x) for x in args.serve_budgets.split(",") if x.strip()],
+                            [float(x) for x in args.serve_policy_rates.split(",") if x.strip()])
 
     dropin = None if args.no_dropin else drop_in_leg()
 

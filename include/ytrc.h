@@ -244,7 +244,8 @@ void ytrc_batch_destroy(ytrc_batch *b);
  * renders one sampler at a time with no queue (render_source.py:594-599,
  * off_axis_projection.py:506-526); a server admits requests while others
  * are on the GPU and runs the 8x8 ray tiles of every admitted request, in
- * arrival order, in chunks of `chunk_tiles` persistent-kernel work items
+ * arrival order (see ytrc_server_set_policy), in chunks of `chunk_tiles`
+ * persistent-kernel work items
  * (up to two chunks in flight).  `budget_bytes` of device memory is
  * allocated once; per-request descriptors and output images come from it.
  * submit returns YTRC_EBUDGET when the request does not fit now (no free
@@ -264,6 +265,18 @@ int ytrc_server_read(ytrc_server *s, int64_t id, double *image_out);
  * nch * nv0 * nv1): the written channels only, no layout conversion. */
 int ytrc_server_read_planar(ytrc_server *s, int64_t id, float *planes, int32_t cap_ch, int32_t *nch);
 int ytrc_server_cancel(ytrc_server *s, int64_t id);
+/* The order in which a server fills its chunks (no counterpart in the
+ * reference, which has no queue).  YTRC_SERVE_FIFO (the default): the
+ * tiles of admitted requests in arrival order.  YTRC_SERVE_SHORTEST_FIRST:
+ * the request with the least estimated work left first (tiles not yet
+ * launched; a tile of an interpolating sampler -- volume render,
+ * interpolated projection, light source -- counts as 4 projection tiles),
+ * except that requests admitted more than `age_limit_ms` ago go before
+ * all others, in arrival order, so that none waits without bound.  Takes
+ * effect from the next chunk; tiles already launched are not reordered. */
+#define YTRC_SERVE_FIFO 0
+#define YTRC_SERVE_SHORTEST_FIRST 1
+int ytrc_server_set_policy(ytrc_server *s, int32_t policy, double age_limit_ms);
 /* stats[8]: requests held, requests with tiles queued, chunks in flight,
  * bytes used, budget, tiles launched, chunks launched, free slots. */
 int ytrc_server_stats(const ytrc_server *s, int64_t *stats);

@@ -119,6 +119,81 @@ def test_server_cancel(ctx, volumes):
         srv.close()
 
 
+@pytest.mark.parametrize("policy,age_ms", [("fifo", 0.0), ("shortest_first", 1e7),
+                                           ("shortest_first", 0.0)])
+def test_server_policy_fill_order(ctx, volumes, policy, age_ms):
+    """One 256^2 volume render (1024 tiles) then six 16^2 requests, chunks of
+    64 tiles: in arrival order the large request finishes first; shortest
+    first lets the small ones overtake it, unless they are past the age
+    limit (0 ms: every request is, so arrival order again).  Every image
+    equals the request run alone."""
+    vols, tfs = volumes
+    rng = np.random.default_rng(13)
+    big = vols[0].vr_request(c0_params(256, normal=(0.3, -0.2, 0.93)), tfs[id(vols[0])])
+    small = []
+    for i in range(6):
+        nrm = rng.normal(size=3)
+        p = c0_params(16, normal=tuple(nrm / np.linalg.norm(nrm)))
+        v = vols[i % 2]
+        small.append(v.vr_request(p, tfs[id(v)]) if i % 3 else v.projection_request(p))
+    specs = [big] + small
+    alone = [_alone(ctx, s) for s in specs]
+    srv = _lib.Server(ctx, max_requests=16, budget_bytes=64 << 20, chunk_tiles=64)
+    try:
+        srv.set_policy(policy, age_limit_ms=age_ms)
+        ids = [srv.submit(s) for s in specs]
+        assert None not in ids
+        got = {}
+        pos = {}  # completion rank: finished ids come back in chunk order
+        while len(got) < len(specs):
+            for rid in srv.wait():
+                pos[rid] = len(pos)
+                got[rid] = srv.read(rid)
+        last_small = max(pos[r] for r in ids[1:])
+        first_small = min(pos[r] for r in ids[1:])
+        print(f"{policy} age={age_ms}: big finished {pos[ids[0]]}th, small "
+              f"{first_small}..{last_small}", flush=True)
+        if policy == "shortest_first" and age_ms > 0:
+            assert last_small < pos[ids[0]]
+        else:
+            assert pos[ids[0]] < first_small
+        for i, rid in enumerate(ids):
+            nch = 1 if specs[i].kind == _lib.PROJECTION else 3
+            assert np.array_equal(got[rid][..., :nch], alone[i][..., :nch]), i
+        assert srv.stats()["used"] == 0
+    finally:
+        srv.close()
+
+
+def test_server_policy_errors_and_cancel(ctx, volumes):
+    vols, tfs = volumes
+    srv = _lib.Server(ctx, max_requests=8, budget_bytes=64 << 20, chunk_tiles=64)
+    try:
+        with pytest.raises(ValueError):
+            srv.set_policy("lifo")
+        with pytest.raises(ValueError):
+            _lib._check(_lib.lib().ytrc_server_set_policy(srv._h, 7, 1.0))
+        with pytest.raises(ValueError):
+            srv.set_policy("shortest_first", age_limit_ms=-1.0)
+        srv.set_policy("shortest_first", age_limit_ms=1e7)
+        specs = _specs(vols, tfs, 5, seed=17)
+        ids = [srv.submit(s) for s in specs]
+        srv.cancel(ids[3])
+        done = set()
+        while len(done) < 4:
+            for rid in srv.wait():
+                done.add(rid)
+                srv.read_planar(rid)
+        assert ids[3] not in done
+        for _ in range(10):
+            srv.progress()
+            if srv.stats()["held"] == 0:
+                break
+        assert srv.stats()["used"] == 0
+    finally:
+        srv.close()
+
+
 def test_server_all_sampler_kinds_and_lenses(ctx, volumes):
     """Requests of every sampler kind and lens family, mixed in the server's
     chunks, equal each request run alone."""

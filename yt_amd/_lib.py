@@ -71,6 +71,7 @@ EXPORTED = [
     "ytrc_server_read",
     "ytrc_server_read_planar",
     "ytrc_server_cancel",
+    "ytrc_server_set_policy",
     "ytrc_server_stats",
     "ytrc_server_destroy",
     "ytrc_dataset_build_vr",
@@ -232,6 +233,7 @@ def _declare(lib):
     lib.ytrc_server_read_planar.argtypes = [P, C.c_int64, C.POINTER(C.c_float), C.c_int32,
                                             C.POINTER(C.c_int32)]
     lib.ytrc_server_cancel.argtypes = [P, C.c_int64]
+    lib.ytrc_server_set_policy.argtypes = [P, C.c_int32, C.c_double]
     lib.ytrc_server_stats.argtypes = [P, I64P]
     lib.ytrc_server_destroy.argtypes = [P]
     lib.ytrc_server_destroy.restype = None
@@ -723,6 +725,16 @@ class Server:
                                              out.ctypes.data_as(C.POINTER(C.c_float)), 4,
                                              C.byref(n)))
         return out[:n.value]
+
+    POLICIES = {"fifo": 0, "shortest_first": 1}
+
+    def set_policy(self, policy, age_limit_ms=100.0):
+        """Chunk fill order: "fifo" (arrival order) or "shortest_first"
+        (least estimated work left first; requests older than
+        age_limit_ms go first, in arrival order)."""
+        if policy not in self.POLICIES:
+            raise ValueError(f"unknown server policy {policy!r}")
+        _check(lib().ytrc_server_set_policy(self._h, self.POLICIES[policy], float(age_limit_ms)))
 
     def cancel(self, rid):
         _check(lib().ytrc_server_cancel(self._h, int(rid)))
