@@ -281,11 +281,13 @@ __device__ __forceinline__ void fit_eval_general(const TFCache &tf, const float4
 // zero entries 0 and nb, nb + 1 and a clamp of u to [0, nb + 1]; the test
 // is thereby made on the FP32 bin coordinate, exact up to its rounding.
 // FIT_get_value's test for values that are not normal is made once per
-// visit by the caller (quad_visit_ok), so v here is finite.
+// visit by the caller (quad_visit_ok), so v is finite.  The caller passes
+// the bin coordinate u, unclamped: along a visit's sub-samples it is the
+// cubic in s whose coefficients are the value's times idbin, plus qc1 on
+// the constant one.
 __device__ __forceinline__ void fit_eval_quad(const TFCache &tf, const float4 *__restrict__ lt,
-                                              float v, float e[3])
+                                              float u, float e[3])
 {
-    float u = __builtin_fmaf(v, tf.idbin, tf.qc1);
     u = __builtin_amdgcn_fmed3f(u, 0.0f, tf.qzb);
     const unsigned bin = (unsigned)u;
     const float f = __builtin_amdgcn_fractf(u);
@@ -446,6 +448,39 @@ struct VRSampler {
         }
     }
 
+    // ColorTransferFunction in transmittance form.  ku: the TF bin
+    // coordinate along the visit's sub-samples, u(s) = ku0 + ku1 s + ku2 s^2
+    // + ku3 s^3.  The visit's opacity Q_c = 1 - prod_s (1 - dt e_s) by the
+    // recurrence Q' = Q + e G, G = dt (1 - Q): Q stays small where the
+    // medium is thin, so FP32 keeps its relative precision; the FP64
+    // transmittance then takes T (1 - Q) once per visit.
+    template <int NS>
+    __device__ __forceinline__ void ctf_samples(const float (&ku)[4], double dt, int ns)
+    {
+        const float dtf = (float)dt;
+        float Q[3] = {0.0f, 0.0f, 0.0f};
+        float G[3] = {dtf, dtf, dtf};
+        const int nsq = (NS > 0) ? NS : ns;
+        float sf = 0.0f;
+#pragma unroll
+        for (int s = 0; s < nsq; s++, sf += 1.0f) {
+            float e[3];
+            // Sub-sample 0 is ku0 (the coefficients are finite here).
+            const float u = (NS > 0 && s == 0)
+                                ? ku[0]
+                                : __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(ku[3], sf, ku[2]), sf, ku[1]),
+                                                 sf, ku[0]);
+            fit_eval_quad(*tf, lt, u, e);
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++) {
+                Q[ch] = __builtin_fmaf(e[ch], G[ch], Q[ch]);
+                G[ch] = __builtin_fmaf(-dtf, Q[ch], dtf);
+            }
+        }
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) st[ch] = __builtin_fma(-st[ch], (double)Q[ch], st[ch]);
+    }
+
     template <int NS>
     __device__ __forceinline__ void samples(const float (&k)[NFP][4], double dt, int ns)
     {
@@ -457,37 +492,7 @@ struct VRSampler {
                 dvs[f] = __builtin_fmaf(
                     __builtin_fmaf(__builtin_fmaf(k[f][3], sf, k[f][2]), sf, k[f][1]), sf, k[f][0]);
         };
-        if constexpr ((MODE & 1) != 0) {
-            // The visit's opacity Q_c = 1 - prod_s (1 - dt e_s) by the
-            // recurrence Q' = Q + e G, G = dt (1 - Q): Q stays small where
-            // the medium is thin, so FP32 keeps its relative precision;
-            // the FP64 transmittance then takes T (1 - Q) once per visit.
-            const float dtf = (float)dt;
-            float Q[3] = {0.0f, 0.0f, 0.0f};
-            float G[3] = {dtf, dtf, dtf};
-            const int nsq = (NS > 0) ? NS : ns;
-            float sf = 0.0f;
-#pragma unroll
-            for (int s = 0; s < nsq; s++, sf += 1.0f) {
-                float dvs[NFP], e[3];
-                // Sub-sample 0 is k0 (the coefficients are finite here).
-#pragma unroll
-                for (int f = 0; f < NFP; f++)
-                    dvs[f] = (NS > 0 && s == 0)
-                                 ? k[f][0]
-                                 : __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(k[f][3], sf, k[f][2]), sf,
-                                                                 k[f][1]),
-                                                  sf, k[f][0]);
-                fit_eval_quad(T, lt, (NFP == 1) ? dvs[0] : pick(dvs, T.fid), e);
-#pragma unroll
-                for (int ch = 0; ch < 3; ch++) {
-                    Q[ch] = __builtin_fmaf(e[ch], G[ch], Q[ch]);
-                    G[ch] = __builtin_fmaf(-dtf, Q[ch], dtf);
-                }
-            }
-#pragma unroll
-            for (int ch = 0; ch < 3; ch++) st[ch] = __builtin_fma(-st[ch], (double)Q[ch], st[ch]);
-        } else if (tform) {
+        if (tform) {
             each_sample<NS>(ns, [&](int s) {
                 float dvs[NFP], e[4];
                 value(s, dvs);
@@ -569,6 +574,14 @@ struct VRSampler {
                 for (int f = 1; f < NFP; f++) kq[j] = (tf->fid == f) ? k[f][j] : kq[j];
             }
             if (!quad_visit_ok(kq)) return;
+            // The bin coordinate u = v idbin + qc1 of the value's cubic.
+            const float ku[4] = {__builtin_fmaf(kq[0], tf->idbin, tf->qc1), kq[1] * tf->idbin,
+                                 kq[2] * tf->idbin, kq[3] * tf->idbin};
+            if (MODE & 2)
+                ctf_samples<10>(ku, dt, 10);
+            else
+                ctf_samples<0>(ku, dt, n_samples);
+            return;
         }
         if (MODE & 2)
             samples<10>(k, dt, 10);
