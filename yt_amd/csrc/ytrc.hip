@@ -1027,33 +1027,29 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
         // unbounded CTFs take the general per-table path.
         ctf = ctf && q.tform && r.kind == YTRC_VOLUME_RENDER;
         if (ctf) {
-            // Transmittance form: per bin, the three emissions e_c(f) =
-            // (y_c + f dy_c)(y_a + f dy_a) as quadratics in the in-bin
-            // coordinate f, dy = y[b+1] - y[b] (FIT_get_value's lerp,
-            // field_interpolation_tables.pxd:63-82, times the alpha weight
-            // table, transfer_functions.py:385-408), coefficients rounded
-            // once from FP64.  Entry k = 1..nb-1 holds bin k-1; entries 0,
-            // nb and nb+1 are zero (values outside (b0, b1)).
+            // Transmittance form: per bin, the two lerps whose product is
+            // the emission, e_c = (y_c + f dy_c)(y_a + f dy_a), c = r, g, b
+            // (FIT_get_value's lerp, field_interpolation_tables.pxd:63-82,
+            // times the alpha weight table, transfer_functions.py:385-408),
+            // dy = y[b+1] - y[b], f = u - k the in-bin coordinate of entry
+            // k.  Stored as {y - k dy} and {dy} for the four tables, so that
+            // a lerp is one FMA in the bin coordinate u: y + f dy =
+            // (y - k dy) + u dy; dy is rounded to FP32 first and the offset
+            // formed from it in FP64.  Entry k = 1..nb-1 holds bin k-1;
+            // entries 0, nb and nb+1 are zero (values outside (b0, b1)).
             const int nb = tf.nbins[0];
             const double b0 = tf.bounds[0], b1 = tf.bounds[1];
             const double idbin = 1.0 / ((b1 - b0) / (double)(nb - 1));
-            std::vector<float4> lay((size_t)3 * (nb + 2), make_float4(0.f, 0.f, 0.f, 0.f));
+            std::vector<float4> lay((size_t)2 * (nb + 2), make_float4(0.f, 0.f, 0.f, 0.f));
             for (int k = 1; k < nb; k++) {
                 const int bb = k - 1;
-                double y[4], dy[4];
+                float y[4], d[4];
                 for (int t = 0; t < 4; t++) {
-                    dy[t] = tf.y[t][bb + 1] - tf.y[t][bb];
-                    y[t] = tf.y[t][bb];
+                    d[t] = (float)(tf.y[t][bb + 1] - tf.y[t][bb]);
+                    y[t] = (float)(tf.y[t][bb] - (double)k * (double)d[t]);
                 }
-                float A[3], B[3], Cq[3];
-                for (int c = 0; c < 3; c++) {
-                    A[c] = (float)(y[c] * y[3]);
-                    B[c] = (float)(y[c] * dy[3] + dy[c] * y[3]);
-                    Cq[c] = (float)(dy[c] * dy[3]);
-                }
-                lay[3 * k] = make_float4(A[0], A[1], A[2], B[0]);
-                lay[3 * k + 1] = make_float4(B[1], B[2], Cq[0], Cq[1]);
-                lay[3 * k + 2] = make_float4(Cq[2], 0.f, 0.f, 0.f);
+                lay[2 * k] = make_float4(y[0], y[1], y[2], y[3]);
+                lay[2 * k + 1] = make_float4(d[0], d[1], d[2], d[3]);
             }
             P.tft = std::move(lay);
             T.idbin[0] = (float)idbin;

@@ -177,7 +177,8 @@ __device__ __forceinline__ float pick(const float (&a)[N], int i)
 // (uniform) registers.  mode 1 is the ColorTransferFunction pattern
 // (transfer_functions.py:385-408): four tables on one field with shared
 // bounds, red/green/blue weighted by the alpha table, channels linked 0..3;
-// its LDS table holds per bin {y_r, y_g, y_b, y_a}, {dy_r..dy_a} and d0.
+// in transmittance form its LDS table holds per bin {y - k dy} and {dy} of
+// the four tables (fit_eval_ctf).
 // mode 0 is the general FieldInterpolationTable set.
 struct TFCache {
     int mode;
@@ -275,29 +276,30 @@ __device__ __forceinline__ void fit_eval_general(const TFCache &tf, const float4
 // An out-of-range or non-normal v reads bin 0 and gets alpha 0, so every
 // channel is 0 without a select per channel.
 // ColorTransferFunction emissions in the transmittance-form layout: entry
-// floor(u) of the per-bin quadratics, in-bin coordinate fract(u), where
-// u = 1 + (v - b0) * idbin.  FIT_get_value's range test v in (b0, b1)
+// k = floor(u) holds {y - k dy} and {dy} of the red, green, blue and alpha
+// tables, so that each lerp y + (u - k) dy is one FMA in the bin coordinate
+// u = 1 + (v - b0) * idbin, and e_c = lerp_c * lerp_alpha.  Two 16-byte LDS
+// reads per sub-sample: the loop is bound by LDS issue as much as by VALU.
+// FIT_get_value's range test v in (b0, b1)
 // (field_interpolation_tables.pxd:63-82) becomes u in [1, nb) through the
 // zero entries 0 and nb, nb + 1 and a clamp of u to [0, nb + 1]; the test
 // is thereby made on the FP32 bin coordinate, exact up to its rounding.
 // FIT_get_value's test for values that are not normal is made once per
 // visit by the caller (quad_visit_ok), so v is finite.  The caller passes
-// the bin coordinate u, unclamped: along a visit's sub-samples it is the
-// cubic in s whose coefficients are the value's times idbin, plus qc1 on
-// the constant one.
-__device__ __forceinline__ void fit_eval_quad(const TFCache &tf, const float4 *__restrict__ lt,
-                                              float u, float e[3])
+// u unclamped: along a visit's sub-samples it is the cubic in s whose
+// coefficients are the value's times idbin, plus qc1 on the constant one.
+__device__ __forceinline__ void fit_eval_ctf(const TFCache &tf, const float4 *__restrict__ lt,
+                                             float u, float e[3])
 {
     u = __builtin_amdgcn_fmed3f(u, 0.0f, tf.qzb);
     const unsigned bin = (unsigned)u;
-    const float f = __builtin_amdgcn_fractf(u);
-    const float4 *q = (const float4 *)((const char *)lt + __umul24(bin, 48u));
-    const float4 q0 = q[0];
-    const float4 q1 = q[1];
-    const float c2 = q[2].x;
-    e[0] = __builtin_fmaf(__builtin_fmaf(q1.z, f, q0.w), f, q0.x);
-    e[1] = __builtin_fmaf(__builtin_fmaf(q1.w, f, q1.x), f, q0.y);
-    e[2] = __builtin_fmaf(__builtin_fmaf(c2, f, q1.y), f, q0.z);
+    const float4 *q = (const float4 *)((const char *)lt + (bin << 5));
+    const float4 y = q[0];
+    const float4 d = q[1];
+    const float a = __builtin_fmaf(d.w, u, y.w);
+    e[0] = __builtin_fmaf(d.x, u, y.x) * a;
+    e[1] = __builtin_fmaf(d.y, u, y.y) * a;
+    e[2] = __builtin_fmaf(d.z, u, y.z) * a;
 }
 
 // Brick fields the samplers read, copied by value (uniform across a wave).
@@ -470,7 +472,7 @@ struct VRSampler {
                                 ? ku[0]
                                 : __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(ku[3], sf, ku[2]), sf, ku[1]),
                                                  sf, ku[0]);
-            fit_eval_quad(*tf, lt, u, e);
+            fit_eval_ctf(*tf, lt, u, e);
 #pragma unroll
             for (int ch = 0; ch < 3; ch++) {
                 Q[ch] = __builtin_fmaf(e[ch], G[ch], Q[ch]);
