@@ -1037,10 +1037,15 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
             // (y - k dy) + u dy; dy is rounded to FP32 first and the offset
             // formed from it in FP64.  Entry k = 1..nb-1 holds bin k-1;
             // entries 0, nb and nb+1 are zero (values outside (b0, b1)).
+            // Entries are 48 B apart (a zero float4 of padding): with an odd
+            // number of 16-B units per entry, the 16-B reads of different
+            // entries start on all 16 bank groups of the LDS rather than
+            // every other one, which halves the bank conflicts of the
+            // scattered per-lane reads.
             const int nb = tf.nbins[0];
             const double b0 = tf.bounds[0], b1 = tf.bounds[1];
             const double idbin = 1.0 / ((b1 - b0) / (double)(nb - 1));
-            std::vector<float4> lay((size_t)2 * (nb + 2), make_float4(0.f, 0.f, 0.f, 0.f));
+            std::vector<float4> lay((size_t)3 * (nb + 2), make_float4(0.f, 0.f, 0.f, 0.f));
             for (int k = 1; k < nb; k++) {
                 const int bb = k - 1;
                 float y[4], d[4];
@@ -1048,8 +1053,8 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
                     d[t] = (float)(tf.y[t][bb + 1] - tf.y[t][bb]);
                     y[t] = (float)(tf.y[t][bb] - (double)k * (double)d[t]);
                 }
-                lay[2 * k] = make_float4(y[0], y[1], y[2], y[3]);
-                lay[2 * k + 1] = make_float4(d[0], d[1], d[2], d[3]);
+                lay[3 * k] = make_float4(y[0], y[1], y[2], y[3]);
+                lay[3 * k + 1] = make_float4(d[0], d[1], d[2], d[3]);
             }
             P.tft = std::move(lay);
             T.idbin[0] = (float)idbin;

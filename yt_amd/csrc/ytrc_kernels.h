@@ -276,8 +276,8 @@ __device__ __forceinline__ void fit_eval_general(const TFCache &tf, const float4
 // An out-of-range or non-normal v reads bin 0 and gets alpha 0, so every
 // channel is 0 without a select per channel.
 // ColorTransferFunction emissions in the transmittance-form layout: entry
-// k = floor(u) holds {y - k dy} and {dy} of the red, green, blue and alpha
-// tables, so that each lerp y + (u - k) dy is one FMA in the bin coordinate
+// k = floor(u) (48 B apart) holds {y - k dy} and {dy} of the red, green,
+// blue and alpha tables, so that each lerp y + (u - k) dy is one FMA in the bin coordinate
 // u = 1 + (v - b0) * idbin, and e_c = lerp_c * lerp_alpha.  Two 16-byte LDS
 // reads per sub-sample: the loop is bound by LDS issue as much as by VALU.
 // FIT_get_value's range test v in (b0, b1)
@@ -293,7 +293,7 @@ __device__ __forceinline__ void fit_eval_ctf(const TFCache &tf, const float4 *__
 {
     u = __builtin_amdgcn_fmed3f(u, 0.0f, tf.qzb);
     const unsigned bin = (unsigned)u;
-    const float4 *q = (const float4 *)((const char *)lt + (bin << 5));
+    const float4 *q = (const float4 *)((const char *)lt + __umul24(bin, 48u));
     const float4 y = q[0];
     const float4 d = q[1];
     const float a = __builtin_fmaf(d.w, u, y.w);
