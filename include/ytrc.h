@@ -84,6 +84,11 @@ typedef struct ytrc_batch ytrc_batch;
 int ytrc_ctx_create(int32_t device, uint64_t budget_bytes, ytrc_ctx **out);
 void ytrc_ctx_destroy(ytrc_ctx *ctx);
 int ytrc_ctx_used_bytes(const ytrc_ctx *ctx, uint64_t *used, uint64_t *budget);
+/* LDS bytes one workgroup may allocate on the context's device (HIP's
+ * sharedMemPerBlock): it bounds the transfer-function tables of one launch,
+ * which sit in LDS (the reference keeps them in host memory,
+ * field_interpolation_tables.pxd:27-50, and has no such bound). */
+int ytrc_ctx_max_lds(const ytrc_ctx *ctx, uint64_t *bytes);
 const char *ytrc_last_error(void);
 
 /* Brick descriptor: PartitionedGrid(parent_grid_id, data, mask, left_edge,

@@ -1,10 +1,12 @@
 """ColorTransferFunction bin counts on the GPU.
 
 The transmittance-form table holds 48 B per bin plus three zero entries in
-LDS, so one launch takes 2 <= nbins <= 1363 (3 * (nbins + 2) * 16 B <= 64 KiB).
-Each size is rendered in a plain launch and in a persistent batch and
-compared with the FP64 oracle (rel-L2 <= 1e-5 per channel); 1364 bins are
-refused with a clear error instead of being launched."""
+LDS, so one launch takes 2 <= nbins <= max_lds / 48 - 2, with max_lds the
+LDS one workgroup may allocate on the device (64 KiB: 1363 bins; 160 KiB:
+3411 bins).  Sizes from 2 bins to that maximum are rendered in a plain
+launch and in a persistent batch and compared with the FP64 oracle (rel-L2
+<= 1e-5 per channel); one bin more is refused with a clear error instead
+of being launched."""
 
 import numpy as np
 import pytest
@@ -19,7 +21,10 @@ from .helpers import c0_params, oracle_vr_bricks, tf_dict, uniform_lognormal
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-5
-MAX_BINS = 1363
+
+
+def _max_bins(ctx):
+    return ctx.max_lds // 48 - 2
 
 
 def _tf(ds, nbins):
@@ -40,9 +45,13 @@ def vol(ctx):
     v.free()
 
 
-@pytest.mark.parametrize("nbins", [2, 3, 1024, MAX_BINS])
+@pytest.mark.parametrize("nbins", [2, 3, 1024, 1363, 2048, "max"])
 @pytest.mark.parametrize("mode", ["plain", "persistent"])
 def test_ctf_bin_counts(ctx, vol, nbins, mode):
+    if nbins == "max":
+        nbins = _max_bins(ctx)
+    if nbins > _max_bins(ctx):
+        pytest.skip(f"{nbins} bins exceed the {ctx.max_lds} B of LDS of a workgroup here")
     tf = _tf(vol.ds, nbins)
     p = c0_params(64, normal=(0.3, -0.45, 0.84))
     spec = vol.vr_request(p, tf, n_samples=10)
@@ -62,7 +71,9 @@ def test_ctf_bin_counts(ctx, vol, nbins, mode):
 
 
 def test_ctf_too_many_bins_refused(ctx, vol):
-    tf = _tf(vol.ds, MAX_BINS + 1)
+    print(f"max_lds = {ctx.max_lds} B: at most {_max_bins(ctx)} bins", flush=True)
+    tf = _tf(vol.ds, _max_bins(ctx) + 1)
     spec = vol.vr_request(c0_params(16), tf, n_samples=10)
-    with pytest.raises(ValueError, match="64 KiB"):
-        _lib.Batch(ctx, [spec]).close()
+    for mode in (_lib.BATCH_PLAIN, _lib.BATCH_PERSISTENT):
+        with pytest.raises(ValueError, match="B of LDS"):
+            _lib.Batch(ctx, [spec], mode).close()

@@ -72,6 +72,7 @@ EXPORTED = [
     "ytrc_server_read_planar",
     "ytrc_server_cancel",
     "ytrc_server_set_policy",
+    "ytrc_ctx_max_lds",
     "ytrc_server_stats",
     "ytrc_server_destroy",
     "ytrc_dataset_build_vr",
@@ -234,6 +235,7 @@ def _declare(lib):
                                             C.POINTER(C.c_int32)]
     lib.ytrc_server_cancel.argtypes = [P, C.c_int64]
     lib.ytrc_server_set_policy.argtypes = [P, C.c_int32, C.c_double]
+    lib.ytrc_ctx_max_lds.argtypes = [P, C.POINTER(C.c_uint64)]
     lib.ytrc_server_stats.argtypes = [P, I64P]
     lib.ytrc_server_destroy.argtypes = [P]
     lib.ytrc_server_destroy.restype = None
@@ -346,6 +348,14 @@ class Context:
         u, b = C.c_uint64(), C.c_uint64()
         _check(lib().ytrc_ctx_used_bytes(self._h, C.byref(u), C.byref(b)))
         return u.value, b.value
+
+    @property
+    def max_lds(self):
+        """LDS bytes one workgroup may allocate: the bound on the
+        transfer-function tables of one launch."""
+        b = C.c_uint64()
+        _check(lib().ytrc_ctx_max_lds(self._h, C.byref(b)))
+        return b.value
 
     def hbm_probe(self, nbytes=1 << 30, iters=20):
         """(GB/s, ms per copy) of a streaming device copy of nbytes."""

@@ -267,6 +267,7 @@ struct ytrc_ctx {
     uint64_t budget = 0;
     uint64_t used = 0;
     int n_cu = 0;
+    size_t max_lds = 64 * 1024;  // LDS bytes one workgroup may allocate
     std::map<int, Dataset> datasets;
     int next_ds = 1;
     std::mutex mu;
@@ -564,6 +565,7 @@ int ytrc_ctx_create(int32_t device, uint64_t budget_bytes, ytrc_ctx **out)
         return fail(YTRC_EDEVICE, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
     }
     c->n_cu = prop.multiProcessorCount;
+    if (prop.sharedMemPerBlock > 0) c->max_lds = prop.sharedMemPerBlock;
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -585,6 +587,13 @@ void ytrc_ctx_destroy(ytrc_ctx *ctx)
     }
     hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+int ytrc_ctx_max_lds(const ytrc_ctx *ctx, uint64_t *bytes)
+{
+    if (ctx == nullptr || bytes == nullptr) return fail(YTRC_EINVAL, "ytrc_ctx_max_lds: NULL");
+    *bytes = ctx->max_lds;
+    return YTRC_OK;
 }
 
 int ytrc_ctx_used_bytes(const ytrc_ctx *ctx, uint64_t *used, uint64_t *budget)
@@ -1069,7 +1078,10 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
                  ((r.flags & (YTRC_FLAG_ERT | YTRC_FLAG_PARTIAL)) ? 4 : 0) | (pp ? 8 : 0);
         T.total_bins = tot;
         q.lds = (size_t)tot * sizeof(float4);
-        if (q.lds > 64 * 1024) return fail(YTRC_EINVAL, tag + "transfer-function tables exceed 64 KiB of LDS");
+        if (q.lds > ctx->max_lds)
+            return fail(YTRC_EINVAL, tag + "transfer-function tables need " + std::to_string(q.lds) +
+                                         " B of LDS, more than the " + std::to_string(ctx->max_lds) +
+                                         " B a workgroup may allocate on this device");
         q.nch_out = (T.grey && r.kind == YTRC_VOLUME_RENDER) ? 4 : 3;
         if (partial) q.nch_out = 8;  // C[0..3], T[0..3]
     }
@@ -1077,6 +1089,13 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
     if (r.kind == YTRC_LIGHT_SOURCE) q.tform = 0;
     q.kfn = pick_kernel(r.kind, q.nfp, rec, q.mode);
     if (q.kfn == nullptr) return fail(YTRC_ENOTIMPL, tag + "no kernel for this field count");
+    if (q.lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute((const void *)q.kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)q.lds);
+        if (e != hipSuccess)
+            return fail(YTRC_EDEVICE, tag + "hipFuncSetAttribute(LDS " + std::to_string(q.lds) + " B): " +
+                                          hipGetErrorString(e));
+    }
     if (r.image_in != nullptr) q.image_in.assign(r.image_in, r.image_in + q.npix * 4);
     P.ext.resize(std::max(1, r.n_order));
     for (int k = 0; k < r.n_order; k++) {
@@ -1326,8 +1345,16 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
         g.n_items = (int)items.size();
         g.counter = (int)B->groups.size();
         g.lds = has_tf(g.kind) ? sizeof(float4) * tf_all.size() : 0;
-        if (g.lds > 64 * 1024)
-            return bad(YTRC_EINVAL, "ytrc_batch_create: the batch's distinct transfer functions exceed 64 KiB of LDS");
+        if (g.lds > ctx->max_lds)
+            return bad(YTRC_EINVAL, "ytrc_batch_create: the batch's distinct transfer functions need " +
+                                        std::to_string(g.lds) + " B of LDS, more than the " +
+                                        std::to_string(ctx->max_lds) + " B a workgroup may allocate on this device");
+        if (g.lds > 64 * 1024) {
+            const hipError_t e = hipFuncSetAttribute((const void *)g.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                     (int)g.lds);
+            if (e != hipSuccess)
+                return bad(YTRC_EDEVICE, std::string("ytrc_batch_create: hipFuncSetAttribute: ") + hipGetErrorString(e));
+        }
         B->groups.push_back(g);
         gitems_off.push_back(off);
         off = align_up(off + sizeof(int2) * std::max<size_t>(1, items.size()), 256);
