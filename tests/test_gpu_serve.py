@@ -260,6 +260,37 @@ def test_server_reuses_released_transfer_functions(ctx, volumes):
         srv.close()
 
 
+def test_server_large_transfer_function(ctx, volumes):
+    """The server's table holds as much as a workgroup's LDS: a 2048-bin
+    ColorTransferFunction (98 KiB of LDS) is served, mixed with a small one,
+    and renders exactly as alone; a table larger than the LDS is refused
+    (ValueError), not left waiting for room that never comes."""
+    vols, _ = volumes
+    va = vols[0]
+    max_bins = ctx.max_lds // 48 - 2
+    if max_bins < 2048:
+        pytest.skip(f"a workgroup has only {ctx.max_lds} B of LDS here")
+    big = vr_tf_for(va.ds, nbins=2048)
+    small = vr_tf_for(va.ds, nbins=64)
+    p = c0_params(64, normal=(0.5, 0.1, -0.86))
+    specs = [va.vr_request(p, big), va.vr_request(p, small)]
+    srv = _lib.Server(ctx, max_requests=8, budget_bytes=64 << 20, chunk_tiles=64)
+    try:
+        ids = [srv.submit(s) for s in specs]
+        assert None not in ids
+        got = {}
+        while len(got) < len(ids):
+            for rid in srv.wait():
+                got[rid] = srv.read(rid)
+        for rid, spec in zip(ids, specs):
+            assert np.array_equal(got[rid][..., :3], _alone(ctx, spec)[..., :3])
+        with pytest.raises(ValueError, match="B of LDS"):
+            srv.submit(va.vr_request(p, vr_tf_for(va.ds, nbins=max_bins + 1)))
+        assert srv.stats()["used"] == 0
+    finally:
+        srv.close()
+
+
 def test_dataset_free_refused_while_in_use(ctx):
     """ytrc_dataset_free refuses a dataset that a live batch or an unread
     server request still reads (YTRC_EBUSY), so no kernel can read freed

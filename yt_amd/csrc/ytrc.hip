@@ -1744,7 +1744,6 @@ struct SrvChunk {
 
 constexpr int SRV_CHUNKS = 3;
 constexpr int SRV_MAX_VARIANTS = 64;  // distinct persistent kernels one server launches
-constexpr int SRV_TF_CAP = 3072;  // float4 entries of the server's table (48 KiB)
 
 }  // namespace
 
@@ -1761,6 +1760,7 @@ struct ytrc_server {
     DImg *d_slots = nullptr;
     std::vector<int> free_slots;
     float4 *d_tf = nullptr;
+    int tf_cap = 0;  // float4 entries of the table: the LDS a workgroup may allocate
     std::vector<TFEntry> tfs;
     int tf_hi = 0;
     std::map<int64_t, SrvReq> reqs;
@@ -1787,8 +1787,18 @@ int srv_variant(ytrc_server *S, const PrepReq &P)
     for (size_t i = 0; i < S->vkeys.size(); i++)
         if (S->vkeys[i] == key) return (int)i;
     if ((int)S->vkeys.size() >= SRV_MAX_VARIANTS) return -1;
+    const pfn_t fn = pick_persist(P.q.r.kind, P.q.nfp, rec, P.q.mode);
+    if (has_tf(P.q.r.kind) && S->tf_cap * sizeof(float4) > 64 * 1024) {
+        // launches copy up to the whole table into LDS
+        const hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)(S->tf_cap * sizeof(float4)));
+        if (e != hipSuccess) {
+            fail(YTRC_EDEVICE, std::string("server: hipFuncSetAttribute: ") + hipGetErrorString(e));
+            return -2;
+        }
+    }
     S->vkeys.push_back(key);
-    S->vfns.push_back(pick_persist(P.q.r.kind, P.q.nfp, rec, P.q.mode));
+    S->vfns.push_back(fn);
     return (int)S->vkeys.size() - 1;
 }
 
@@ -1814,7 +1824,12 @@ int srv_tf_acquire(ytrc_server *S, const std::vector<float4> &t, int &entry)
         if (u.first - pos >= (int)t.size()) break;
         pos = std::max(pos, u.first + u.second);
     }
-    if (pos + (int)t.size() > SRV_TF_CAP)
+    if ((int)t.size() > S->tf_cap)
+        return fail(YTRC_EINVAL, "server: the transfer-function table needs " +
+                                     std::to_string(t.size() * sizeof(float4)) + " B of LDS, more than the " +
+                                     std::to_string(S->tf_cap * sizeof(float4)) +
+                                     " B a workgroup may allocate on this device");
+    if (pos + (int)t.size() > S->tf_cap)
         return fail(YTRC_EBUDGET, "server: no room for another transfer function in LDS");
     // Released tables keep their place (and are found again by key) until a
     // new table is written over them: forget every released table the new
@@ -2043,7 +2058,8 @@ int ytrc_server_create(ytrc_ctx *ctx, int32_t max_requests, uint64_t budget_byte
     S->alloc.init(S->arena, budget_bytes);
     e = hipMalloc(&S->d_slots, sizeof(DImg) * max_requests);
     if (e != hipSuccess) return bad(e, "hipMalloc(slots)");
-    e = hipMalloc(&S->d_tf, sizeof(float4) * SRV_TF_CAP);
+    S->tf_cap = (int)(ctx->max_lds / sizeof(float4));
+    e = hipMalloc(&S->d_tf, sizeof(float4) * S->tf_cap);
     if (e != hipSuccess) return bad(e, "hipMalloc(tf)");
     for (auto &c : S->chunks) {
         e = hipHostMalloc(&c.h_items, sizeof(int2) * chunk_tiles, hipHostMallocDefault);
@@ -2113,6 +2129,10 @@ int ytrc_server_submit(ytrc_server *S, const ytrc_request *r, int64_t *id_out)
         return rc;
     }
     R.variant = srv_variant(S, R.P);
+    if (R.variant == -2) {
+        srv_release(S, R);
+        return YTRC_EDEVICE;
+    }
     if (R.variant < 0) {
         srv_release(S, R);
         return fail(YTRC_ENOTIMPL, "server: too many distinct kernel variants");
