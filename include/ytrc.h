@@ -254,7 +254,9 @@ void ytrc_batch_destroy(ytrc_batch *b);
  * (up to two chunks in flight).  `budget_bytes` of device memory is
  * allocated once; per-request descriptors and output images come from it.
  * submit returns YTRC_EBUDGET when the request does not fit now (no free
- * slot or budget): the caller retries after reading finished requests. */
+ * slot or budget): the caller retries after reading finished requests; and
+ * YTRC_EINVAL when it can never fit (more bytes than the whole budget, or a
+ * transfer-function table larger than a workgroup's LDS). */
 typedef struct ytrc_server ytrc_server;
 int ytrc_server_create(ytrc_ctx *ctx, int32_t max_requests, uint64_t budget_bytes,
                        int32_t chunk_tiles, ytrc_server **out);

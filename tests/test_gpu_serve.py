@@ -260,6 +260,25 @@ def test_server_reuses_released_transfer_functions(ctx, volumes):
         srv.close()
 
 
+def test_server_refuses_request_larger_than_budget(ctx, volumes):
+    """A request whose image and descriptors exceed the server's whole
+    budget can never be admitted: submit raises ValueError instead of
+    returning None (wait and retry)."""
+    vols, tfs = volumes
+    srv = _lib.Server(ctx, max_requests=4, budget_bytes=1 << 20, chunk_tiles=64)
+    try:
+        small = vols[0].projection_request(c0_params(32))
+        assert srv.submit(small) is not None
+        with pytest.raises(ValueError, match="whole budget"):
+            srv.submit(vols[0].projection_request(c0_params(1024)))
+        while srv.stats()["held"]:
+            for rid in srv.wait():
+                srv.read(rid)
+        assert srv.stats()["used"] == 0
+    finally:
+        srv.close()
+
+
 def test_server_large_transfer_function(ctx, volumes):
     """The server's table holds as much as a workgroup's LDS: a 2048-bin
     ColorTransferFunction (98 KiB of LDS) is served, mixed with a small one,

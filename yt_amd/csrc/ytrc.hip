@@ -2099,6 +2099,12 @@ int ytrc_server_submit(ytrc_server *S, const ytrc_request *r, int64_t *id_out)
         srv_release(S, R);
         return fail(YTRC_EBUDGET, "server: all request slots are in use");
     }
+    if (align_up(R.P.bytes, 256) > S->budget) {
+        srv_release(S, R);
+        return fail(YTRC_EINVAL, "server: request needs " + std::to_string(R.P.bytes) +
+                                     " bytes, more than the server's whole budget of " +
+                                     std::to_string(S->budget));
+    }
     size_t off = 0;
     if (!S->alloc.alloc(R.P.bytes, off)) {
         srv_release(S, R);
