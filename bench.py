@@ -218,7 +218,7 @@ def drop_in_leg(reps=3):
     return out
 
 
-def serve_leg(ctx, vol512, tf512, rates, n, budgets=(), policy_rates=(), age_limit_ms=25.0):
+def serve_leg(ctx, vol512, tf512, rates, n, budgets=(), policy_rates=(), age_limit_ms=100.0):
     """Continuous-batching server over three resident datasets: 256^3 and
     512^3 uniform lognormal fields and the seeded AMR hierarchy (128^3 root,
     3 levels); requests of side uniform in [256, 2048], half projection and
@@ -317,7 +317,7 @@ def main():
     ap.add_argument("--serve-budgets", default="1,0.25",
                     help="smaller server budgets (GiB) swept at the highest rate")
     ap.add_argument("--serve-n", type=int, default=48, help="requests per rate")
-    ap.add_argument("--serve-age-ms", type=float, default=25.0,
+    ap.add_argument("--serve-age-ms", type=float, default=100.0,
                     help="age limit of the shortest-first chunk fill order (ms)")
     ap.add_argument("--serve-policy-rates", default="50,100,200",
                     help="rates repeated with chunks filled in arrival order")

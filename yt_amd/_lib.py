@@ -738,7 +738,7 @@ class Server:
 
     POLICIES = {"fifo": 0, "shortest_first": 1}
 
-    def set_policy(self, policy, age_limit_ms=25.0):
+    def set_policy(self, policy, age_limit_ms=100.0):
         """Chunk fill order: "fifo" (arrival order) or "shortest_first"
         (least estimated work left first; requests older than
         age_limit_ms go first, in arrival order)."""
