@@ -90,7 +90,7 @@ template <int NFP, bool REC>
 __device__ __forceinline__ void walk_projection(const DBrick &b, Walk &w, double max_t,
                                                 ProjSampler<NFP, REC> &S)
 {
-    constexpr int PF = 8;
+    constexpr int PF = 12;
     double t0 = w.tmax[0], t1 = w.tmax[1], t2 = w.tmax[2];
     int c0 = w.cur[0], c1 = w.cur[1], c2 = w.cur[2];
     double enter_t = w.enter_t;
