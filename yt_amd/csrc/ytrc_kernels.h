@@ -538,8 +538,14 @@ struct VRSampler {
         const int cell = (int)__umul24(__umul24(c0, d1) + c1, d2) + c2;
         const int voff = vertex_base(b, c0, c1, c2);
         // The mask byte and the corner values are independent loads: issue
-        // both before testing the mask.
+        // both first, then form the sample positions while they are in
+        // flight (the barrier keeps the scheduler from moving that work
+        // ahead of the loads; the empty asm below keeps the first use of a
+        // corner after it).
         const int mval = (mask != nullptr) ? (int)mask[cell] : 1;
+        float cv[8][NFP];
+        load_corners<NFP>(data + (size_t)voff * NFP, cv);
+        __builtin_amdgcn_sched_barrier(0);
         // dt = (exit_t - enter_t) / n_samples; positions from FP64 enter_t
         // in the order of image_samplers.pyx:475-479.
         const double dt = (ex - en) * inv_ns;
@@ -556,8 +562,12 @@ struct VRSampler {
             dp[i] = (float)(__builtin_fma(tt, pa[i], pb[i]) - (double)ci);
             ds[i] = paf[i] * dtf;
         }
-        float cv[8][NFP];
-        load_corners<NFP>(data + (size_t)voff * NFP, cv);
+        // The positions are formed before the corner values are first used.
+        asm volatile("" : "+v"(dp[0]), "+v"(dp[1]), "+v"(dp[2]), "+v"(ds[0]), "+v"(ds[1]), "+v"(ds[2]));
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+#pragma unroll
+            for (int f = 0; f < NFP; f++) asm volatile("" : "+v"(cv[q][f]));
         if (mval != 1) return;
         if (REC) rec->samples += n_samples;
         float k[NFP][4];
