@@ -1038,13 +1038,18 @@ constexpr int QSPLIT = 8;
 // tables of the batch sit in LDS; each image reads its own slot.  The item
 // index is made wave-uniform so that image, brick and table descriptors are
 // read with scalar loads.
-// Minimum waves per SIMD requested from the register allocator for the
-// production volume-render variants (one field, ColorTransferFunction path,
-// no record/ERT): 6 waves, i.e. 80 VGPRs with a 32-byte spill, measured
-// 0.8 % faster than 5 waves (95 VGPRs, no spill) and 2.5 % faster than 4.
+// Minimum waves per SIMD requested from the register allocator, 6 (80
+// VGPRs) for the production variants:
+// - volume render, one field, ColorTransferFunction path, no record/ERT: a
+//   32-byte spill; 0.8 % faster than 5 waves (95 VGPRs, no spill), 2.5 %
+//   faster than 4, and 7 waves (72 VGPRs, 68-byte spill) 0.5 % slower;
+// - projection, one field, no record: 3.6 % faster than the 5 waves its
+//   90 VGPRs give.
 template <int KIND, int NFP, bool REC, int MODE>
 struct MinWaves {
-    static constexpr int value = (KIND == 1 && NFP == 1 && !REC && (MODE == 1 || MODE == 3)) ? 6 : 1;
+    static constexpr int value = (KIND == 1 && NFP == 1 && !REC && (MODE == 1 || MODE == 3)) ? 6
+                                 : (KIND == 0 && NFP == 1 && !REC)                          ? 6
+                                                                                             : 1;
 };
 
 template <int KIND, int NFP, bool REC, int MODE>
