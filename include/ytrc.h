@@ -89,6 +89,10 @@ int ytrc_ctx_used_bytes(const ytrc_ctx *ctx, uint64_t *used, uint64_t *budget);
  * which sit in LDS (the reference keeps them in host memory,
  * field_interpolation_tables.pxd:27-50, and has no such bound). */
 int ytrc_ctx_max_lds(const ytrc_ctx *ctx, uint64_t *bytes);
+/* The device's name, ISA and CU count as one NUL-terminated string, cut to
+ * cap bytes (including the NUL).  Labels tuning caches and
+ * reports (the reference runs on the host and has no counterpart). */
+int ytrc_ctx_device_name(const ytrc_ctx *ctx, char *buf, int32_t cap);
 const char *ytrc_last_error(void);
 
 /* Brick descriptor: PartitionedGrid(parent_grid_id, data, mask, left_edge,

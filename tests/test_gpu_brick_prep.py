@@ -99,3 +99,31 @@ def test_thin_grid_is_rejected(ctx):
     with pytest.raises(ValueError):
         ctx.build_vr([((0, 0, 0), (1, 1, 1), (1, 4, 4))], [[np.ones((1, 4, 4))]], [False],
                      [(0, (0, 0, 0), (1, 4, 4), (0, 0, 0), (1, 1, 1))])
+
+
+def test_device_vertex_centring_equals_reference_bits(ctx):
+    """k_vertex_centre against ghost_zone_interpolate's own output
+    (interpolators.pyx:177-226, tests/golden/vertex_centre_vectors.npz):
+    the device's FP64 vertex values, stored as float32, equal the
+    reference's FP64 values uploaded as float32, point for point (no log,
+    so the only rounding is the final float32 store in both paths)."""
+    import os
+
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "vertex_centre_vectors.npz"))
+    for i in range(int(z["n"])):
+        a, want = z[f"in{i}"], z[f"out{i}"]
+        dims = a.shape
+        if min(dims) < 2:
+            continue
+        print(f"vertex centring vector {i}: dims {dims}", flush=True)
+        le, re = (0.0, 0.0, 0.0), tuple(float(d) / max(dims) for d in dims)
+        dev = ctx.build_vr([(le, re, dims)], [[a]], [False], [(0, (0, 0, 0), dims, le, re)])
+        host = ctx.upload([(le, re, dims, True)], [[want]])
+        try:
+            got = ctx.read_points(dev, 0)
+            ref = ctx.read_points(host, 0)
+            assert got.shape == ref.shape
+            assert np.array_equal(got.view(np.int32), ref.view(np.int32)), i
+        finally:
+            ctx.free(dev)
+            ctx.free(host)

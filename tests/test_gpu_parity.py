@@ -7,6 +7,8 @@ Bars (written into each test):
   * volume-render images: relative L2 <= 1e-5 vs FP64.
 """
 
+import sys
+
 import numpy as np
 import pytest
 
@@ -347,6 +349,10 @@ def test_autotune_picks_a_working_shape(ctx, tmp_path):
         assert len(table) == 3 * 2 * 2 and all(ms > 0 for _, ms in table)
         again, table2 = autotune.tune(ctx, specs, path=path)
         assert again == best and table2 == []
+        # the cache key comes from the library (ytrc_ctx_device_name): no
+        # second HIP runtime (torch's) is loaded into the process
+        assert "torch" not in sys.modules
+        assert "gfx" in ctx.device_name
         base = _run(ctx, specs)
         b = autotune.make_batch(ctx, specs, best)
         try:

@@ -300,3 +300,15 @@ def test_compute_stddev_image():
     assert out[0, 0] == 0.0 and np.all(np.isfinite(out))
     with pytest.raises(ValueError):
         compute_stddev_image(np.full((2, 2), 1.0), np.full((2, 2), 2.0))
+
+
+def test_product_modules_do_not_import_torch():
+    """The package, its autotuner and server driver load no second HIP
+    runtime: torch is never imported by the product path."""
+    import subprocess
+    import sys
+
+    code = ("import sys; import yt_amd, yt_amd.autotune, yt_amd.serve, yt_amd.scene, "
+            "yt_amd.off_axis_projection, yt_amd.image_samplers; "
+            "assert 'torch' not in sys.modules, 'torch imported'")
+    subprocess.run([sys.executable, "-c", code], check=True, cwd=ROOT)

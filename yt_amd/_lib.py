@@ -73,6 +73,7 @@ EXPORTED = [
     "ytrc_server_cancel",
     "ytrc_server_set_policy",
     "ytrc_ctx_max_lds",
+    "ytrc_ctx_device_name",
     "ytrc_server_stats",
     "ytrc_server_destroy",
     "ytrc_dataset_build_vr",
@@ -236,6 +237,7 @@ def _declare(lib):
     lib.ytrc_server_cancel.argtypes = [P, C.c_int64]
     lib.ytrc_server_set_policy.argtypes = [P, C.c_int32, C.c_double]
     lib.ytrc_ctx_max_lds.argtypes = [P, C.POINTER(C.c_uint64)]
+    lib.ytrc_ctx_device_name.argtypes = [P, C.c_char_p, C.c_int32]
     lib.ytrc_server_stats.argtypes = [P, I64P]
     lib.ytrc_server_destroy.argtypes = [P]
     lib.ytrc_server_destroy.restype = None
@@ -356,6 +358,13 @@ class Context:
         b = C.c_uint64()
         _check(lib().ytrc_ctx_max_lds(self._h, C.byref(b)))
         return b.value
+
+    @property
+    def device_name(self):
+        """The device's name, ISA and CU count (labels tuning caches)."""
+        buf = C.create_string_buffer(256)
+        _check(lib().ytrc_ctx_device_name(self._h, buf, len(buf)))
+        return buf.value.decode(errors="replace")
 
     def hbm_probe(self, nbytes=1 << 30, iters=20):
         """(GB/s, ms per copy) of a streaming device copy of nbytes."""

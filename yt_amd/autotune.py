@@ -67,16 +67,6 @@ def signature(specs, device_name=""):
     return "|".join(parts)
 
 
-def _device_name():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            return torch.cuda.get_device_name(0)
-    except Exception:  # noqa: BLE001 - the name only labels the cache entry
-        pass
-    return "device0"
-
-
 def make_batch(ctx, specs, cfg, flags=_lib.BATCH_PERSISTENT | _lib.BATCH_GRAPH):
     """A batch built and launched with the launch shape ``cfg``."""
     b = _lib.Batch(ctx, specs, flags, tile=tuple(cfg["tile"]))
@@ -90,7 +80,7 @@ def tune(ctx, specs, flags=_lib.BATCH_PERSISTENT | _lib.BATCH_GRAPH, tiles=TILES
     """Returns (best config dict, list of (config, ms) measured).  With
     ``use_cache`` a cached entry for this signature is returned unmeasured."""
     path = path or cache_path()
-    key = signature(specs, _device_name())
+    key = signature(specs, ctx.device_name)
     if use_cache:
         hit = _load(path).get(key)
         if hit is not None:

@@ -268,6 +268,7 @@ struct ytrc_ctx {
     uint64_t used = 0;
     int n_cu = 0;
     size_t max_lds = 64 * 1024;  // LDS bytes one workgroup may allocate
+    std::string name;            // marketing name and ISA of the device
     std::map<int, Dataset> datasets;
     int next_ds = 1;
     std::mutex mu;
@@ -566,6 +567,8 @@ int ytrc_ctx_create(int32_t device, uint64_t budget_bytes, ytrc_ctx **out)
     }
     c->n_cu = prop.multiProcessorCount;
     if (prop.sharedMemPerBlock > 0) c->max_lds = prop.sharedMemPerBlock;
+    c->name = std::string(prop.name) + " (" + prop.gcnArchName + ", " +
+              std::to_string(prop.multiProcessorCount) + " CUs)";
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -593,6 +596,16 @@ int ytrc_ctx_max_lds(const ytrc_ctx *ctx, uint64_t *bytes)
 {
     if (ctx == nullptr || bytes == nullptr) return fail(YTRC_EINVAL, "ytrc_ctx_max_lds: NULL");
     *bytes = ctx->max_lds;
+    return YTRC_OK;
+}
+
+int ytrc_ctx_device_name(const ytrc_ctx *ctx, char *buf, int32_t cap)
+{
+    if (ctx == nullptr || buf == nullptr || cap <= 0)
+        return fail(YTRC_EINVAL, "ytrc_ctx_device_name: NULL or empty buffer");
+    const size_t n = std::min(ctx->name.size(), (size_t)cap - 1);
+    memcpy(buf, ctx->name.data(), n);
+    buf[n] = '\0';
     return YTRC_OK;
 }
 
