@@ -98,14 +98,53 @@ def measured_traffic(path, kind, args):
     return int(round(1024 * (t["fetch_kib"] * t["fetch_factor"] + t["write_kib"] * t["write_factor"])))
 
 
-def cpu_baseline(ds, vol, tf, cams, samples, budget_s):
-    """FP64 oracle (OpenMP, all host threads of this process, up to 16) on a
-    bounded sample: requests taken smallest first, alternating kinds, until
-    the time budget is used."""
+def host_threads():
+    """Host threads this process may use: its CPU affinity mask, bounded by
+    the cgroup CPU quota when one is set (cpu.max)."""
+    n = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) // int(per)))
+    except (OSError, ValueError):
+        pass
+    return (n if quota is None else min(n, quota)), n, quota
+
+
+# Rows of each image side the CPU sample casts (a central strip of about
+# 16k pixels per image).
+SAMPLE_ROWS = {128: 128, 256: 64, 512: 32, 1024: 16}
+
+
+def sample_strata(cams):
+    """One camera per (kind, side): the first of the 64 with that kind and
+    side, and the central rows of its image the CPU sample casts."""
+    out = []
+    for kind in (0, 1):
+        for side in sorted(SAMPLE_ROWS):
+            idx = [i for i in range(len(cams)) if cams[i][0] == kind and cams[i][1]["nv"][0] == side]
+            if idx:
+                n = min(side, SAMPLE_ROWS[side])
+                r0 = (side - n) // 2
+                out.append((idx[0], (r0, r0 + n)))
+    return out
+
+
+def cpu_baseline(ds, vol, tf, cams, strata, window_samples, gpu_images, runs=5):
+    """The FP64 oracle (OpenMP static/100) on a bounded, stratified sample of
+    the workload: for each kind and image side one camera, cast over a
+    central strip of its image (sample_strata).  Timed as the median of
+    `runs` passes over the whole sample after one warm-up pass, on every
+    host thread available, and on one thread (as off_axis_projection runs
+    by default, off_axis_projection.py:34) as the median of 3 passes.  The
+    oracle's images of the sample are also compared with the GPU's images
+    of the timed batch (per-image rel-L2 and max-abs / max)."""
     from oracle import oracle
     from tests.helpers import oracle_proj_bricks, tf_dict
 
-    threads = min(16, len(os.sched_getaffinity(0)))
+    threads, affinity, quota = host_threads()
     pbricks = oracle_proj_bricks(ds, ["density"])
     vbricks = []
     for node in vol.leaves:
@@ -114,41 +153,49 @@ def cpu_baseline(ds, vol, tf, cams, samples, budget_s):
                             mask=pg.source_mask))
         node.data = None
     tfd = tf_dict(tf)
-    order = sorted(range(len(cams)), key=lambda i: (cams[i][1]["nv"][0], cams[i][0]))
-    tot_s, tot_samp, done = 0.0, 0, []
-    for i in order:
-        kind, p = cams[i]
+    orders = {i: vol.vr_order(cams[i][1]["viewpoint"]) for i, _ in strata if cams[i][0] == 1}
+
+    def one_pass(nt, keep=None):
         t = time.perf_counter()
-        if kind == 0:
-            oracle.cast(0, p, pbricks, num_threads=threads)
-        else:
-            oracle.cast(1, p, vbricks, order=vol.vr_order(p["viewpoint"]), tf=tfd, n_samples=10,
-                        num_threads=threads)
-        dt = time.perf_counter() - t
-        tot_s += dt
-        tot_samp += samples[i]
-        done.append(f"{'proj' if kind == 0 else 'vr'}{p['nv'][0]}")
-        if tot_s >= budget_s:
-            break
-    log(f"cpu oracle: {len(done)} requests, {tot_samp} samples in {tot_s:.1f}s on {threads} threads")
-    # One thread, as off_axis_projection runs by default
-    # (off_axis_projection.py:34, num_threads=1): the smallest requests for
-    # about a fifth of the budget.
-    one_s, one_samp, one_n = 0.0, 0, 0
-    for i in order:
-        kind, p = cams[i]
-        t = time.perf_counter()
-        if kind == 0:
-            oracle.cast(0, p, pbricks, num_threads=1)
-        else:
-            oracle.cast(1, p, vbricks, order=vol.vr_order(p["viewpoint"]), tf=tfd, n_samples=10,
-                        num_threads=1)
-        one_s += time.perf_counter() - t
-        one_samp += samples[i]
-        one_n += 1
-        if one_s >= 0.2 * budget_s:
-            break
-    log(f"cpu oracle, 1 thread: {one_n} requests, {one_samp / one_s / 1e9:.4f} Gsamples/s")
+        for i, rows in strata:
+            kind, p = cams[i]
+            if kind == 0:
+                img = oracle.cast(0, p, pbricks, num_threads=nt, rows=rows)
+            else:
+                img = oracle.cast(1, p, vbricks, order=orders[i], tf=tfd, n_samples=10,
+                                  num_threads=nt, rows=rows)
+            if keep is not None:
+                keep[i] = img[rows[0]:rows[1]]
+        return time.perf_counter() - t
+
+    total = int(sum(window_samples[i] for i, _ in strata))
+    oimgs = {}
+    one_pass(threads, oimgs)
+    times = [one_pass(threads) for _ in range(runs)]
+    t_med = float(np.median(times))
+    single = [one_pass(1) for _ in range(3)]
+    t_one = float(np.median(single))
+    names = [f"{'proj' if cams[i][0] == 0 else 'vr'}{cams[i][1]['nv'][0]} rows {r[0]}-{r[1]}"
+             for i, r in strata]
+    log(f"cpu oracle: {len(strata)} images ({', '.join(names)}), {total} samples, "
+        f"median {t_med:.2f}s of {runs} on {threads} threads, {t_one:.2f}s on 1 thread")
+    parity = []
+    for i, rows in strata:
+        kind = cams[i][0]
+        nch = 1 if kind == 0 else 3
+        g = gpu_images[i][rows[0]:rows[1]]
+        o = oimgs[i]
+        errs = []
+        for c in range(nch):
+            d = g[..., c].astype(np.float64) - o[..., c]
+            den = float(np.sqrt(np.sum(o[..., c] ** 2)))
+            mx = float(np.max(np.abs(o[..., c])))
+            errs.append({"rel_l2": float(np.sqrt(np.sum(d * d)) / den) if den else 0.0,
+                         "max_abs_over_max": float(np.max(np.abs(d)) / mx) if mx else 0.0})
+        parity.append({"image": i, "kind": "projection" if kind == 0 else "vr",
+                       "side": int(cams[i][1]["nv"][0]), "rows": list(rows), "channels": errs})
+    worst = max(max(c["rel_l2"] for c in e["channels"]) for e in parity)
+    log(f"sampled images vs the FP64 oracle: largest rel-L2 {worst:.2e}")
     # The 64^3 float64 field at 256^2 (the off_axis_projection case of the
     # drop-in leg), per call, on 1 and on `threads` threads.
     from yt_amd import data, render
@@ -159,22 +206,26 @@ def cpu_baseline(ds, vol, tf, cams, samples, budget_s):
     c0 = {}
     for nt in (1, threads):
         xs = []
-        for _ in range(4):
+        for _ in range(6):
             t = time.perf_counter()
             oracle.cast(0, p0, b0, num_threads=nt)
             xs.append((time.perf_counter() - t) * 1e3)
         c0[f"{nt}_threads"] = round(float(np.median(xs[1:])), 2)
     log(f"cpu oracle, 64^3 projection at 256^2 (ms): {c0}")
     return {
-        "value": tot_samp / tot_s / 1e9,
+        "value": total / t_med / 1e9,
         "unit": "Gsamples/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{len(done)} of the 64 requests, smallest first ({', '.join(done)}), "
-                  f"FP64 C oracle with OpenMP static/100, {tot_samp} samples in {tot_s:.1f}s",
-        "single_thread": {"value": one_samp / one_s / 1e9, "requests": one_n,
-                          "seconds": round(one_s, 2)},
+        "sample": f"one camera per kind and side of the 64 ({', '.join(names)}): "
+                  f"{total} samples; FP64 C oracle, OpenMP static/100; median of {runs} passes "
+                  f"after one warm-up",
+        "affinity_cpus": affinity,
+        "cgroup_quota_cpus": quota,
+        "seconds": [round(x, 3) for x in times],
+        "single_thread": {"value": total / t_one / 1e9, "seconds": [round(x, 3) for x in single]},
         "c0_projection_64cubed_256_ms": c0,
+        "parity_sample": parity,
     }
 
 
@@ -305,7 +356,8 @@ def main():
     ap.add_argument("--psize", type=int, default=2, help="grids (bricks) per axis")
     ap.add_argument("--cams", type=int, default=64)
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-runs", type=int, default=5,
+                    help="timed passes of the CPU baseline's sample (median reported)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--lat-reps", type=int, default=20)
@@ -316,7 +368,7 @@ def main():
                     help="Poisson arrival rates (req/s) of the serving leg")
     ap.add_argument("--serve-budgets", default="1,0.25",
                     help="smaller server budgets (GiB) swept at the highest rate")
-    ap.add_argument("--serve-n", type=int, default=48, help="requests per rate")
+    ap.add_argument("--serve-n", type=int, default=200, help="requests per rate")
     ap.add_argument("--serve-age-ms", type=float, default=100.0,
                     help="age limit of the shortest-first chunk fill order (ms)")
     ap.add_argument("--serve-policy-rates", default="50,100,200",
@@ -356,11 +408,17 @@ def main():
 
     # Sample counts (untimed record pass): visits with dt > 0 for projection,
     # mask-1 visits x n_samples for VR.
+    strata = sample_strata(cams)
+    strata_rows = dict(strata)
+    window_samples = {}
     rec = _lib.Batch(ctx, make_specs(vol, tf, cams, record=True), _lib.BATCH_PERSISTENT)
     rec.run()
     samples, npix, visits = [], [], 0
     for i in range(len(cams)):
         v, _, s = rec.read_record(i)
+        if i in strata_rows:
+            r0, r1 = strata_rows[i]
+            window_samples[i] = int(s[r0:r1].sum())
         samples.append(int(s.sum()))
         npix.append(int(v.size))
         visits += int(v.sum())
@@ -397,6 +455,9 @@ def main():
     fp64_ms = visits * FP64_OPS_PER_VISIT / (FP64_VECTOR_TFLOPS * 1e12) * 1e3
     imgs_s = len(cams) / (wall / args.steps)
     log(f"batched: {ms_step:.2f} ms/step, {gsps:.1f} Gsamples/s, {imgs_s:.1f} images/s")
+    # the timed batch's images of the CPU-sampled cameras (compared with the
+    # oracle in the cpu_baseline leg)
+    gpu_images = {i: batch.read_image(i) for i, _ in strata}
     batch.close()
 
     # Per-kind persistent kernels (dominant-kernel roofline).
@@ -521,7 +582,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(ds, vol, tf, cams, samples, args.cpu_seconds)
+        cpu = cpu_baseline(ds, vol, tf, cams, strata, window_samples, gpu_images,
+                           runs=args.cpu_runs)
         log(f"cpu baseline {cpu['value']:.4f} Gsamples/s on {cpu['cores']} threads")
 
     vol.free()

@@ -226,7 +226,12 @@ int ytrc_batch_launch(ytrc_batch *b);
  * the HIP events recorded around it on the same stream. */
 int ytrc_batch_sync(ytrc_batch *b, float *ms_out);
 /* Copies request i's image to image_out (float64, nv0*nv1*4, the reference's
- * aimage layout); channels the sampler does not write keep image_in. */
+ * aimage layout, which cast_through_kdtree updates in float64,
+ * image_samplers.pyx:177-196); channels the sampler does not write keep
+ * image_in.  With an image_in, projection samplers' sums (computed from 0 on
+ * the device) are added to it in float64, and compositing samplers' results
+ * that still equal float(image_in) return image_in itself: a pixel no sample
+ * changes keeps its input bits. */
 int ytrc_batch_read_image(ytrc_batch *b, int32_t i, double *image_out);
 /* YTRC_FLAG_PARTIAL only: request i's map to map_out (float64,
  * nv0*nv1*8: C[0..3] then T[0..3] per pixel).  Applied in call order, the

@@ -43,6 +43,8 @@ def lib():
         L = C.CDLL(LIB_PATH)
         D = C.POINTER(C.c_double)
         I32 = C.POINTER(C.c_int32)
+        L.orc_set_rows.restype = None
+        L.orc_set_rows.argtypes = [C.c_int, C.c_int]
         L.orc_cast.restype = C.c_int
         L.orc_cast.argtypes = [
             C.c_int, D, D, D, D, D, D, D, C.c_int, C.c_int, C.c_int, D, D, I32, C.c_int,
@@ -100,7 +102,7 @@ class _TFDesc(C.Structure):
 
 
 def cast(kind, params, bricks, order=None, tf=None, n_samples=10, image=None, zbuffer=None,
-         num_threads=0, record=False, with_samples=False):
+         num_threads=0, record=False, with_samples=False, rows=None):
     """Runs the oracle for one image.
 
     params: sampler parameters (vp_pos, vp_dir, center, bounds, x_vec, y_vec,
@@ -108,7 +110,9 @@ def cast(kind, params, bricks, order=None, tf=None, n_samples=10, image=None, zb
     float64 arrays), mask (uint8 array or None).  tf: dict with tables
     [(y, b0, b1)], field_ids, weight_field_ids, weight_table_ids,
     field_table_ids, grey_opacity.  Returns image (nv0, nv1, 4) [, visits,
-    hashes]."""
+    hashes].  rows=(r0, r1) casts only the pixels of image rows r0 <= vi < r1
+    (a bounded timing sample of a large image); the others keep their
+    input values."""
     L = lib()
     nv0, nv1 = int(params["nv"][0]), int(params["nv"][1])
     if image is None:
@@ -182,6 +186,7 @@ def cast(kind, params, bricks, order=None, tf=None, n_samples=10, image=None, zb
     x_vec = _v3(params["x_vec"], 3)
     y_vec = _v3(params["y_vec"], 3)
     width = _v3(params["width"], 3)
+    L.orc_set_rows(*((0, 0x7fffffff) if rows is None else (int(rows[0]), int(rows[1]))))
     rc = L.orc_cast_lens(
         int(kind), lens, _d(pix_pos) if lens else None, _d(pix_dir) if lens else None,
         _d(cam) if cam is not None else None,
@@ -192,6 +197,8 @@ def cast(kind, params, bricks, order=None, tf=None, n_samples=10, image=None, zb
         visits.ctypes.data_as(C.POINTER(C.c_int64)) if record else None,
         hashes.ctypes.data_as(C.POINTER(C.c_uint64)) if record else None,
         samples.ctypes.data_as(C.POINTER(C.c_int64)) if record else None)
+    if rows is not None:
+        L.orc_set_rows(0, 0x7fffffff)
     if rc != 0:
         raise ValueError(f"orc_cast failed ({rc})")
     if record and with_samples:

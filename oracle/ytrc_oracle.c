@@ -721,6 +721,17 @@ typedef struct {
  * arrays or NULL; samples: per-pixel sample counts (orc_count_subsamples) or
  * NULL, filled only with visits/hashes.  Returns 0, or -1 on bad arguments.
  */
+/* Rows of the image a cast covers (default: all).  Only for timing the
+ * oracle on a bounded sample of a large image: the pixels of rows outside
+ * [orc_row0, orc_row1) are left as they are. */
+static int orc_row0 = 0, orc_row1 = 0x7fffffff;
+
+void orc_set_rows(int r0, int r1)
+{
+    orc_row0 = r0 < 0 ? 0 : r0;
+    orc_row1 = r1;
+}
+
 int orc_cast_lens(int kind, int lens, const double *pix_pos, const double *pix_dir,
                   const double *camera_data, const double *vp_pos, const double *vp_dir,
                   const double *center, const double *bounds, const double *x_vec,
@@ -804,6 +815,8 @@ int orc_cast_lens(int kind, int lens, const double *pix_pos, const double *pix_d
         it[1] = y_iclip(it[1] + 1, 0, cam.nv[0]);
         it[2] = y_iclip(it[2] - 1, 0, cam.nv[1]);
         it[3] = y_iclip(it[3] + 1, 0, cam.nv[1]);
+        if (it[0] < orc_row0) it[0] = orc_row0;
+        if (it[1] > orc_row1) it[1] = orc_row1;
         int64_t nx = it[1] - it[0];
         int64_t ny = it[3] - it[2];
         int64_t size = nx * ny;

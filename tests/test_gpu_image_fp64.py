@@ -1,0 +1,152 @@
+"""The caller's float64 image across a cast (image_samplers.pyx:177-196:
+cast_through_kdtree reads and writes image[vi, vj] in float64, so a pixel no
+sample changes keeps its exact input value).
+
+An input image of values that float32 cannot hold is given to the drop-in
+ProjectionSampler, InterpolatedProjectionSampler and VolumeRenderSampler and
+to Scene.composite with an opaque source; the camera is wider than the
+volume, so border pixels see no brick.  Bars: those pixels come back bit
+for bit; every written channel is within rel-L2 <= 1e-5 of the FP64 oracle
+run from the same float64 image; channels a sampler does not write keep
+their input bits everywhere.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from yt_amd import scene
+from yt_amd.amr_kdtree import AMRKDTree
+from yt_amd.image_samplers import (InterpolatedProjectionSampler, ProjectionSampler,
+                                   VolumeRenderSampler)
+from yt_amd.partitioned_grid import PartitionedGrid
+
+from .conftest import rel_l2
+from .helpers import c0_params, oracle_vr_bricks, small_amr, tf_dict, uniform_lognormal, vr_tf_for
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _image(shape, seed):
+    rng = np.random.default_rng(seed)
+    img = 0.1 + rng.uniform(0.0, 0.3, size=shape)  # not float32 values
+    assert not np.any(img.astype(np.float32).astype(np.float64) == img)
+    return img
+
+
+def _untouched(out, start, visits):
+    miss = visits == 0
+    assert miss.sum() > 20  # the border sees no brick
+    assert np.array_equal(out[miss].view(np.int64), start[miss].view(np.int64))
+
+
+def test_projection_sampler_keeps_fp64_image():
+    ds = uniform_lognormal(20, seed=201, nprocs=2)
+    p = c0_params(36, normal=(0.3, 0.5, 0.81), width=1.6)
+    start = _image((36, 36, 4), 202)
+    image = start.copy()
+    s = ProjectionSampler(np.atleast_3d(p["vp_pos"]), np.atleast_3d(p["vp_dir"]), p["center"],
+                          p["bounds"], image, p["x_vec"], p["y_vec"], p["width"], "KDTree", 1)
+    bricks = []
+    for g, m in ds.blocks():
+        d = np.asarray(g["density"], dtype="float64") * m
+        s(PartitionedGrid(g.id, [d], m.astype("uint8"), g.LeftEdge, g.RightEdge,
+                          g.ActiveDimensions))
+        bricks.append(dict(le=g.LeftEdge, re=g.RightEdge, dims=g.ActiveDimensions, data=[d],
+                           mask=None))
+    out = s.aimage
+    assert out is image
+    oimg, ov, _ = oracle.cast(0, p, bricks, image=start.copy(), record=True)
+    _untouched(out, start, ov)
+    assert np.array_equal(out[..., 1:].view(np.int64), start[..., 1:].view(np.int64))
+    e = rel_l2(out[..., 0], oimg[..., 0])
+    d = rel_l2(out[..., 0] - start[..., 0], oimg[..., 0] - start[..., 0])
+    print(f"projection over a float64 image: rel-L2 {e:.2e} (of the added integral {d:.2e})",
+          flush=True)
+    assert e <= TOL and d <= TOL
+
+
+@pytest.mark.parametrize("kind", ["vr", "iproj"])
+def test_vertex_samplers_keep_fp64_image(kind):
+    ds = small_amr(root_n=16, levels=2, seed=203)
+    kd = AMRKDTree(ds)
+    kd.set_fields(["density"], [True])
+    tf = vr_tf_for(ds, nbins=64)
+    p = c0_params(40, normal=(-0.2, 0.6, 0.77), width=1.5)
+    start = _image((40, 40, 4), 204)
+    image = start.copy()
+    args = (np.atleast_3d(p["vp_pos"]), np.atleast_3d(p["vp_dir"]), p["center"], p["bounds"],
+            image, p["x_vec"], p["y_vec"], p["width"], "KDTree")
+    if kind == "vr":
+        s = VolumeRenderSampler(*args, tf, 10)
+    else:
+        s = InterpolatedProjectionSampler(*args, 10)
+    bricks = []
+    for pg in kd.traverse(viewpoint=p["viewpoint"]):
+        s(pg)
+        bricks.append(dict(le=pg.LeftEdge, re=pg.RightEdge, dims=pg.dims,
+                           data=[np.asarray(x, dtype=np.float64) for x in pg.my_data],
+                           mask=pg.source_mask if kind == "vr" else None))
+    out = s.aimage
+    if kind == "vr":
+        oimg, ov, _ = oracle.cast(1, p, bricks, tf=tf_dict(tf), image=start.copy(), record=True)
+        nch = 3
+    else:
+        oimg, ov, _ = oracle.cast(2, p, bricks, n_samples=10, image=start.copy(), record=True)
+        nch = 1
+    _untouched(out, start, ov)
+    assert np.array_equal(out[..., nch:].view(np.int64), start[..., nch:].view(np.int64))
+    for c in range(nch):
+        e = rel_l2(out[..., c], oimg[..., c])
+        print(f"{kind} over a float64 image ch{c}: rel-L2 {e:.2e}", flush=True)
+        assert e <= TOL
+
+
+def test_scene_composite_opaque_source_keeps_fp64_image(ctx):
+    """Scene.composite (scene.py:581-625): the opaque source's float64
+    colours are the volume source's starting image; pixels whose rays miss
+    the volume keep them exactly."""
+    start_rgb = _image((44, 44, 3), 205)
+
+    class Plane(scene.OpaqueSource):
+        def render(self, camera, zbuffer=None):
+            zbuffer.rgba[:, :, :3] = start_rgb
+            zbuffer.rgba[:, :, 3] = 1.0
+            zbuffer.z[:] = 0.6
+            self.zbuffer = zbuffer
+            return zbuffer.rgba
+
+    ds = small_amr(root_n=32, levels=2, seed=206)
+    tf = vr_tf_for(ds, nbins=96)
+    sc = scene.Scene()
+    cam = sc.add_camera(ds)
+    cam.resolution = (44, 44)
+    cam.set_width(1.9)
+    src = scene.VolumeSource(ds, "density", transfer_function=tf, ctx=ctx)
+    sc.add_source(src)
+    sc.add_source(Plane())
+    try:
+        im = sc.render()
+        p = cam._get_sampler_params(None)
+        p["viewpoint"] = np.array(cam.lens.viewpoint)
+        start = np.zeros((44, 44, 4))
+        start[:, :, :3] = start_rgb
+        start[:, :, 3] = 1.0
+        oimg, ov, _ = oracle.cast(1, p, oracle_vr_bricks(src.volume),
+                                  order=src.volume.vr_order(p["viewpoint"]), tf=tf_dict(tf),
+                                  image=start.copy(), zbuffer=np.full((44, 44), 0.6), record=True)
+        oimg[:, :, 3] = 1.0
+        want = np.rot90(oimg, k=2)
+        miss = np.rot90(ov == 0, k=2)
+        assert miss.sum() > 20
+        got = np.ascontiguousarray(im[..., :3])
+        assert np.array_equal(got[miss].view(np.int64),
+                              np.ascontiguousarray(want[..., :3])[miss].view(np.int64))
+        for c in range(3):
+            e = rel_l2(im[..., c], want[..., c])
+            print(f"opaque float64 plane + VR ch{c}: rel-L2={e:.3e}", flush=True)
+            assert e <= TOL
+    finally:
+        src.free()

@@ -312,3 +312,30 @@ def test_product_modules_do_not_import_torch():
             "yt_amd.off_axis_projection, yt_amd.image_samplers; "
             "assert 'torch' not in sys.modules, 'torch imported'")
     subprocess.run([sys.executable, "-c", code], check=True, cwd=ROOT)
+
+
+def test_serving_summary_and_p99_interval():
+    """serve.summarize: percentiles over all requests and the side <= 1024
+    ones; the bootstrap interval of the p99 brackets the sample p99 of a
+    large sample and narrows as the sample grows."""
+    from yt_amd import serve
+
+    rng = np.random.default_rng(4)
+    lat = rng.exponential(10.0, size=4000)
+    lo, hi = serve.p99_interval(lat)
+    p99 = float(np.percentile(lat, 99))
+    assert lo <= p99 <= hi
+    lo2, hi2 = serve.p99_interval(lat[:200])
+    assert hi2 - lo2 > hi - lo
+
+    class S:
+        def __init__(self, side):
+            self.nv = (side, side)
+
+    specs = [S(256), S(2048), S(512), S(1024)]
+    res = dict(latency_s=np.array([0.001, 0.1, 0.002, 0.003]), wall_s=2.0, first_arrival=0.0,
+               last_arrival=1.5, admission_waits=3)
+    out = serve.summarize(res, specs)
+    assert out["n"] == 4 and out["n_side_le_1024"] == 3
+    assert out["p99_ms_side_le_1024"] < 3.0 + 1e-9 and out["p99_ms"] > 90.0
+    assert abs(out["offered_images_per_s"] - 2.0) < 1e-12 and out["images_per_s"] == 2.0

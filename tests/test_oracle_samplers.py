@@ -353,3 +353,21 @@ def test_oracle_sample_counts():
     assert np.all(s % 7 == 0) and np.all(s <= 7 * v) and np.any(s < 7 * v)
     v, s = counts(0, unmasked, 1)
     assert np.array_equal(s, v)
+
+
+def test_oracle_row_window():
+    """cast(rows=(r0, r1)) computes exactly the full cast's pixels on those
+    rows and leaves every other pixel at its input value (the timing sample
+    of the CPU baseline)."""
+    rng = np.random.default_rng(3)
+    data = [np.exp(rng.normal(size=(12, 12, 12)))]
+    p = _params(40, (0.3, 0.5, 0.81))
+    full = oracle.cast(0, p, [_brick(12, data)])
+    img0 = rng.normal(size=(40, 40, 4))
+    part = oracle.cast(0, p, [_brick(12, data)], image=img0.copy(), rows=(13, 29))
+    whole = oracle.cast(0, p, [_brick(12, data)], image=img0.copy())
+    assert np.array_equal(part[13:29], whole[13:29])
+    assert not np.array_equal(whole[13:29], img0[13:29])
+    assert np.array_equal(part[:13], img0[:13]) and np.array_equal(part[29:], img0[29:])
+    again = oracle.cast(0, p, [_brick(12, data)])
+    assert np.array_equal(again, full)  # the window is reset after the call

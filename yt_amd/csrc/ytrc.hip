@@ -1134,7 +1134,7 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
     P.o_zbuf = off;
     if (r.zbuffer) off = align_up(off + sizeof(double) * q.npix, 256);
     P.o_init = off;
-    if (r.image_in) off = align_up(off + sizeof(float) * 4 * q.npix, 256);
+    if (r.image_in && !additive_kind(r.kind)) off = align_up(off + sizeof(float) * 4 * q.npix, 256);
     P.o_out = off;
     off = align_up(off + sizeof(float) * std::max(4, q.nch_out) * q.npix, 256);
     P.o_rv = P.o_rh = P.o_rs = off;
@@ -1159,6 +1159,16 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
     return YTRC_OK;
 }
 
+// Projection samplers add to the image (image_samplers.pyx:330-343,
+// 368-402): the device integrates from zero and the caller's float64
+// image_in is added on the host at read-back, so a pixel no visit touches
+// keeps its input bits.  Compositing samplers start from image_in on the
+// device.
+static bool additive_kind(int32_t kind)
+{
+    return kind == YTRC_PROJECTION || kind == YTRC_INTERPOLATED_PROJECTION;
+}
+
 // Copies a prepared request into its device region `A` and fills its
 // device descriptor.  `tf_table` is the device address of its (deduplicated)
 // transfer-function table, at float4 offset `tf_slot` of the launch's table.
@@ -1179,7 +1189,8 @@ int stage_request(ytrc_ctx *ctx, PrepReq &P, char *A, const float4 *tf_table, in
     d.order = (const int32_t *)(A + P.o_order);
     d.ext = (const int4 *)(A + P.o_ext);
     d.zbuf = r.zbuffer ? (const double *)(A + P.o_zbuf) : nullptr;
-    d.init = r.image_in ? (const float *)(A + P.o_init) : nullptr;
+    const bool init_dev = r.image_in != nullptr && !additive_kind(r.kind);
+    d.init = init_dev ? (const float *)(A + P.o_init) : nullptr;
     d.out = (float *)(A + P.o_out);
     d.rec_visits = rec ? (int64_t *)(A + P.o_rv) : nullptr;
     d.rec_hash = rec ? (uint64_t *)(A + P.o_rh) : nullptr;
@@ -1213,7 +1224,7 @@ int stage_request(ytrc_ctx *ctx, PrepReq &P, char *A, const float4 *tf_table, in
         HIPCHK(hipMemcpy(A + P.o_ppos, r.pix_pos, sizeof(double) * 3 * q.npix, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(A + P.o_pdir, r.pix_dir, sizeof(double) * 3 * q.npix, hipMemcpyHostToDevice));
     }
-    if (r.image_in) {
+    if (init_dev) {
         stage.resize((size_t)q.npix * 4);
         for (int64_t p = 0; p < q.npix; p++)
             for (int c = 0; c < 4; c++) stage[(size_t)c * q.npix + p] = (float)r.image_in[p * 4 + c];
@@ -1228,6 +1239,21 @@ int stage_request(ytrc_ctx *ctx, PrepReq &P, char *A, const float4 *tf_table, in
     return YTRC_OK;
 }
 
+// The float64 value of channel value `o` (float32, from the device) of a
+// pixel whose input was `in` (image_samplers.pyx:177-196 keeps the image in
+// float64 across bricks):
+//   projection samplers: the device's sum starts from 0, so in + o, and a
+//   pixel no visit touched (o == 0) keeps `in` exactly;
+//   compositing samplers: the device started from float(in); where the
+//   result still equals float(in), nothing changed the pixel by a float32
+//   step, and `in` is returned exactly (for an untouched pixel, or one whose
+//   samples emit nothing, yt leaves the float64 value as it was).
+static inline double merge_input(int32_t kind, float o, double in)
+{
+    if (additive_kind(kind)) return o == 0.0f ? in : in + (double)o;
+    return o == (float)in ? in : (double)o;
+}
+
 // Copies a finished request's output (and image_in for channels it does
 // not write) into the caller's float64 (nv0, nv1, 4) image.
 int read_request_image(const ReqHost &q, const DImg &d, double *image_out)
@@ -1237,13 +1263,16 @@ int read_request_image(const ReqHost &q, const DImg &d, double *image_out)
                                  "read it with ytrc_batch_read_partial");
     std::vector<float> h((size_t)q.npix * q.nch_out);
     HIPCHK(hipMemcpy(h.data(), d.out, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
+    const bool has_in = !q.image_in.empty();
     for (int64_t p = 0; p < q.npix; p++) {
         for (int c = 0; c < 4; c++) {
+            const double in = has_in ? q.image_in[(size_t)p * 4 + c] : 0.0;
             double v;
             if (c < q.nch_out)
-                v = (double)h[(size_t)c * q.npix + p];
+                v = has_in ? merge_input(q.r.kind, h[(size_t)c * q.npix + p], in)
+                           : (double)h[(size_t)c * q.npix + p];
             else
-                v = q.image_in.empty() ? 0.0 : q.image_in[(size_t)p * 4 + c];
+                v = in;
             image_out[(size_t)p * 4 + c] = v;
         }
     }
@@ -2232,6 +2261,12 @@ int ytrc_server_read_planar(ytrc_server *S, int64_t id, float *planes, int32_t c
     HIPCHK(hipSetDevice(S->ctx->device));
     HIPCHK(hipMemcpy(planes, it->second.d.out, sizeof(float) * (size_t)q.npix * q.nch_out,
                      hipMemcpyDeviceToHost));
+    if (!q.image_in.empty())
+        for (int c = 0; c < q.nch_out; c++)
+            for (int64_t p = 0; p < q.npix; p++) {
+                float &o = planes[(size_t)c * q.npix + p];
+                o = (float)merge_input(q.r.kind, o, q.image_in[(size_t)p * 4 + c]);
+            }
     *nch = q.nch_out;
     srv_release(S, it->second);
     S->reqs.erase(it);

@@ -94,19 +94,38 @@ def run_poisson(server, specs, rate, seed=1005, readback="planar", idle_sleep=2e
                 first_arrival=float(arrivals[0]), last_arrival=float(arrivals[-1]))
 
 
+def p99_interval(lat_ms, n_boot=2000, seed=0):
+    """Bootstrap 95 % interval of the 99th percentile of lat_ms: the 2.5th
+    and 97.5th percentiles of the p99 of n_boot resamples (with
+    replacement) of the same size."""
+    lat_ms = np.asarray(lat_ms, dtype=np.float64)
+    if lat_ms.size < 2:
+        return None
+    rng = np.random.default_rng(seed)
+    idx = rng.integers(0, lat_ms.size, size=(n_boot, lat_ms.size))
+    boots = np.percentile(lat_ms[idx], 99, axis=1)
+    return [round(float(np.percentile(boots, 2.5)), 3), round(float(np.percentile(boots, 97.5)), 3)]
+
+
 def summarize(res, specs):
     """Latency percentiles over all requests and over those of side <= 1024
-    (the 1024^2-and-smaller requests), images/s and admission waits."""
+    (the 1024^2-and-smaller requests), with bootstrap 95 % intervals of the
+    p99s; images/s offered (arrivals) and served; admission waits."""
     lat = res["latency_s"] * 1e3
     span = res["wall_s"] - res["first_arrival"]
     sides = np.array([max(s.nv) if hasattr(s, "nv") else 0 for s in specs])
     small = lat[sides <= 1024]
+    arr_span = res["last_arrival"] - res["first_arrival"]
     return dict(
         n=len(specs),
+        n_side_le_1024=int(small.size),
         p50_ms=float(np.percentile(lat, 50)),
         p99_ms=float(np.percentile(lat, 99)),
+        p99_ms_ci95=p99_interval(lat),
         mean_ms=float(np.mean(lat)),
         p99_ms_side_le_1024=float(np.percentile(small, 99)) if small.size else None,
+        p99_ms_side_le_1024_ci95=p99_interval(small) if small.size else None,
+        offered_images_per_s=(len(specs) - 1) / arr_span if arr_span > 0 else None,
         images_per_s=len(specs) / span,
         admission_waits=int(res["admission_waits"]),
     )
