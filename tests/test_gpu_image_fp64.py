@@ -48,7 +48,8 @@ def test_projection_sampler_keeps_fp64_image():
     start = _image((36, 36, 4), 202)
     image = start.copy()
     s = ProjectionSampler(np.atleast_3d(p["vp_pos"]), np.atleast_3d(p["vp_dir"]), p["center"],
-                          p["bounds"], image, p["x_vec"], p["y_vec"], p["width"], "KDTree", 1)
+                          p["bounds"], image, p["x_vec"], p["y_vec"], p["width"], "KDTree", 1,
+                          lens_type="plane-parallel")
     bricks = []
     for g, m in ds.blocks():
         d = np.asarray(g["density"], dtype="float64") * m
@@ -80,9 +81,9 @@ def test_vertex_samplers_keep_fp64_image(kind):
     args = (np.atleast_3d(p["vp_pos"]), np.atleast_3d(p["vp_dir"]), p["center"], p["bounds"],
             image, p["x_vec"], p["y_vec"], p["width"], "KDTree")
     if kind == "vr":
-        s = VolumeRenderSampler(*args, tf, 10)
+        s = VolumeRenderSampler(*args, tf, 10, lens_type="plane-parallel")
     else:
-        s = InterpolatedProjectionSampler(*args, 10)
+        s = InterpolatedProjectionSampler(*args, 10, lens_type="plane-parallel")
     bricks = []
     for pg in kd.traverse(viewpoint=p["viewpoint"]):
         s(pg)
