@@ -276,6 +276,16 @@ struct ytrc_ctx {
 
 namespace {
 
+// Projection samplers add to the image (image_samplers.pyx:330-343,
+// 368-402): the device integrates from zero and the caller's float64
+// image_in is added on the host at read-back, so a pixel no visit touches
+// keeps its input bits.  Compositing samplers start from image_in on the
+// device.
+static bool additive_kind(int32_t kind)
+{
+    return kind == YTRC_PROJECTION || kind == YTRC_INTERPOLATED_PROJECTION;
+}
+
 struct ReqHost {
     ytrc_request r;
     std::vector<double> image_in;  // nv0*nv1*4 or empty
@@ -1157,16 +1167,6 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
     }
     P.bytes = off;
     return YTRC_OK;
-}
-
-// Projection samplers add to the image (image_samplers.pyx:330-343,
-// 368-402): the device integrates from zero and the caller's float64
-// image_in is added on the host at read-back, so a pixel no visit touches
-// keeps its input bits.  Compositing samplers start from image_in on the
-// device.
-static bool additive_kind(int32_t kind)
-{
-    return kind == YTRC_PROJECTION || kind == YTRC_INTERPOLATED_PROJECTION;
 }
 
 // Copies a prepared request into its device region `A` and fills its
