@@ -95,3 +95,23 @@ def test_volume_source_transfer_function_rules():
         s.transfer_function = MultiVariateTransferFunction()
     with pytest.raises(NotImplementedError):
         ProjectionTransferFunction(n_fields=4)
+
+
+def test_zbuffer_ties_and_columns():
+    """Equal depths clear the pixel in 2-d depth maps; one-column depth maps
+    blend with weights 1 / 0, the second buffer winning ties
+    (zbuffer_array.py:44-57)."""
+    z = np.array([[0.2, 0.5], [0.7, 0.1]])
+    a = ZBuffer(np.full((2, 2, 4), 1.0), z.copy())
+    b = ZBuffer(np.full((2, 2, 4), 2.0), np.array([[0.3, 0.5], [0.6, 0.1]]))
+    c = a + b
+    assert np.array_equal(c.rgba[:, :, 0], [[1.0, 0.0], [2.0, 0.0]])
+    zc = np.array([[0.2], [0.5], [0.9]])
+    d = ZBuffer(np.full((3, 1, 4), 1.0), zc) + ZBuffer(np.full((3, 1, 4), 2.0),
+                                                      np.array([[0.4], [0.5], [0.1]]))
+    assert np.array_equal(d.rgba[:, 0, 0], [1.0, 2.0, 2.0])
+    a += b
+    assert a == c
+    a.paint((0, 0), np.full(4, 7.0), 0.05)
+    a.paint((1, 1), np.full(4, 9.0), 0.5)
+    assert a.rgba[0, 0, 0] == 7.0 and a.z[0, 0] == 0.05 and a.rgba[1, 1, 0] == 0.0

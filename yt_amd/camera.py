@@ -145,7 +145,7 @@ class PerspectiveLens(PlaneParallelLens):
 
     def _get_sampler_params(self, camera, render_source=None):
         """lens.py:166-234, in the same numpy operations and order."""
-        # Enforce width[1] / width[0] = resolution[1] / resolution[0]
+        # the second width follows from the first and the image's aspect ratio
         camera.width[1] = camera.width[0] * (camera.resolution[1] / camera.resolution[0])
         res0, res1 = camera.resolution
         east_vec = camera.unit_vectors[0]
@@ -160,10 +160,10 @@ class PerspectiveLens(PlaneParallelLens):
         sample_x = np.repeat(sample_x.reshape(res0, 1, 3), res1, axis=1)
         sample_y = np.repeat(sample_y.reshape(1, res1, 3), res0, axis=0)
         normal_vecs = np.tile(normal_vec, res0 * res1).reshape(res0, res1, 3)
-        # The maximum possible length of ray
+        # no point of the domain lies farther from the camera than this
         max_length = np.linalg.norm(camera.position - camera._domain_center) + 0.5 * np.linalg.norm(
             camera._domain_width)
-        # Rescale the ray to be long enough to cover the entire domain
+        # scaled so that t = 1 reaches beyond the domain's farthest point
         vectors = (sample_x + sample_y + normal_vecs * camera.width[2]) * (
             max_length / camera.width[2])
         positions = np.tile(camera.position, res0 * res1).reshape(res0, res1, 3)

@@ -31,42 +31,46 @@ from .transfer_functions import (
 
 
 class ZBuffer:
-    """zbuffer_array.py:5-80: an image and its per-pixel depth; `a + b`
-    keeps, per pixel, the nearer of the two."""
+    """An image with a depth per pixel (zbuffer_array.py:5-80).  Adding two
+    keeps, pixel by pixel, the colour of the nearer one: with 2-d depth maps
+    a pixel at equal depth in both becomes 0; with one-column depth maps the
+    two are blended with weights 1 and 0, the second winning ties
+    (zbuffer_array.py:44-57)."""
 
     def __init__(self, rgba, z):
-        assert rgba.shape[: len(z.shape)] == z.shape
+        if rgba.shape[: z.ndim] != z.shape:
+            raise AssertionError("ZBuffer: the image and its depths differ in shape")
         self.rgba = rgba
         self.z = z
         self.shape = z.shape
 
     def __add__(self, other):
-        assert self.shape == other.shape
-        f = self.z < other.z
+        if self.shape != other.shape:
+            raise AssertionError("ZBuffer: adding buffers of different shapes")
+        mine = self.z < other.z
         if self.z.shape[1] == 1:
-            rgba = self.rgba * f[:, None, :]
-            rgba += other.rgba * (1.0 - f)[:, None, :]
+            w = mine[:, None, :].astype(np.float64)
+            rgba = self.rgba * w + other.rgba * (1.0 - w)
         else:
-            b = self.z > other.z
+            extra = (1,) * (self.rgba.ndim - mine.ndim)  # per-pixel masks over the channels
             rgba = np.zeros(self.rgba.shape)
-            rgba[f] = self.rgba[f]
-            rgba[b] = other.rgba[b]
-        z = np.min([self.z, other.z], axis=0)
-        return ZBuffer(rgba, z)
+            np.copyto(rgba, self.rgba, where=mine.reshape(mine.shape + extra))
+            np.copyto(rgba, other.rgba, where=(other.z < self.z).reshape(mine.shape + extra))
+        return ZBuffer(rgba, np.minimum(self.z, other.z))
 
     def __iadd__(self, other):
-        tmp = self + other
-        self.rgba = tmp.rgba
-        self.z = tmp.z
+        merged = self + other
+        self.rgba, self.z = merged.rgba, merged.z
         return self
 
     def __eq__(self, other):
-        return bool(np.all(self.rgba == other.rgba) and np.all(self.z == other.z))
+        return bool((self.rgba == other.rgba).all() and (self.z == other.z).all())
 
     def paint(self, ind, value, z):
-        if z < self.z[ind]:
-            self.rgba[ind] = value
+        """Draws value at pixel ind if z is nearer than what is there."""
+        if self.z[ind] > z:
             self.z[ind] = z
+            self.rgba[ind] = value
 
 
 # ---------------------------------------------------------------------------
@@ -88,19 +92,21 @@ def rescale(image, cmax=None, amax=None):
     return out
 
 
+_BACKGROUNDS = {"white": (1.0, 1.0, 1.0, 1.0), "black": (0.0, 0.0, 0.0, 1.0)}
+
+
 def add_background_color(image, background="black"):
-    """ImageArray.add_background_color (image_array.py:138-189), on a copy."""
-    if background is None:
-        background = (0.0, 0.0, 0.0, 0.0)
-    elif isinstance(background, str) and background == "white":
-        background = (1.0, 1.0, 1.0, 1.0)
-    elif isinstance(background, str) and background == "black":
-        background = (0.0, 0.0, 0.0, 1.0)
+    """ImageArray.add_background_color (image_array.py:138-189), on a copy:
+    the image, its colours weighted by its alpha, over an opaque or
+    transparent background colour ("white", "black", None for transparent
+    black, or an RGBA tuple)."""
+    bg = (0.0, 0.0, 0.0, 0.0) if background is None else _BACKGROUNDS.get(background, background)
+    alpha = image[:, :, 3]
+    behind = 1.0 - alpha
     out = np.array(image, copy=True)
-    for i in range(3):
-        out[:, :, i] = image[:, :, i] * image[:, :, 3]
-        out[:, :, i] += background[i] * background[3] * (1.0 - image[:, :, 3])
-    out[:, :, 3] = image[:, :, 3] + background[3] * (1.0 - image[:, :, 3])
+    for c in range(3):
+        out[:, :, c] = image[:, :, c] * alpha + (bg[c] * bg[3]) * behind
+    out[:, :, 3] = alpha + bg[3] * behind
     return out
 
 
