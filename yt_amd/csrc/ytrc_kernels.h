@@ -546,12 +546,15 @@ struct VRSampler {
         float cv[8][NFP];
         load_corners<NFP>(data + (size_t)voff * NFP, cv);
         __builtin_amdgcn_sched_barrier(0);
-        // dt = (exit_t - enter_t) / n_samples; positions from FP64 enter_t
-        // in the order of image_samplers.pyx:475-479.
+        // dt = (exit_t - enter_t) / n_samples, as a product with 1/n_samples.
         const double dt = (ex - en) * inv_ns;
         const double tt = en + 0.5 * dt;
-        // Cell-relative position (tt*v_dir + v_pos - (idx*dds + LE)) * idds,
-        // as tt*pa + pb - idx with pa = v_dir*idds, pb = (v_pos - LE)*idds.
+        // The reference's cell-relative position is
+        // (tt*v_dir + v_pos - (idx*dds + LE)) * idds (image_samplers.pyx:
+        // 475-479).  Here it is refactored as fma(tt, pa, pb) - idx with
+        // pa = v_dir*idds and pb = (v_pos - LE)*idds formed once per brick:
+        // the same value up to a few FP64 roundings, far below the FP32 it
+        // is converted to.
         // The sub-sample step v_dir * idds * dt in FP32 (paf = (float)pa,
         // per brick): it only spaces the sub-samples within the cell.
         const float dtf = (float)dt;
