@@ -225,6 +225,21 @@ int ytrc_batch_launch(ytrc_batch *b);
 /* Waits for the last launch; ms_out (may be NULL) gets the GPU time between
  * the HIP events recorded around it on the same stream. */
 int ytrc_batch_sync(ytrc_batch *b, float *ms_out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): renders one request with the persistent tile-queue
+ * kernels, exactly as a YTRC_BATCH_PERSISTENT batch of that request does,
+ * and copies its image to image_out (float64, nv0*nv1*4, as
+ * ytrc_batch_read_image), calling no hipMalloc/hipFree.  Every staging buffer
+ * (descriptor, brick order, extents, z-buffer, input image, output planes,
+ * transfer-function table, tile list, queue counters) lives in one fixed
+ * device array of YTRC_NOALLOC_SCRATCH_BYTES that the library holds on each
+ * device until the process ends.  A request that needs more is refused
+ * (YTRC_EINVAL), as are record and partial-map requests.  Synchronous.
+ * Calls on one device share its array, which keeps the last call's contents;
+ * the library serialises them (a call waits for the one before it). */
+#define YTRC_NOALLOC_SCRATCH_BYTES (64u << 20)
+int ytrc_render_noalloc(ytrc_ctx *ctx, const ytrc_request *r, double *image_out);
 /* Copies request i's image to image_out (float64, nv0*nv1*4, the reference's
  * aimage layout, which cast_through_kdtree updates in float64,
  * image_samplers.pyx:177-196); channels the sampler does not write keep

@@ -74,6 +74,7 @@ EXPORTED = [
     "ytrc_server_set_policy",
     "ytrc_ctx_max_lds",
     "ytrc_ctx_device_name",
+    "ytrc_render_noalloc",
     "ytrc_server_stats",
     "ytrc_server_destroy",
     "ytrc_dataset_build_vr",
@@ -238,6 +239,7 @@ def _declare(lib):
     lib.ytrc_server_set_policy.argtypes = [P, C.c_int32, C.c_double]
     lib.ytrc_ctx_max_lds.argtypes = [P, C.POINTER(C.c_uint64)]
     lib.ytrc_ctx_device_name.argtypes = [P, C.c_char_p, C.c_int32]
+    lib.ytrc_render_noalloc.argtypes = [P, C.POINTER(Request), C.POINTER(C.c_double)]
     lib.ytrc_server_stats.argtypes = [P, I64P]
     lib.ytrc_server_destroy.argtypes = [P]
     lib.ytrc_server_destroy.restype = None
@@ -365,6 +367,15 @@ class Context:
         buf = C.create_string_buffer(256)
         _check(lib().ytrc_ctx_device_name(self._h, buf, len(buf)))
         return buf.value.decode(errors="replace")
+
+    def render_noalloc(self, spec):
+        """One request through ytrc_render_noalloc (no device allocation:
+        the library's fixed per-device array); returns its (nv0, nv1, 4)
+        float64 image."""
+        out = np.empty((spec.nv[0], spec.nv[1], 4), dtype=np.float64)
+        r = spec.to_c()
+        _check(lib().ytrc_render_noalloc(self._h, C.byref(r), _dptr(out)))
+        return out
 
     def hbm_probe(self, nbytes=1 << 30, iters=20):
         """(GB/s, ms per copy) of a streaming device copy of nbytes."""

@@ -33,6 +33,14 @@
 #include "ytrc_kernels.h"
 #include "ytrc_pick.h"
 
+// Device memory of ytrc_render_noalloc: every staging buffer of one request
+// (its descriptor, brick order, extents, z-buffer, input image, output
+// planes, transfer-function table, tile list and queue counters).  Each
+// device holds its own copy until the process ends.  In host code the name
+// `scratch` is not a device address: host code uses only the address that
+// hipGetSymbolAddress returns on each device (ScratchSlot::ptr).
+static __device__ __align__(256) char scratch[YTRC_NOALLOC_SCRATCH_BYTES];
+
 using namespace ytrc;
 
 // ---------------------------------------------------------------------------
@@ -269,6 +277,8 @@ struct ytrc_ctx {
     int n_cu = 0;
     size_t max_lds = 64 * 1024;  // LDS bytes one workgroup may allocate
     std::string name;            // marketing name and ISA of the device
+    char *scratch = nullptr;     // this device's `scratch` array (ytrc_render_noalloc)
+    std::mutex *scratch_mu = nullptr;  // serialises the calls that share it
     std::map<int, Dataset> datasets;
     int next_ds = 1;
     std::mutex mu;
@@ -554,6 +564,35 @@ int dataset_commit(ytrc_ctx *ctx, Dataset &ds, int32_t *ds_id_out)
 
 }  // namespace
 
+namespace {
+
+// The address of `scratch` on each device, looked up once per device with
+// that device current, and a mutex per device for the calls sharing it.
+struct ScratchSlot {
+    char *ptr = nullptr;
+    std::mutex mu;
+};
+std::mutex g_scratch_map_mu;
+std::map<int, ScratchSlot> g_scratch;
+
+int scratch_setup(ytrc_ctx *c)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_map_mu);
+    ScratchSlot &slot = g_scratch[c->device];
+    if (slot.ptr == nullptr) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || ptr == nullptr)
+            return fail(YTRC_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        slot.ptr = (char *)ptr;
+    }
+    c->scratch = slot.ptr;
+    c->scratch_mu = &slot.mu;
+    return YTRC_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *ytrc_last_error(void) { return g_err.c_str(); }
@@ -583,6 +622,11 @@ int ytrc_ctx_create(int32_t device, uint64_t budget_bytes, ytrc_ctx **out)
     if (e != hipSuccess) {
         delete c;
         return fail(YTRC_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    if (scratch_setup(c) != YTRC_OK) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return YTRC_EDEVICE;
     }
     *out = c;
     return YTRC_OK;
@@ -1548,6 +1592,76 @@ int ytrc_batch_launch(ytrc_batch *b)
     HIPCHK(hipEventRecord(b->ev1, s));
     b->launched += 1;
     return YTRC_OK;
+}
+
+int ytrc_render_noalloc(ytrc_ctx *ctx, const ytrc_request *req, double *image_out)
+{
+    if (ctx == nullptr || req == nullptr || image_out == nullptr)
+        return fail(YTRC_EINVAL, "ytrc_render_noalloc: NULL argument");
+    if (req->flags & (YTRC_FLAG_RECORD | YTRC_FLAG_PARTIAL))
+        return fail(YTRC_EINVAL, "ytrc_render_noalloc: record and partial-map requests need the batch "
+                                 "API (their extra outputs are not staged in the fixed array)");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->scratch == nullptr || ctx->scratch_mu == nullptr)
+        return fail(YTRC_EDEVICE, "ytrc_render_noalloc: no scratch address for this device");
+    std::lock_guard<std::mutex> sl(*ctx->scratch_mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    PrepReq P;
+    int rc = prep_request(ctx, *req, "ytrc_render_noalloc: ", P, TILE_SHIFT_DEFAULT);
+    if (rc != YTRC_OK) return rc;
+    // Layout in the array: the request's region (as in a batch), its
+    // transfer-function table, its tile list, the queue counters and its
+    // descriptor.
+    size_t off = align_up(P.bytes, 256);
+    const size_t tf_off = off;
+    off = align_up(off + sizeof(float4) * std::max<size_t>(1, P.tft.size()), 256);
+    const size_t items_off = off;
+    off = align_up(off + sizeof(int2) * (size_t)std::max(1, P.q.ntiles), 256);
+    const size_t cnt_off = off;
+    off = align_up(off + sizeof(unsigned int) * QSPLIT, 256);
+    const size_t img_off = off;
+    off = align_up(off + sizeof(DImg), 256);
+    if (off > (size_t)YTRC_NOALLOC_SCRATCH_BYTES)
+        return fail(YTRC_EINVAL, "ytrc_render_noalloc: the request needs " + std::to_string(off) +
+                                     " B of device staging, more than the " +
+                                     std::to_string((size_t)YTRC_NOALLOC_SCRATCH_BYTES) +
+                                     " B fixed array; use ytrc_batch_create");
+    char *S = ctx->scratch;
+    DImg d;
+    std::vector<float> stage;
+    rc = stage_request(ctx, P, S, (const float4 *)(S + tf_off), 0, d, stage);
+    if (rc != YTRC_OK) return rc;
+    if (!P.tft.empty())
+        HIPCHK(hipMemcpy(S + tf_off, P.tft.data(), sizeof(float4) * P.tft.size(), hipMemcpyHostToDevice));
+    // One request: the tiles in order (the QSPLIT queue ranges are its
+    // QSPLIT contiguous chunks, as in a persistent batch of this request).
+    std::vector<int2> items((size_t)P.q.ntiles);
+    for (int t = 0; t < P.q.ntiles; t++) items[(size_t)t] = make_int2(0, t);
+    if (!items.empty())
+        HIPCHK(hipMemcpy(S + items_off, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(S + img_off, &d, sizeof(DImg), hipMemcpyHostToDevice));
+    const pfn_t fn = pick_persist(req->kind, P.q.nfp, false, P.q.mode);
+    if (fn == nullptr) return fail(YTRC_ENOTIMPL, "ytrc_render_noalloc: no kernel for this request");
+    const size_t lds = has_tf(req->kind) ? sizeof(float4) * P.tft.size() : 0;
+    if (lds > 64 * 1024)
+        HIPCHK(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipStream_t s = ctx->stream;
+    unsigned int *counters = (unsigned int *)(S + cnt_off);
+    // the array keeps the previous call's counters: clear them
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(256), 0, s, counters, QSPLIT);
+    HIPCHK(hipGetLastError());
+    if (P.q.ntiles > 0) {
+        const int wpg = 4, threads = 64 * wpg;
+        int nb = 0;
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, threads, lds));
+        const int grid = std::max(1, std::min(std::max(1, nb) * ctx->n_cu, (P.q.ntiles + wpg - 1) / wpg));
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(threads), lds, s, (const DImg *)(S + img_off),
+                           (const int2 *)(S + items_off), P.q.ntiles, counters,
+                           (const float4 *)(S + tf_off), (int)P.tft.size());
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return read_request_image(P.q, d, image_out);
 }
 
 int ytrc_batch_request_ms(ytrc_batch *b, int32_t i, float *ms)
