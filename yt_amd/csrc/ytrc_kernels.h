@@ -164,6 +164,126 @@ __device__ __forceinline__ void walk_projection(const DBrick &b, Walk &w, double
     }
 }
 
+// Loads of one point's NFP fields through a raw buffer resource: an offset
+// at or beyond the resource's byte count reads 0 (the hardware's range
+// check), so a pipelined walk gives the steps past a ray's end the offset
+// 0xffffffff instead of a branch around their loads.
+template <int NFP>
+struct BufLoad;
+template <>
+struct BufLoad<1> {
+    static __device__ __forceinline__ void get(__amdgpu_buffer_rsrc_t r, unsigned off, float v[1])
+    {
+        v[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    }
+};
+template <>
+struct BufLoad<2> {
+    static __device__ __forceinline__ void get(__amdgpu_buffer_rsrc_t r, unsigned off, float v[2])
+    {
+        const auto t = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+        v[0] = __uint_as_float(t[0]);
+        v[1] = __uint_as_float(t[1]);
+    }
+};
+template <>
+struct BufLoad<4> {
+    static __device__ __forceinline__ void get(__amdgpu_buffer_rsrc_t r, unsigned off, float v[4])
+    {
+        const auto t = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+#pragma unroll
+        for (int f = 0; f < 4; f++) v[f] = __uint_as_float(t[f]);
+    }
+};
+
+// A raw buffer resource (gfx9 layout: base, stride 0, byte count, and the
+// third dword's 32-bit data format) over `bytes` bytes at p.  Both are the
+// same in every lane (one brick per wave) and are made scalar, so that the
+// resource sits in SGPRs.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t brick_rsrc(const float *p, unsigned bytes)
+{
+    const uint64_t a = (uint64_t)p;
+    const uint64_t lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)a);
+    const uint64_t hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(a >> 32));
+    return __builtin_amdgcn_make_buffer_rsrc((void *)((hi << 32) | lo), (short)0,
+                                             __builtin_amdgcn_readfirstlane((int)bytes), 0x00020000);
+}
+
+// walk_projection for the common case, taken per wave and brick when
+// every walking lane starts with all three cell indices inside the brick
+// (so that, stepping one axis at a time and stopping when that axis leaves
+// the brick, every visit is inside it), the brick's points span fewer than
+// 2^32 bytes and its 4x4x4-block strides fit 24-bit multiplies:
+// * no per-visit bounds test (walk_projection keeps it for rays that start
+//   within 1e-10 dds outside a face, whose indices stay out of range on the
+//   axis they never step);
+// * the blocked offset as 16 c0 + 4 c1 + c2 + (c0>>2) K0 + (c1>>2) K1 +
+//   (c2>>2) 60, with K0 = 64 (nbj nbk - 1), K1 = 64 nbk - 16 (the same
+//   value as blocked_offset, in fewer operations);
+// * the steps of a round run for every lane of the wave; a lane past its
+//   ray's end gives its load the out-of-range offset (reads 0) and its
+//   length 0;
+// * each round's visits are summed in visit order with fused multiply-adds
+//   and the round's sum is added to the image with Kahan compensation.
+template <int NFP>
+__device__ __forceinline__ void walk_projection_fast(Walk &w, double max_t, ProjSampler<NFP, false> &S,
+                                                     __amdgpu_buffer_rsrc_t rsrc, unsigned K0, unsigned K1)
+{
+    constexpr int PF = 12;
+    double t0 = w.tmax[0], t1 = w.tmax[1], t2 = w.tmax[2];
+    int c0 = w.cur[0], c1 = w.cur[1], c2 = w.cur[2];
+    double enter_t = w.enter_t;
+    const int st0 = w.step[0], st1 = w.step[1], st2 = w.step[2];
+    const double td0 = w.tdelta[0], td1 = w.tdelta[1], td2 = w.tdelta[2];
+    bool live = true;
+    while (live) {
+        unsigned off[PF];
+        float dl[PF];
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const bool a01 = t0 < t1;
+            const bool a02 = t0 < t2;
+            const bool a12 = t1 < t2;
+            const bool s0 = a01 & a02;
+            const bool s1 = !a01 & a12;
+            const bool s2 = !(s0 | s1);
+            const double tm = s0 ? t0 : (s1 ? t1 : t2);
+            // exit_t = fmin(tm, max_t), and the loop's enter_t >= max_t
+            // test on the next enter_t (= tm), from one comparison
+            const bool ge = tm >= max_t;
+            const double exit_t = ge ? max_t : tm;
+            const unsigned o = __umul24((unsigned)c0 >> 2, K0) + __umul24((unsigned)c1 >> 2, K1) +
+                               __umul24((unsigned)c2 >> 2, 60u) + ((unsigned)c0 << 4) +
+                               ((unsigned)c1 << 2) + (unsigned)c2;
+            off[u] = live ? o * (unsigned)(4 * NFP) : 0xffffffffu;
+            dl[u] = live ? (float)(exit_t - enter_t) : 0.0f;
+            c0 += s0 ? st0 : 0;
+            c1 += s1 ? st1 : 0;
+            c2 += s2 ? st2 : 0;
+            t0 = s0 ? t0 + td0 : t0;
+            t1 = s1 ? t1 + td1 : t1;
+            t2 = s2 ? t2 + td2 : t2;
+            const bool out = (s0 & ((unsigned)c0 >= (unsigned)S.dims0)) |
+                             (s1 & ((unsigned)c1 >= (unsigned)S.d1)) |
+                             (s2 & ((unsigned)c2 >= (unsigned)S.d2));
+            enter_t = tm;
+            live = live & !(out | ge);
+        }
+        float v[PF][NFP];
+#pragma unroll
+        for (int u = 0; u < PF; u++) BufLoad<NFP>::get(rsrc, off[u], v[u]);
+        float r[NFP];
+#pragma unroll
+        for (int f = 0; f < NFP; f++) r[f] = 0.0f;
+#pragma unroll
+        for (int u = 0; u < PF; u++)
+#pragma unroll
+            for (int f = 0; f < NFP; f++) r[f] = __builtin_fmaf(v[u][f], dl[u], r[f]);
+#pragma unroll
+        for (int f = 0; f < NFP; f++) kahan_add(S.s[f], S.c[f], r[f]);
+    }
+}
+
 template <int N>
 __device__ __forceinline__ float pick(const float (&a)[N], int i)
 {
@@ -881,7 +1001,22 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
                     s.c[f] = pc[f];
                 }
                 s.rec = &rec;
-                walk_projection(b, w, max_t, s);
+                bool fastw = false;
+                if constexpr (!REC && NFP <= 4) {
+                    // the fast walk's conditions (walk_projection_fast)
+                    const unsigned nb0 = ((unsigned)s.dims0 + 3u) >> 2;
+                    const uint64_t bytes = (uint64_t)nb0 * (unsigned)s.nbj * (unsigned)s.nbk * 64u * 4u * NFP;
+                    const uint64_t k0 = (uint64_t)s.nbj * (unsigned)s.nbk * 64u;
+                    const bool inside = (unsigned)w.cur[0] < (unsigned)s.dims0 &&
+                                        (unsigned)w.cur[1] < (unsigned)s.d1 &&
+                                        (unsigned)w.cur[2] < (unsigned)s.d2;
+                    if (bytes < (1ull << 32) && k0 < (1u << 24) && __all(inside)) {
+                        walk_projection_fast<NFP>(w, max_t, s, brick_rsrc(s.data, (unsigned)bytes),
+                                                  (unsigned)k0 - 64u, (unsigned)s.nbk * 64u - 16u);
+                        fastw = true;
+                    }
+                }
+                if (!fastw) walk_projection(b, w, max_t, s);
 #pragma unroll
                 for (int f = 0; f < NFP; f++) {
                     ps[f] = s.s[f];
