@@ -241,28 +241,42 @@ __device__ __forceinline__ void walk_projection_fast(Walk &w, double max_t, Proj
         float dl[PF];
 #pragma unroll
         for (int u = 0; u < PF; u++) {
+            // The reference's axis choice (tmax[0] < tmax[1] ? (tmax[0] <
+            // tmax[2] ? 0 : 2) : (tmax[1] < tmax[2] ? 1 : 2)) from two
+            // comparisons: m = the first pair's choice, then m against t2.
             const bool a01 = t0 < t1;
-            const bool a02 = t0 < t2;
-            const bool a12 = t1 < t2;
-            const bool s0 = a01 & a02;
-            const bool s1 = !a01 & a12;
-            const bool s2 = !(s0 | s1);
-            const double tm = s0 ? t0 : (s1 ? t1 : t2);
+            const double m = a01 ? t0 : t1;
+            const bool bm = m < t2;
+            const bool s0 = a01 & bm;
+            const bool s1 = !a01 & bm;
+            const double tm = bm ? m : t2;
             // exit_t = fmin(tm, max_t), and the loop's enter_t >= max_t
             // test on the next enter_t (= tm), from one comparison
             const bool ge = tm >= max_t;
             const double exit_t = ge ? max_t : tm;
-            const unsigned o = __umul24((unsigned)c0 >> 2, K0) + __umul24((unsigned)c1 >> 2, K1) +
-                               __umul24((unsigned)c2 >> 2, 60u) + ((unsigned)c0 << 4) +
-                               ((unsigned)c1 << 2) + (unsigned)c2;
+            const unsigned lin = ((unsigned)c0 << 4) + ((unsigned)c1 << 2) + (unsigned)c2;
+            const unsigned o = __umul24((unsigned)c0 >> 2, K0) +
+                               (__umul24((unsigned)c1 >> 2, K1) + (__umul24((unsigned)c2 >> 2, 60u) + lin));
             off[u] = live ? o * (unsigned)(4 * NFP) : 0xffffffffu;
             dl[u] = live ? (float)(exit_t - enter_t) : 0.0f;
-            c0 += s0 ? st0 : 0;
-            c1 += s1 ? st1 : 0;
-            c2 += s2 ? st2 : 0;
-            t0 = s0 ? t0 + td0 : t0;
-            t1 = s1 ? t1 + td1 : t1;
-            t2 = s2 ? t2 + td2 : t2;
+            // Step the chosen axis under an execution mask (the empty asm
+            // keeps each update a masked region rather than selects).
+            if (s0) {
+                asm volatile("");
+                c0 += st0;
+                t0 += td0;
+            }
+            if (s1) {
+                asm volatile("");
+                c1 += st1;
+                t1 += td1;
+            }
+            const bool s2 = !(s0 | s1);
+            if (s2) {
+                asm volatile("");
+                c2 += st2;
+                t2 += td2;
+            }
             const bool out = (s0 & ((unsigned)c0 >= (unsigned)S.dims0)) |
                              (s1 & ((unsigned)c1 >= (unsigned)S.d1)) |
                              (s2 & ((unsigned)c2 >= (unsigned)S.d2));
