@@ -265,28 +265,46 @@ __device__ __forceinline__ int walk_loop(const DBrick &b, Walk &w, double max_t,
     while (true) {
         hit += 1;
         // The axis of the smallest tmax, ties to the later axis as the
-        // reference's comparisons order it (grid_traversal.pyx:120-130), as
-        // three lane masks (no divergent branches, no axis index).
+        // reference's comparisons order it (grid_traversal.pyx:120-130):
+        // tmax[0] < tmax[1] ? (tmax[0] < tmax[2] ? 0 : 2) : (tmax[1] <
+        // tmax[2] ? 1 : 2), from two comparisons (m = the first pair's
+        // choice, then m against tmax[2]) as lane masks (no axis index).
         const bool a01 = t0 < t1;
-        const bool a02 = t0 < t2;
-        const bool a12 = t1 < t2;
-        const bool s0 = a01 & a02;
-        const bool s1 = !a01 & a12;
+        const double m = a01 ? t0 : t1;
+        const bool bm = m < t2;
+        const bool s0 = a01 & bm;
+        const bool s1 = !a01 & bm;
         const bool s2 = !(s0 | s1);
-        const double tm = s0 ? t0 : (s1 ? t1 : t2);
-        const double exit_t = y_fmin(tm, max_t);
+        const double tm = bm ? m : t2;
+        // exit_t = fmin(tm, max_t) and the loop's enter_t >= max_t test on
+        // the next enter_t (= tm), from one comparison
+        const bool ge = tm >= max_t;
+        const double exit_t = ge ? max_t : tm;
         sample(enter_t, exit_t, c0, c1, c2);
-        // Step the chosen axis and test only its new index.
-        c0 += s0 ? w.step[0] : 0;
-        c1 += s1 ? w.step[1] : 0;
-        c2 += s2 ? w.step[2] : 0;
-        t0 = s0 ? t0 + w.tdelta[0] : t0;
-        t1 = s1 ? t1 + w.tdelta[1] : t1;
-        t2 = s2 ? t2 + w.tdelta[2] : t2;
-        const bool out = (s0 & ((unsigned)c0 >= (unsigned)D0)) | (s1 & ((unsigned)c1 >= (unsigned)D1)) |
-                         (s2 & ((unsigned)c2 >= (unsigned)D2));
+        // Step the chosen axis under an execution mask (the empty asm keeps
+        // each update a masked region rather than selects) and test only
+        // its new index.
+        bool out = false;
+        if (s0) {
+            asm volatile("");
+            c0 += w.step[0];
+            t0 += w.tdelta[0];
+            out = (unsigned)c0 >= (unsigned)D0;
+        }
+        if (s1) {
+            asm volatile("");
+            c1 += w.step[1];
+            t1 += w.tdelta[1];
+            out = (unsigned)c1 >= (unsigned)D1;
+        }
+        if (s2) {
+            asm volatile("");
+            c2 += w.step[2];
+            t2 += w.tdelta[2];
+            out = (unsigned)c2 >= (unsigned)D2;
+        }
         enter_t = tm;
-        if (out || enter_t >= max_t) break;
+        if (out || ge) break;
     }
     return hit;
 }
