@@ -82,6 +82,27 @@ def alg_bytes(kind, samples, npix, nf=1, n_samples=10):
     return samples * (8.0 * nf * 4.0 + 1.0) / n_samples + npix * 3 * 4.0
 
 
+# MI355X FP32 vector peak 157.3 TFLOP/s = 7.86e13 lane FMAs per second: the
+# rate at which the VALUs retire lane operations when every instruction is a
+# full-rate FP32 one (FP64 operations issue at half that rate).
+VALU_LANE_OPS_PEAK = 157.3e12 / 2.0
+
+
+def measured_valu(path, kind, args):
+    """VALU instructions per dispatch of the kernel of `kind` and its
+    VALUUtilization from the committed rocprofv3 --pmc summary, when taken
+    on this same workload; None otherwise."""
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    want = {"n": args.n, "psize": args.psize, "cams": args.cams}
+    if any(t.get("workload", {}).get(k) != v for k, v in want.items()):
+        return None
+    return t.get("valu", {}).get(str(kind))
+
+
 def measured_traffic(path, kind, args):
     """HBM bytes per dispatch of the dominant kernel from a committed
     rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE in KiB per dispatch,
@@ -478,6 +499,23 @@ def main():
     dom = max(kinds, key=lambda k: kinds[k]["ms"])
     ach = kinds[dom]["bytes"] / kinds[dom]["ms"] / 1e6
     traffic = measured_traffic(args.traffic_file, dom, args)
+    # The binding roof of both kernels is instruction issue: lane-VALU
+    # operations per sample and their rate against the FP32 lane peak.
+    valu = {}
+    for kind in kinds:
+        v = measured_valu(args.traffic_file, kind, args)
+        if v is None:
+            continue
+        lane_ops = v["sq_insts_valu"] * 64.0
+        valu["vr" if kind == 1 else "projection"] = {
+            "lane_valu_per_sample": round(lane_ops / kinds[kind]["samples"], 2),
+            "lane_ops_per_s": round(lane_ops / (kinds[kind]["ms"] * 1e-3), 1),
+            "frac_of_fp32_lane_peak": round(lane_ops / (kinds[kind]["ms"] * 1e-3) / VALU_LANE_OPS_PEAK, 4),
+            "valu_utilization_pct": v["valu_utilization_pct"],
+            "insts_per_launch": v["sq_insts_valu"],
+        }
+    for k, v in valu.items():
+        log(f"{k} kernel VALU: {v}")
 
     # Plain: one launch + sync per image.
     plain = None
@@ -636,6 +674,10 @@ def main():
             "algorithmic_bytes": int(kinds[dom]["bytes"]),
             "samples": int(kinds[dom]["samples"]),
             "copy_probe_gbps": round(copy_gbps, 1),
+            "valu": valu or None,
+            "projection_kernel": {"kernel_ms": round(kinds[0]["ms"], 4),
+                                  "achieved": round(kinds[0]["bytes"] / kinds[0]["ms"] / 1e6, 1),
+                                  "frac": round(kinds[0]["bytes"] / kinds[0]["ms"] / 1e6 / HBM_PEAK_GBS, 4)},
         },
         "cpu_baseline": cpu,
     }
