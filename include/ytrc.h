@@ -35,6 +35,7 @@ extern "C" {
 #define YTRC_ENOTIMPL (-4)  /* e.g. unknown volume method (image_samplers.pyx:137-141)    */
 #define YTRC_ECANCELLED (-5)
 #define YTRC_EBUSY (-6)     /* the dataset is still used by a batch or a server request    */
+#define YTRC_ETIMEDOUT (-7) /* a wait's deadline passed before the work finished           */
 
 /* Sampler kinds: ProjectionSampler (image_samplers.pyx:327-343),
  * VolumeRenderSampler (image_samplers.pyx:405-494) and
@@ -225,6 +226,16 @@ int ytrc_batch_launch(ytrc_batch *b);
 /* Waits for the last launch; ms_out (may be NULL) gets the GPU time between
  * the HIP events recorded around it on the same stream. */
 int ytrc_batch_sync(ytrc_batch *b, float *ms_out);
+/* ytrc_batch_sync with a deadline: YTRC_ETIMEDOUT when the launch is still
+ * running after timeout_ms (it keeps running; sync again or cancel). */
+int ytrc_batch_sync_for(ytrc_batch *b, double timeout_ms, float *ms_out);
+/* Asks the running launch to stop, the analog of the reference's
+ * PyErr_CheckSignals every 1000 rays (image_samplers.pyx:188-190): every
+ * wave reads the batch's stop flag (host-pinned) once per tile and exits at
+ * its next tile; the next sync returns YTRC_ECANCELLED and the images stay
+ * unreadable until a later launch completes.  Safe to call from another
+ * thread while one waits in ytrc_batch_sync. */
+int ytrc_batch_cancel(ytrc_batch *b);
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): renders one request with the persistent tile-queue
@@ -290,6 +301,11 @@ int ytrc_server_submit(ytrc_server *s, const ytrc_request *r, int64_t *id_out);
 int ytrc_server_progress(ytrc_server *s, int64_t *done_ids, int32_t cap, int32_t *n_done);
 /* As progress, but first waits for the oldest chunk in flight. */
 int ytrc_server_wait(ytrc_server *s, int64_t *done_ids, int32_t cap, int32_t *n_done);
+/* ytrc_server_wait with a deadline: YTRC_ETIMEDOUT (n_done = 0) when no
+ * request finished within timeout_ms; YTRC_OK with n_done = 0 when nothing
+ * is queued or running. */
+int ytrc_server_wait_for(ytrc_server *s, double timeout_ms, int64_t *done_ids, int32_t cap,
+                         int32_t *n_done);
 /* Copies a finished request's image (float64 nv0*nv1*4) and releases it. */
 int ytrc_server_read(ytrc_server *s, int64_t id, double *image_out);
 /* Same, as the float32 planes the kernels write (channel-major,

@@ -21,6 +21,7 @@ YTRC_EBUDGET = -3
 YTRC_ENOTIMPL = -4
 YTRC_ECANCELLED = -5
 YTRC_EBUSY = -6
+YTRC_ETIMEDOUT = -7
 
 PROJECTION = 0
 VOLUME_RENDER = 1
@@ -75,6 +76,9 @@ EXPORTED = [
     "ytrc_ctx_max_lds",
     "ytrc_ctx_device_name",
     "ytrc_render_noalloc",
+    "ytrc_batch_sync_for",
+    "ytrc_batch_cancel",
+    "ytrc_server_wait_for",
     "ytrc_server_stats",
     "ytrc_server_destroy",
     "ytrc_dataset_build_vr",
@@ -94,6 +98,14 @@ class YtrcError(RuntimeError):
 
 class YtrcBudgetError(YtrcError):
     """The request does not fit the context's HBM budget."""
+
+
+class YtrcCancelled(YtrcError):
+    """The launch was stopped (Batch.cancel); its images are incomplete."""
+
+
+class YtrcTimeout(YtrcError, TimeoutError):
+    """A wait's deadline passed before the work finished."""
 
 
 class BrickDesc(C.Structure):
@@ -208,6 +220,8 @@ def _declare(lib):
     lib.ytrc_batch_create.argtypes = [P, C.c_int32, C.POINTER(Request), C.c_uint32, C.POINTER(P)]
     lib.ytrc_batch_launch.argtypes = [P]
     lib.ytrc_batch_sync.argtypes = [P, C.POINTER(C.c_float)]
+    lib.ytrc_batch_sync_for.argtypes = [P, C.c_double, C.POINTER(C.c_float)]
+    lib.ytrc_batch_cancel.argtypes = [P]
     lib.ytrc_batch_read_image.argtypes = [P, C.c_int32, C.POINTER(C.c_double)]
     lib.ytrc_batch_read_partial.argtypes = [P, C.c_int32, C.POINTER(C.c_double)]
     lib.ytrc_kdtree_build.argtypes = [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -232,6 +246,7 @@ def _declare(lib):
     lib.ytrc_server_submit.argtypes = [P, C.POINTER(Request), I64P]
     lib.ytrc_server_progress.argtypes = [P, I64P, C.c_int32, C.POINTER(C.c_int32)]
     lib.ytrc_server_wait.argtypes = [P, I64P, C.c_int32, C.POINTER(C.c_int32)]
+    lib.ytrc_server_wait_for.argtypes = [P, C.c_double, I64P, C.c_int32, C.POINTER(C.c_int32)]
     lib.ytrc_server_read.argtypes = [P, C.c_int64, C.POINTER(C.c_double)]
     lib.ytrc_server_read_planar.argtypes = [P, C.c_int64, C.POINTER(C.c_float), C.c_int32,
                                             C.POINTER(C.c_int32)]
@@ -277,6 +292,10 @@ def _check(rc):
             raise NotImplementedError(msg)
         if rc == YTRC_EINVAL:
             raise ValueError(msg)
+        if rc == YTRC_ECANCELLED:
+            raise YtrcCancelled(rc, msg)
+        if rc == YTRC_ETIMEDOUT:
+            raise YtrcTimeout(rc, msg)
         raise YtrcError(rc, msg)
 
 
@@ -647,10 +666,38 @@ class Batch:
     def launch(self):
         _check(lib().ytrc_batch_launch(self._h))
 
-    def sync(self):
+    def sync(self, timeout=None):
+        """Waits for the last launch; returns its GPU ms.  With a timeout
+        (seconds) raises YtrcTimeout if it is still running then; raises
+        YtrcCancelled if it was stopped."""
         ms = C.c_float()
-        _check(lib().ytrc_batch_sync(self._h, C.byref(ms)))
+        if timeout is None:
+            _check(lib().ytrc_batch_sync(self._h, C.byref(ms)))
+        else:
+            _check(lib().ytrc_batch_sync_for(self._h, float(timeout) * 1e3, C.byref(ms)))
         return ms.value
+
+    def cancel(self):
+        """Asks the running launch to stop at every wave's next tile."""
+        _check(lib().ytrc_batch_cancel(self._h))
+
+    def wait_interruptibly(self, poll=0.05):
+        """sync() in slices of `poll` seconds, so that Python handles
+        signals between them (the reference checks PyErr_CheckSignals every
+        1000 rays, image_samplers.pyx:188-190): on KeyboardInterrupt the
+        launch is stopped, waited for, and the interrupt re-raised."""
+        while True:
+            try:
+                return self.sync(timeout=poll)
+            except YtrcTimeout:
+                continue
+            except KeyboardInterrupt:
+                self.cancel()
+                try:
+                    self.sync()
+                except YtrcCancelled:
+                    pass
+                raise
 
     def run(self):
         self.launch()
@@ -737,8 +784,15 @@ class Server:
     def progress(self):
         return self._done(lib().ytrc_server_progress)
 
-    def wait(self):
-        return self._done(lib().ytrc_server_wait)
+    def wait(self, timeout=None):
+        """Ids finished since the last call, waiting for at least one; with
+        a timeout (seconds) raises YtrcTimeout when none finished by then.
+        Returns [] when nothing is queued or running."""
+        if timeout is None:
+            return self._done(lib().ytrc_server_wait)
+        n = C.c_int32()
+        _check(lib().ytrc_server_wait_for(self._h, float(timeout) * 1e3, self._ids, 1024, C.byref(n)))
+        return [self._ids[i] for i in range(n.value)]
 
     def read(self, rid):
         spec = self._specs.pop(rid)

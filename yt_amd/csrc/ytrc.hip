@@ -24,6 +24,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -347,6 +348,8 @@ struct ytrc_batch {
     double total_ms = 0.0;
     int timed = 0;
     std::vector<int> ds_ids;  // datasets this batch holds a reference on
+    int *h_stop = nullptr;    // host-pinned stop flag read by the kernels (ytrc_batch_cancel)
+    bool cancelled = false;   // the last launch was asked to stop: its images are incomplete
 };
 
 namespace {
@@ -1358,6 +1361,7 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
             hipFree(B->arena);
             ctx->used -= B->arena_bytes;
         }
+        if (B->h_stop) hipHostFree(B->h_stop);
         if (B->ev0) hipEventDestroy(B->ev0);
         if (B->ev1) hipEventDestroy(B->ev1);
         for (auto &ev : B->ev_req)
@@ -1479,6 +1483,15 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
     }
     B->counters = (unsigned int *)(A + counters_off);
     B->d_imgs = (DImg *)(A + imgs_off);
+    // The stop flag: fine-grained pinned host memory that the kernels read
+    // at system scope once per tile.
+    e = hipHostMalloc((void **)&B->h_stop, sizeof(int), hipHostMallocCoherent | hipHostMallocMapped);
+    if (e != hipSuccess) return bad(YTRC_EDEVICE, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    __atomic_store_n(B->h_stop, 0, __ATOMIC_SEQ_CST);
+    int *d_stop = nullptr;
+    e = hipHostGetDevicePointer((void **)&d_stop, B->h_stop, 0);
+    if (e != hipSuccess) return bad(YTRC_EDEVICE, std::string("hipHostGetDevicePointer: ") + hipGetErrorString(e));
+    for (int i = 0; i < n_req; i++) B->h_imgs[i].stop = d_stop;
     e = hipMemcpy(B->d_imgs, B->h_imgs.data(), sizeof(DImg) * n_req, hipMemcpyHostToDevice);
     if (e != hipSuccess) return bad(YTRC_EDEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
     if ((e = hipEventCreate(&B->ev0)) != hipSuccess || (e = hipEventCreate(&B->ev1)) != hipSuccess)
@@ -1567,6 +1580,9 @@ int ytrc_batch_launch(ytrc_batch *b)
 {
     if (b == nullptr) return fail(YTRC_EINVAL, "ytrc_batch_launch: NULL batch");
     HIPCHK(hipSetDevice(b->ctx->device));
+    // a new launch clears an earlier stop request
+    __atomic_store_n(b->h_stop, 0, __ATOMIC_SEQ_CST);
+    b->cancelled = false;
     hipStream_t s = b->ctx->stream;
     const bool persistent = (b->flags & YTRC_BATCH_PERSISTENT) != 0;
     if ((b->flags & YTRC_BATCH_GRAPH) && b->gexec == nullptr) {
@@ -1675,16 +1691,57 @@ int ytrc_batch_request_ms(ytrc_batch *b, int32_t i, float *ms)
     return YTRC_OK;
 }
 
+namespace {
+
+// The end of a batch's last launch: its GPU time, or YTRC_ECANCELLED when
+// it was asked to stop.
+int batch_finish(ytrc_batch *b, float *ms_out)
+{
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    if (ms_out) *ms_out = ms;
+    if (b->cancelled)
+        return fail(YTRC_ECANCELLED, "ytrc_batch_sync: the launch was stopped (ytrc_batch_cancel); "
+                                     "its images are incomplete");
+    b->total_ms += ms;
+    b->timed += 1;
+    return YTRC_OK;
+}
+
+}  // namespace
+
 int ytrc_batch_sync(ytrc_batch *b, float *ms_out)
 {
     if (b == nullptr) return fail(YTRC_EINVAL, "ytrc_batch_sync: NULL batch");
     HIPCHK(hipSetDevice(b->ctx->device));
     HIPCHK(hipEventSynchronize(b->ev1));
-    float ms = 0.0f;
-    HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-    b->total_ms += ms;
-    b->timed += 1;
-    if (ms_out) *ms_out = ms;
+    return batch_finish(b, ms_out);
+}
+
+int ytrc_batch_sync_for(ytrc_batch *b, double timeout_ms, float *ms_out)
+{
+    if (b == nullptr) return fail(YTRC_EINVAL, "ytrc_batch_sync_for: NULL batch");
+    if (!(timeout_ms >= 0.0)) return fail(YTRC_EINVAL, "ytrc_batch_sync_for: timeout_ms must be >= 0");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    while (true) {
+        const hipError_t e = hipEventQuery(b->ev1);
+        if (e == hipSuccess) break;
+        if (e != hipErrorNotReady) return fail(YTRC_EDEVICE, std::string("hipEventQuery: ") + hipGetErrorString(e));
+        const double el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (el >= timeout_ms)
+            return fail(YTRC_ETIMEDOUT, "ytrc_batch_sync_for: the launch is still running after " +
+                                            std::to_string(timeout_ms) + " ms");
+        std::this_thread::sleep_for(std::chrono::microseconds(50));
+    }
+    return batch_finish(b, ms_out);
+}
+
+int ytrc_batch_cancel(ytrc_batch *b)
+{
+    if (b == nullptr) return fail(YTRC_EINVAL, "ytrc_batch_cancel: NULL batch");
+    __atomic_store_n(b->h_stop, 1, __ATOMIC_SEQ_CST);
+    b->cancelled = true;
     return YTRC_OK;
 }
 
@@ -1700,6 +1757,8 @@ int ytrc_batch_read_image(ytrc_batch *b, int32_t i, double *image_out)
 {
     if (b == nullptr || image_out == nullptr) return fail(YTRC_EINVAL, "ytrc_batch_read_image: NULL");
     if (i < 0 || i >= (int)b->reqs.size()) return fail(YTRC_EINVAL, "ytrc_batch_read_image: bad index");
+    if (b->cancelled)
+        return fail(YTRC_ECANCELLED, "ytrc_batch_read_image: the last launch was stopped; launch again");
     HIPCHK(hipSetDevice(b->ctx->device));
     HIPCHK(hipStreamSynchronize(b->ctx->stream));
     return read_request_image(b->reqs[i], b->h_imgs[i], image_out);
@@ -1801,6 +1860,7 @@ void ytrc_batch_destroy(ytrc_batch *b)
     for (auto &e : b->ev_req) hipEventDestroy(e);
     if (b->gexec) hipGraphExecDestroy(b->gexec);
     if (b->graph) hipGraphDestroy(b->graph);
+    if (b->h_stop) hipHostFree(b->h_stop);
     delete b;
 }
 
@@ -2346,6 +2406,28 @@ int ytrc_server_wait(ytrc_server *S, int64_t *done_ids, int32_t cap, int32_t *n_
         }
     }
     return ytrc_server_progress(S, done_ids, cap, n_done);
+}
+
+int ytrc_server_wait_for(ytrc_server *S, double timeout_ms, int64_t *done_ids, int32_t cap, int32_t *n_done)
+{
+    if (S == nullptr || n_done == nullptr) return fail(YTRC_EINVAL, "ytrc_server_wait_for: NULL");
+    if (!(timeout_ms >= 0.0)) return fail(YTRC_EINVAL, "ytrc_server_wait_for: timeout_ms must be >= 0");
+    const auto t0 = std::chrono::steady_clock::now();
+    while (true) {
+        const int rc = ytrc_server_progress(S, done_ids, cap, n_done);
+        if (rc != YTRC_OK || *n_done > 0) return rc;
+        bool idle;
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            idle = S->inflight.empty() && S->fifo.empty();
+        }
+        if (idle) return YTRC_OK;  // nothing left to wait for
+        const double el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (el >= timeout_ms)
+            return fail(YTRC_ETIMEDOUT, "ytrc_server_wait_for: no request finished within " +
+                                            std::to_string(timeout_ms) + " ms");
+        std::this_thread::sleep_for(std::chrono::microseconds(50));
+    }
 }
 
 int ytrc_server_read(ytrc_server *S, int64_t id, double *image_out)

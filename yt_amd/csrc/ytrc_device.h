@@ -69,6 +69,7 @@ struct alignas(16) DImg {
     double lrgba[4];        //   and light colour
     const double *pix_pos;  // per-pixel lens: nv0*nv1*3 origins, else null
     const double *pix_dir;  // per-pixel lens: nv0*nv1*3 directions
+    const int *stop;        // host-pinned stop flag of the launch, or null
     int32_t nv0, nv1;
     int32_t kind, n_samples;
     int32_t n_order, nf;
@@ -147,6 +148,14 @@ __device__ __forceinline__ double uni_d(double v)
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 __device__ __forceinline__ double y_fmax(double a, double b) { return (a > b) ? a : b; }
+
+// The host's stop request for a launch (ytrc_batch_cancel), read from
+// host-pinned memory at system scope once per tile: the analog of the
+// reference's PyErr_CheckSignals every 1000 rays (image_samplers.pyx:188-190).
+__device__ __forceinline__ bool stop_requested(const int *stop)
+{
+    return stop != nullptr && __hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0;
+}
 
 // generate_vector_info_plane_parallel: lenses.pyx:179-192.
 __device__ __forceinline__ void ray_setup(const DImg &im, int vi, int vj, double pos[3],

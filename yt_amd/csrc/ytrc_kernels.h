@@ -1178,6 +1178,7 @@ __global__ __launch_bounds__(256) void k_render(const DImg *__restrict__ imgs, i
     const int waves = blockDim.x >> 6;
     const int tile = blockIdx.x * waves + (threadIdx.x >> 6);
     if (tile >= im.tiles0 * im.tiles1) return;
+    if (uni((int)stop_requested(im.stop))) return;
     render_tile<KIND, NFP, REC, MODE>(im, tile, threadIdx.x & 63, s_tf);
 }
 
@@ -1240,6 +1241,8 @@ void k_persist(const DImg *__restrict__ imgs,
         const int img = __builtin_amdgcn_readfirstlane(w.x);
         const int tile = __builtin_amdgcn_readfirstlane(w.y);
         const DImg &im = imgs[img];
+        // a stop request ends the wave at its next item: every wave exits
+        if (uni((int)stop_requested(im.stop))) break;
         render_tile<KIND, NFP, REC, MODE>(im, tile, lane, s_tf + im.tf_slot);
     }
 }

@@ -135,7 +135,8 @@ class ImageSampler:
                 **self._light())
             batch = _lib.Batch(ctx, [spec])
             try:
-                batch.run()
+                batch.launch()
+                batch.wait_interruptibly()
                 self._image[...] = batch.read_image(0)
                 rec = batch.read_record(0) if record else None
             finally:
