@@ -230,11 +230,14 @@ int ytrc_batch_sync(ytrc_batch *b, float *ms_out);
  * running after timeout_ms (it keeps running; sync again or cancel). */
 int ytrc_batch_sync_for(ytrc_batch *b, double timeout_ms, float *ms_out);
 /* Asks the running launch to stop, the analog of the reference's
- * PyErr_CheckSignals every 1000 rays (image_samplers.pyx:188-190): every
- * wave reads the batch's stop flag (host-pinned) once per tile and exits at
- * its next tile; the next sync returns YTRC_ECANCELLED and the images stay
- * unreadable until a later launch completes.  Safe to call from another
- * thread while one waits in ytrc_batch_sync. */
+ * PyErr_CheckSignals every 1000 rays (image_samplers.pyx:188-190): each
+ * wave of a persistent launch (YTRC_BATCH_PERSISTENT) reads the batch's stop
+ * flag (host-pinned) at every 16th work item it takes (each wave at its own
+ * phase), and exits when it is set; plain launches finish the image they
+ * render.  The
+ * next sync returns YTRC_ECANCELLED and the images stay unreadable until a
+ * later launch completes.  Safe to call from another thread while one
+ * waits in ytrc_batch_sync. */
 int ytrc_batch_cancel(ytrc_batch *b);
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

@@ -150,8 +150,8 @@ __device__ __forceinline__ double uni_d(double v)
 __device__ __forceinline__ double y_fmax(double a, double b) { return (a > b) ? a : b; }
 
 // The host's stop request for a launch (ytrc_batch_cancel), read from
-// host-pinned memory at system scope once per tile: the analog of the
-// reference's PyErr_CheckSignals every 1000 rays (image_samplers.pyx:188-190).
+// host-pinned memory at system scope by persistent waves (the analog of the
+// reference's PyErr_CheckSignals every 1000 rays, image_samplers.pyx:188-190).
 __device__ __forceinline__ bool stop_requested(const int *stop)
 {
     return stop != nullptr && __hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0;

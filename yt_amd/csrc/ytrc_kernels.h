@@ -1178,7 +1178,6 @@ __global__ __launch_bounds__(256) void k_render(const DImg *__restrict__ imgs, i
     const int waves = blockDim.x >> 6;
     const int tile = blockIdx.x * waves + (threadIdx.x >> 6);
     if (tile >= im.tiles0 * im.tiles1) return;
-    if (uni((int)stop_requested(im.stop))) return;
     render_tile<KIND, NFP, REC, MODE>(im, tile, threadIdx.x & 63, s_tf);
 }
 
@@ -1226,6 +1225,9 @@ void k_persist(const DImg *__restrict__ imgs,
     // QSPLIT ranges empty.
     int part = (int)(blockIdx.x % QSPLIT);
     int left = QSPLIT;
+    // Items this wave has taken, started at a per-wave phase so that the
+    // waves' reads of the stop flag are spread over time (below).
+    int taken = (int)((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & 15);
     while (true) {
         unsigned int it = 0;
         if (lane == 0) it = atomicAdd(counter + part, 1u);
@@ -1241,8 +1243,12 @@ void k_persist(const DImg *__restrict__ imgs,
         const int img = __builtin_amdgcn_readfirstlane(w.x);
         const int tile = __builtin_amdgcn_readfirstlane(w.y);
         const DImg &im = imgs[img];
-        // a stop request ends the wave at its next item: every wave exits
-        if (uni((int)stop_requested(im.stop))) break;
+        // A stop request ends the wave at one of every 16th item it takes,
+        // each wave at its own phase.  The flag sits in host memory: read
+        // per item by every wave, or by every wave at its first item, those
+        // reads queue on the PCIe link (the projection kernel took 2x and
+        // 1.1x its time).
+        if ((taken++ & 15) == 15 && uni((int)stop_requested(im.stop))) break;
         render_tile<KIND, NFP, REC, MODE>(im, tile, lane, s_tf + im.tf_slot);
     }
 }

@@ -133,7 +133,8 @@ class ImageSampler:
                 n_samples=getattr(self, "n_samples", 10), tf=self._tf(), zbuffer=self.azbuffer,
                 image_in=self._image if np.any(self._image) else None, record=record,
                 **self._light())
-            batch = _lib.Batch(ctx, [spec])
+            # persistent: a stop request (Ctrl-C, below) reaches its waves
+            batch = _lib.Batch(ctx, [spec], _lib.BATCH_PERSISTENT)
             try:
                 batch.launch()
                 batch.wait_interruptibly()
