@@ -450,14 +450,22 @@ def main():
     flags = _lib.BATCH_PERSISTENT | (0 if args.no_graph else _lib.BATCH_GRAPH)
     specs = make_specs(vol, tf, cams)
     cfg = dict(autotune.DEFAULT)
+    tune_info = None
     if not args.no_tune:
         t = time.time()
         cfg, table = autotune.tune(ctx, specs, flags, use_cache=False)
         ranked = sorted(table, key=lambda x: x[1])
+        by_key = {json.dumps(c, sort_keys=True): ms for c, ms in table}
+        default_ms = by_key.get(json.dumps(autotune.DEFAULT, sort_keys=True), float("nan"))
+        tune_info = {"search": "coordinate descent over tile shape, rays per item, waves per "
+                               "workgroup and workgroups per CU",
+                     "measured": len(table), "seconds": round(time.time() - t, 1),
+                     "best": autotune.describe(cfg), "best_ms": round(ranked[0][1], 3),
+                     "default_ms": round(default_ms, 3),
+                     "worst_ms": round(ranked[-1][1], 3)}
         log(f"autotune: {len(table)} launch shapes in {time.time() - t:.1f}s; best "
-            + ", ".join(f"{c['tile'][0]}x{c['tile'][1]}/w{c['waves_per_group']}/g{c['groups_per_cu']}"
-                        f"={ms:.2f}ms" for c, ms in ranked[:4])
-            + f"; default {dict((str(c), ms) for c, ms in table).get(str(autotune.DEFAULT), float('nan')):.2f}ms")
+            + ", ".join(f"{autotune.describe(c)}={ms:.2f}ms" for c, ms in ranked[:4])
+            + f"; default {default_ms:.2f}ms; worst {ranked[-1][1]:.2f}ms")
     batch = autotune.make_batch(ctx, specs, cfg, flags)
     for _ in range(args.warmup):
         batch.run()
@@ -603,8 +611,11 @@ def main():
         w4 = (time.perf_counter() - t0) / args.steps
         b4.close()
         vol4.free()
+        # the brick split (brick chunk) as the tuner would choose it
+        # (autotune.tune_bricks): the faster of the two measured splits
         alt = dict(bricks="4x4x4 of 128^3", samples_per_step=s4, ms_per_step=round(w4 * 1e3, 3),
-                   gsamples_per_s=round(s4 / w4 / 1e9, 3))
+                   gsamples_per_s=round(s4 / w4 / 1e9, 3),
+                   chosen_split=f"{args.psize}^3" if ms_step <= w4 * 1e3 else "4^3")
         log(f"64 bricks of 128^3: {w4 * 1e3:.2f} ms/step, {s4 / w4 / 1e9:.1f} Gsamples/s")
     del rho
 
@@ -652,6 +663,7 @@ def main():
             "launch": cfg,
             "parallelism": "1 GPU",
         },
+        "autotune": tune_info,
         "images_per_s": round(imgs_s, 2),
         "compulsory": {"bytes": int(compulsory), "ms_at_peak": round(compulsory / HBM_PEAK_GBS / 1e6, 4),
                        "note": "resident bricks read once + image written once, at 8 TB/s"},

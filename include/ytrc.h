@@ -76,6 +76,12 @@ extern "C" {
 #define YTRC_BATCH_TILE_8x8 0u
 #define YTRC_BATCH_TILE_4x16 16u
 #define YTRC_BATCH_TILE_16x4 32u
+/* Tiles per work item of a persistent launch (rays per item: 64 x this):
+ * one wave renders an item's consecutive tiles of one image in a row. */
+#define YTRC_BATCH_ITEM_TILES_1 0u
+#define YTRC_BATCH_ITEM_TILES_2 64u
+#define YTRC_BATCH_ITEM_TILES_4 128u
+#define YTRC_BATCH_ITEM_TILES_8 192u
 
 typedef struct ytrc_ctx ytrc_ctx;
 typedef struct ytrc_batch ytrc_batch;

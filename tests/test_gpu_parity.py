@@ -332,9 +332,11 @@ def test_tile_shapes(ctx, tile):
 
 
 def test_autotune_picks_a_working_shape(ctx, tmp_path):
-    """The autotuner measures each candidate, caches the fastest, returns the
-    cached entry unmeasured the second time, and the chosen launch shape
-    renders the same images as the default one."""
+    """The autotuner measures candidates (a full grid over a subset of the
+    knobs, and the default coordinate descent), caches the fastest, returns
+    the cached entry unmeasured the second time, and every launch shape it
+    can choose -- tile shape, rays per work item (1-8 tiles), waves per
+    workgroup, workgroups per CU -- renders the same images as the default."""
     from yt_amd import autotune
 
     ds = uniform_lognormal(32, seed=71)
@@ -344,23 +346,27 @@ def test_autotune_picks_a_working_shape(ctx, tmp_path):
         specs = [vol.projection_request(c0_params(96, normal=(0.3, 0.5, 0.81))),
                  vol.vr_request(c0_params(80, normal=(-0.4, 0.1, 0.9)), tf)]
         path = str(tmp_path / "tune.json")
-        best, table = autotune.tune(ctx, specs, waves_per_group=(1, 4), groups_per_cu=(0, 2),
-                                    reps=1, path=path)
-        assert len(table) == 3 * 2 * 2 and all(ms > 0 for _, ms in table)
+        best, table = autotune.tune(ctx, specs, item_tiles=(1, 4), waves_per_group=(1, 4),
+                                    groups_per_cu=(0, 2), reps=1, path=path, search="grid")
+        assert len(table) == 3 * 2 * 2 * 2 and all(ms > 0 for _, ms in table)
         again, table2 = autotune.tune(ctx, specs, path=path)
         assert again == best and table2 == []
         # the cache key comes from the library (ytrc_ctx_device_name): no
         # second HIP runtime (torch's) is loaded into the process
         assert "torch" not in sys.modules
         assert "gfx" in ctx.device_name
+        d_best, d_table = autotune.tune(ctx, specs, reps=1, use_cache=False)
+        assert 0 < len(d_table) < 3 * 4 * 3 * 5
         base = _run(ctx, specs)
-        b = autotune.make_batch(ctx, specs, best)
-        try:
-            b.run()
-            for i in range(len(specs)):
-                assert np.array_equal(b.read_image(i), base[i][0]), i
-        finally:
-            b.close()
+        for cfg in [best, d_best, dict(autotune.DEFAULT, item_tiles=2),
+                    dict(autotune.DEFAULT, item_tiles=8, tile=[4, 16], waves_per_group=1)]:
+            b = autotune.make_batch(ctx, specs, cfg)
+            try:
+                b.run()
+                for i in range(len(specs)):
+                    assert np.array_equal(b.read_image(i), base[i][0]), (cfg, i)
+            finally:
+                b.close()
     finally:
         vol.free()
 

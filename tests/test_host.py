@@ -339,3 +339,39 @@ def test_serving_summary_and_p99_interval():
     assert out["n"] == 4 and out["n_side_le_1024"] == 3
     assert out["p99_ms_side_le_1024"] < 3.0 + 1e-9 and out["p99_ms"] > 90.0
     assert abs(out["offered_images_per_s"] - 2.0) < 1e-12 and out["images_per_s"] == 2.0
+
+
+def test_autotune_descent_finds_a_separable_optimum(monkeypatch, tmp_path):
+    """Coordinate descent over the four launch knobs reaches the minimum of a
+    separable cost (what the knobs' effects roughly are) with far fewer
+    measurements than the grid, and the grid finds the same point."""
+    from yt_amd import autotune
+
+    target = {"tile": [4, 16], "item_tiles": 4, "waves_per_group": 2, "groups_per_cu": 2}
+
+    class FakeTimer:
+        def __init__(self, ctx, specs, flags, reps, log):
+            self.seen = {}
+
+        def __call__(self, cfg):
+            import json
+            key = json.dumps(cfg, sort_keys=True)
+            cost = 10.0 + sum(0.0 if cfg[k] == target[k] else 1.0 + 0.1 * i
+                              for i, k in enumerate(autotune.KNOBS))
+            self.seen[key] = cost
+            return cost
+
+        def close(self):
+            pass
+
+    class Ctx:
+        device_name = "fake"
+
+    monkeypatch.setattr(autotune, "_Timer", FakeTimer)
+    best, table = autotune.tune(Ctx(), [], use_cache=False)
+    assert best == target
+    assert len(table) < 3 * 4 * 3 * 5 // 3
+    gbest, gtable = autotune.tune(Ctx(), [], use_cache=False, search="grid")
+    assert gbest == target and len(gtable) == 3 * 4 * 3 * 5
+    with pytest.raises(ValueError):
+        autotune.tune(Ctx(), [], use_cache=False, search="random")

@@ -39,6 +39,7 @@ BATCH_PERSISTENT = 1
 BATCH_GRAPH = 2
 # rays per 64-lane tile, rows x columns
 BATCH_TILE = {(8, 8): 0, (4, 16): 16, (16, 4): 32}
+BATCH_ITEM_TILES = {1: 0, 2: 64, 4: 128, 8: 192}
 
 EXPORTED = [
     "ytrc_ctx_create",
@@ -646,13 +647,16 @@ class RequestSpec:
 class Batch:
     """A set of requests staged on the device; launch() runs them."""
 
-    def __init__(self, ctx, specs, flags=BATCH_PLAIN, tile=(8, 8)):
+    def __init__(self, ctx, specs, flags=BATCH_PLAIN, tile=(8, 8), item_tiles=1):
         self.ctx = ctx
         self.specs = list(specs)
         if tuple(tile) not in BATCH_TILE:
             raise ValueError(f"tile must be one of {sorted(BATCH_TILE)}")
+        if item_tiles not in BATCH_ITEM_TILES:
+            raise ValueError(f"item_tiles must be one of {sorted(BATCH_ITEM_TILES)}")
         self.tile = tuple(tile)
-        flags = int(flags) | BATCH_TILE[self.tile]
+        self.item_tiles = int(item_tiles)
+        flags = int(flags) | BATCH_TILE[self.tile] | BATCH_ITEM_TILES[self.item_tiles]
         n = len(self.specs)
         arr = (Request * n)()
         for i, s in enumerate(self.specs):

@@ -1,18 +1,32 @@
 """Launch-shape autotuner for persistent ray-casting batches.
 
 The reference has no launch parameters: each sampler call loops over pixels
-with an OpenMP ``prange`` (image_samplers.pyx:167-197).  On the GPU the same
-work is cut into 64-ray tiles, one per wavefront, and three choices set how
-well the tiles fill the chip:
+with an OpenMP ``prange(schedule="static", chunk=100)``
+(image_samplers.pyx:167-197).  On the GPU the same work is cut into 64-ray
+tiles, one per wavefront, and these choices set how well the tiles fill the
+chip:
 
   * tile shape: 8x8, 4x16 or 16x4 rays (rows along image axis 0);
-  * waves per workgroup: 1, 2 or 4 tiles launched together;
+  * rays per work item: 1, 2, 4 or 8 consecutive tiles (64-512 rays) that
+    one wave renders in a row per item of the persistent queue (the
+    analog of the prange chunk);
+  * waves per workgroup: 1, 2 or 4 tiles launched together, which share
+    one copy of the batch's transfer-function tables in LDS (LDS per wave:
+    table bytes / waves per workgroup);
   * workgroups per CU for the persistent grid (0 = the occupancy limit).
 
-``tune`` times every candidate on the caller's own batch (HIP events around
+``tune`` measures candidates on the caller's own batch (HIP events around
 each launch, best of ``reps`` after one warm launch that also captures the
 graph) and keeps the fastest in a JSON cache keyed by the device and a
 signature of the batch, so later batches of the same shape skip the search.
+The default search is coordinate descent: starting from DEFAULT, each knob
+in turn takes every value with the others held, the best is kept, and the
+sweep repeats until no knob changes (at most ``rounds`` sweeps); ``search =
+"grid"`` measures every combination.
+
+``tune_bricks`` picks the brick edge of a uniform dataset (the brick chunk
+the kd-tree and the projection blocks are cut into) by the same measurement
+over the candidate splits.
 """
 
 import json
@@ -22,9 +36,11 @@ import time
 from . import _lib
 
 TILES = ((8, 8), (4, 16), (16, 4))
+ITEM_TILES = (1, 2, 4, 8)
 WAVES_PER_GROUP = (1, 2, 4)
 GROUPS_PER_CU = (0, 1, 2, 4, 8)
-DEFAULT = {"tile": [8, 8], "waves_per_group": 4, "groups_per_cu": 0}
+DEFAULT = {"tile": [8, 8], "item_tiles": 1, "waves_per_group": 4, "groups_per_cu": 0}
+KNOBS = ("tile", "item_tiles", "waves_per_group", "groups_per_cu")
 
 
 def cache_path():
@@ -67,16 +83,62 @@ def signature(specs, device_name=""):
     return "|".join(parts)
 
 
+def _full(cfg):
+    out = dict(DEFAULT)
+    out.update(cfg or {})
+    out["tile"] = list(out["tile"])
+    return out
+
+
 def make_batch(ctx, specs, cfg, flags=_lib.BATCH_PERSISTENT | _lib.BATCH_GRAPH):
     """A batch built and launched with the launch shape ``cfg``."""
-    b = _lib.Batch(ctx, specs, flags, tile=tuple(cfg["tile"]))
+    cfg = _full(cfg)
+    b = _lib.Batch(ctx, specs, flags, tile=tuple(cfg["tile"]), item_tiles=int(cfg["item_tiles"]))
     b.set_launch(cfg["waves_per_group"], cfg["groups_per_cu"])
     return b
 
 
+class _Timer:
+    """Times launch shapes on one set of specs, reusing one batch per
+    (tile shape, rays per item) and each measurement per configuration."""
+
+    def __init__(self, ctx, specs, flags, reps, log):
+        self.ctx, self.specs, self.flags, self.reps, self.log = ctx, specs, flags, reps, log
+        self.batches = {}
+        self.seen = {}
+
+    def __call__(self, cfg):
+        key = json.dumps(cfg, sort_keys=True)
+        if key in self.seen:
+            return self.seen[key]
+        bk = (tuple(cfg["tile"]), cfg["item_tiles"])
+        if bk not in self.batches:
+            self.batches[bk] = _lib.Batch(self.ctx, self.specs, self.flags, tile=bk[0],
+                                          item_tiles=bk[1])
+        b = self.batches[bk]
+        b.set_launch(cfg["waves_per_group"], cfg["groups_per_cu"])
+        b.run()
+        ms = min(b.run() for _ in range(max(1, self.reps)))
+        self.seen[key] = ms
+        if self.log:
+            self.log(f"autotune {describe(cfg)}: {ms:.3f} ms")
+        return ms
+
+    def close(self):
+        for b in self.batches.values():
+            b.close()
+        self.batches = {}
+
+
+def describe(cfg):
+    cfg = _full(cfg)
+    return (f"{cfg['tile'][0]}x{cfg['tile'][1]}/i{cfg['item_tiles']}/w{cfg['waves_per_group']}"
+            f"/g{cfg['groups_per_cu']}")
+
+
 def tune(ctx, specs, flags=_lib.BATCH_PERSISTENT | _lib.BATCH_GRAPH, tiles=TILES,
-         waves_per_group=WAVES_PER_GROUP, groups_per_cu=GROUPS_PER_CU, reps=2,
-         use_cache=True, path=None, log=None):
+         item_tiles=ITEM_TILES, waves_per_group=WAVES_PER_GROUP, groups_per_cu=GROUPS_PER_CU,
+         reps=2, use_cache=True, path=None, log=None, search="descent", rounds=3):
     """Returns (best config dict, list of (config, ms) measured).  With
     ``use_cache`` a cached entry for this signature is returned unmeasured."""
     path = path or cache_path()
@@ -84,27 +146,74 @@ def tune(ctx, specs, flags=_lib.BATCH_PERSISTENT | _lib.BATCH_GRAPH, tiles=TILES
     if use_cache:
         hit = _load(path).get(key)
         if hit is not None:
-            return dict(hit["cfg"]), []
-    table = []
+            return _full(hit["cfg"]), []
+    values = {"tile": [list(t) for t in tiles], "item_tiles": list(item_tiles),
+              "waves_per_group": list(waves_per_group), "groups_per_cu": list(groups_per_cu)}
     t0 = time.time()
-    for tile in tiles:
-        b = _lib.Batch(ctx, specs, flags, tile=tile)
-        try:
-            for wpg in waves_per_group:
-                for gpc in groups_per_cu:
-                    b.set_launch(wpg, gpc)
-                    b.run()
-                    ms = min(b.run() for _ in range(max(1, reps)))
-                    cfg = {"tile": list(tile), "waves_per_group": wpg, "groups_per_cu": gpc}
-                    table.append((cfg, ms))
-                    if log:
-                        log(f"autotune {tile[0]}x{tile[1]} wpg={wpg} gpc={gpc}: {ms:.3f} ms")
-        finally:
-            b.close()
-    best = min(table, key=lambda t: t[1])[0]
+    timer = _Timer(ctx, specs, flags, reps, log)
+    try:
+        if search == "grid":
+            for tile in values["tile"]:
+                for it in values["item_tiles"]:
+                    for wpg in values["waves_per_group"]:
+                        for gpc in values["groups_per_cu"]:
+                            timer({"tile": tile, "item_tiles": it, "waves_per_group": wpg,
+                                   "groups_per_cu": gpc})
+            best = min(timer.seen.items(), key=lambda kv: kv[1])
+            best = json.loads(best[0])
+        elif search == "descent":
+            best = _full(None)
+            for v in values.values():
+                if not v:
+                    raise ValueError("tune: every knob needs at least one value")
+            for knob in KNOBS:  # start from a point inside the given ranges
+                if best[knob] not in values[knob]:
+                    best[knob] = values[knob][0]
+            best_ms = timer(best)
+            for _ in range(max(1, rounds)):
+                changed = False
+                for knob in KNOBS:
+                    for v in values[knob]:
+                        cand = dict(best, **{knob: v})
+                        ms = timer(cand)
+                        if ms < best_ms:
+                            best, best_ms, changed = cand, ms, True
+                if not changed:
+                    break
+        else:
+            raise ValueError(f"tune: unknown search {search!r}")
+    finally:
+        timer.close()
+    table = [(json.loads(k), ms) for k, ms in timer.seen.items()]
     if use_cache:
         d = _load(path)
         d[key] = {"cfg": best, "ms": min(t[1] for t in table), "searched": len(table),
                   "seconds": round(time.time() - t0, 2)}
         _store(path, d)
-    return best, table
+    return _full(best), table
+
+
+def tune_bricks(build, candidates, cfg=None, flags=_lib.BATCH_PERSISTENT | _lib.BATCH_GRAPH, reps=3,
+                log=None):
+    """Picks the brick split of a dataset by measurement: for each candidate
+    (e.g. bricks per axis of load_uniform_grid), ``build(candidate)`` returns
+    (context, specs, release) -- the resident dataset's requests for the same
+    cameras and a function that frees it; the batch runs with launch shape
+    ``cfg``.  Returns (best candidate, {candidate: ms per launch})."""
+    out = {}
+    for c in candidates:
+        ctx, specs, release = build(c)
+        try:
+            b = make_batch(ctx, specs, cfg or DEFAULT, flags)
+            try:
+                b.run()
+                ms = min(b.run() for _ in range(max(1, reps)))
+            finally:
+                b.close()
+        finally:
+            release()
+        out[c] = ms
+        if log:
+            log(f"brick split {c}: {ms:.3f} ms")
+    best = min(out, key=lambda c: out[c])
+    return best, out

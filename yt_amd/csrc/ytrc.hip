@@ -1344,9 +1344,12 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
     const int tcode = (int)((batch_flags >> 4) & 3);
     if (tcode == 3) return fail(YTRC_EINVAL, "ytrc_batch_create: unknown tile shape");
     const int tsh = tcode == 1 ? 4 : (tcode == 2 ? 2 : TILE_SHIFT_DEFAULT);
+    const int per_item = 1 << (int)((batch_flags >> 6) & 3);  // tiles per work item
     for (int i = 0; i < n_req; i++) {
         const int rc = prep_request(ctx, reqs[i], "request " + std::to_string(i) + ": ", P[i], tsh);
         if (rc != YTRC_OK) return rc;
+        if (P[i].q.ntiles >= (1 << 24))
+            return fail(YTRC_EINVAL, "request " + std::to_string(i) + ": more than 2^24 ray tiles");
         base[i] = off;
         off = align_up(off + P[i].bytes, 256);
     }
@@ -1418,8 +1421,12 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
                 const int i = ids[o];
                 const int T = P[i].q.ntiles;
                 const int t0 = (int)((int64_t)r * T / QSPLIT), t1 = (int)((int64_t)(r + 1) * T / QSPLIT);
-                for (int t = t0; t < t1; t++)
-                    ks.push_back({(double)(t - t0) / (double)(t1 - t0), (int)o, make_int2(i, t)});
+                // an item: up to per_item consecutive tiles, the count in
+                // the top 8 bits of its tile field
+                for (int t = t0; t < t1; t += per_item) {
+                    const int cnt = std::min(per_item, t1 - t);
+                    ks.push_back({(double)(t - t0) / (double)(t1 - t0), (int)o, make_int2(i, t | (cnt << 24))});
+                }
             }
             std::stable_sort(ks.begin(), ks.end(), [](const Key &a, const Key &b) {
                 return a.frac < b.frac || (a.frac == b.frac && a.ord < b.ord);

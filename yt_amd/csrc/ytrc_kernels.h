@@ -1241,7 +1241,10 @@ void k_persist(const DImg *__restrict__ imgs,
         }
         const int2 w = items[lo + it];
         const int img = __builtin_amdgcn_readfirstlane(w.x);
-        const int tile = __builtin_amdgcn_readfirstlane(w.y);
+        const int tword = __builtin_amdgcn_readfirstlane(w.y);
+        // an item holds 1 tile (count field 0 or 1) or a run of them
+        const int tile = tword & 0xffffff;
+        const int cnt = max(1, tword >> 24);
         const DImg &im = imgs[img];
         // A stop request ends the wave at one of every 16th item it takes,
         // each wave at its own phase.  The flag sits in host memory: read
@@ -1249,7 +1252,7 @@ void k_persist(const DImg *__restrict__ imgs,
         // reads queue on the PCIe link (the projection kernel took 2x and
         // 1.1x its time).
         if ((taken++ & 15) == 15 && uni((int)stop_requested(im.stop))) break;
-        render_tile<KIND, NFP, REC, MODE>(im, tile, lane, s_tf + im.tf_slot);
+        for (int t = 0; t < cnt; t++) render_tile<KIND, NFP, REC, MODE>(im, tile + t, lane, s_tf + im.tf_slot);
     }
 }
 
