@@ -959,6 +959,8 @@ int prep_request(ytrc_ctx *ctx, const ytrc_request &r, const std::string &tag, P
     q.npix = (int64_t)r.nv[0] * r.nv[1];
     const int th = 64 >> tsh, tw = 1 << tsh;
     const int t0 = (r.nv[0] + th - 1) / th, t1 = (r.nv[1] + tw - 1) / tw;
+    // the persistent queue's items hold a tile index in 24 bits
+    if ((int64_t)t0 * t1 >= (1 << 24)) return fail(YTRC_EINVAL, tag + "more than 2^24 ray tiles in one image");
     q.ntiles = t0 * t1;
     q.nfp = ds.nfp;
     const bool rec = (r.flags & YTRC_FLAG_RECORD) != 0;
@@ -1348,8 +1350,6 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
     for (int i = 0; i < n_req; i++) {
         const int rc = prep_request(ctx, reqs[i], "request " + std::to_string(i) + ": ", P[i], tsh);
         if (rc != YTRC_OK) return rc;
-        if (P[i].q.ntiles >= (1 << 24))
-            return fail(YTRC_EINVAL, "request " + std::to_string(i) + ": more than 2^24 ray tiles");
         base[i] = off;
         off = align_up(off + P[i].bytes, 256);
     }
