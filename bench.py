@@ -250,6 +250,100 @@ def cpu_baseline(ds, vol, tf, cams, strata, window_samples, gpu_images, runs=5):
     }
 
 
+def configs_leg(ctx, cpu=True, reps=5):
+    """The single-image configurations beside the batched workload: 64^3
+    projection at 256^2, the 256^3 density-weighted projection of
+    temperature at 1024^2 and the 643-grid AMR volume render at 1024^2.
+    Per configuration: GPU time of one plain launch (one kernel per image)
+    and of one persistent launch (HIP events, median of `reps`), the sample
+    count (record pass), Gsamples/s, and -- with `cpu` -- the FP64 oracle's
+    wall time on all host threads (and on one thread for the 64^3 case,
+    off_axis_projection's default) with the GPU image's rel-L2 against it."""
+    from oracle import oracle
+    from tests.helpers import oracle_proj_bricks, oracle_vr_bricks, tf_dict, vr_tf_for
+    from yt_amd import _lib, data, render
+
+    threads = host_threads()[0]
+    out = {}
+
+    def gpu_times(spec):
+        res = {}
+        for name, flags in (("plain", _lib.BATCH_PLAIN), ("persistent", _lib.BATCH_PERSISTENT)):
+            b = _lib.Batch(ctx, [spec], flags)
+            try:
+                b.run()
+                res[name] = float(np.median([b.run() for _ in range(reps)]))
+                img = b.read_image(0)
+            finally:
+                b.close()
+        spec.record = True
+        b = _lib.Batch(ctx, [spec], _lib.BATCH_PERSISTENT)
+        try:
+            b.run()
+            samples = int(b.read_record(0)[2].sum())
+        finally:
+            b.close()
+            spec.record = False
+        return res, samples, img
+
+    def cpu_times(fn, threads_list):
+        res = {}
+        for nt in threads_list:
+            fn(nt)
+            xs = []
+            for _ in range(3):
+                t = time.perf_counter()
+                o = fn(nt)
+                xs.append((time.perf_counter() - t) * 1e3)
+            res[f"{nt}_threads"] = round(float(np.median(xs)), 2)
+        return res, o
+
+    cases = []
+    ds0 = data.load_uniform_grid({"density": data.lognormal_density(64, seed=1001)})
+    cases.append(("c0_64cubed_projection_256", ds0, "proj", "density", None,
+                  render.camera_params((0.5,) * 3, (0.3, 0.5, 0.81), 1.0, 256), (1, threads)))
+    rho = data.lognormal_density(256, seed=1001)
+    ds1 = data.load_uniform_grid({"density": rho,
+                                  "temperature": data.temperature_from_density(rho, seed=1002)})
+    cases.append(("c1_256cubed_weighted_projection_1024", ds1, "proj", "temperature", "density",
+                  render.camera_params((0.5,) * 3, (0.3, 0.5, 0.81), 1.0, 1024), (threads,)))
+    amr = data.load_amr_grids(data.make_amr_hierarchy(128, 3), [128] * 3)
+    cases.append(("c2_amr_vr_1024", amr, "vr", "density", None,
+                  render.camera_params((0.5,) * 3, (0.3, 0.5, 0.81), 1.0, 1024), (threads,)))
+    for name, ds, kind, field, weight, p, tl in cases:
+        if kind == "proj":
+            vol = render.ResidentVolume(ctx, ds, proj_fields=field, weight=weight)
+            spec = vol.projection_request(p)
+        else:
+            vol = render.ResidentVolume(ctx, ds, vr_field=field)
+            tf = vr_tf_for(ds)
+            spec = vol.vr_request(p, tf)
+        try:
+            g, samples, img = gpu_times(spec)
+            r = {"gpu_plain_ms": round(g["plain"], 3), "gpu_persistent_ms": round(g["persistent"], 3),
+                 "samples": samples,
+                 "gsamples_per_s": round(samples / (g["persistent"] * 1e-3) / 1e9, 2)}
+            if cpu:
+                if kind == "proj":
+                    bricks = oracle_proj_bricks(ds, [field], weight=weight)
+                    c, oimg = cpu_times(lambda nt: oracle.cast(0, p, bricks, num_threads=nt), tl)
+                    nch = 1
+                else:
+                    bricks = oracle_vr_bricks(vol)
+                    tfd = tf_dict(tf)
+                    c, oimg = cpu_times(lambda nt: oracle.cast(1, p, bricks, order=spec.order, tf=tfd,
+                                                               num_threads=nt), tl)
+                    nch = 3
+                r["cpu_oracle_ms"] = c
+                r["rel_l2"] = max(float(np.linalg.norm(img[..., k] - oimg[..., k]) /
+                                        np.linalg.norm(oimg[..., k])) for k in range(nch))
+        finally:
+            vol.free()
+        out[name] = r
+        log(f"config {name}: {r}")
+    return out
+
+
 def drop_in_leg(reps=3):
     """End to end through the drop-in entry points, host arrays in, image
     out (upload, brick preparation and cast included): off_axis_projection
@@ -394,6 +488,8 @@ def main():
                     help="age limit of the shortest-first chunk fill order (ms)")
     ap.add_argument("--serve-policy-rates", default="50,100,200",
                     help="rates repeated with chunks filled in arrival order")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the single-image configurations (64^3, 256^3 weighted, AMR VR)")
     ap.add_argument("--no-alt", action="store_true",
                     help="skip the 64-brick (128^3 bricks) variant of the batched step")
     ap.add_argument("--no-tune", action="store_true",
@@ -628,6 +724,7 @@ def main():
                             args.serve_age_ms)
 
     dropin = None if args.no_dropin else drop_in_leg()
+    configs = None if args.no_configs else configs_leg(ctx, cpu=not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:
@@ -674,6 +771,7 @@ def main():
         "latency_ms": lat,
         "serving": serving,
         "drop_in_ms": dropin,
+        "configs": configs,
         "roofline": {
             "bound": "hbm",
             "kernel": "k_persist<VR>" if dom == 1 else "k_persist<projection>",
