@@ -542,6 +542,40 @@ __device__ __forceinline__ bool quad_visit_ok(const float (&k)[4])
     return __builtin_amdgcn_classf(m, (1 << 7) | (1 << 8));  // +subnormal or +normal
 }
 
+// The sub-samples of one visit for a ColorTransferFunction in transmittance
+// form.  ku: the TF bin coordinate along the visit's sub-samples, u(s) =
+// ku0 + ku1 s + ku2 s^2 + ku3 s^3; dtf: the sub-sample length.  The visit's
+// opacity Q_c = 1 - prod_s (1 - dt e_s) by the recurrence Q' = Q + e G,
+// G = dt (1 - Q): Q stays small where the medium is thin, so FP32 keeps its
+// relative precision; the FP64 transmittance st then takes T (1 - Q) once
+// per visit.
+template <int NS>
+__device__ __forceinline__ void ctf_visit(const TFCache &tf, const float4 *__restrict__ lt,
+                                          const float (&ku)[4], float dtf, int ns, double (&st)[4])
+{
+    float Q[3] = {0.0f, 0.0f, 0.0f};
+    float G[3] = {dtf, dtf, dtf};
+    const int nsq = (NS > 0) ? NS : ns;
+    float sf = 0.0f;
+#pragma unroll
+    for (int s = 0; s < nsq; s++, sf += 1.0f) {
+        float e[3];
+        // Sub-sample 0 is ku0 (the coefficients are finite here).
+        const float u = (NS > 0 && s == 0)
+                            ? ku[0]
+                            : __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(ku[3], sf, ku[2]), sf, ku[1]),
+                                             sf, ku[0]);
+        fit_eval_ctf(tf, lt, u, e);
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) {
+            Q[ch] = __builtin_fmaf(e[ch], G[ch], Q[ch]);
+            G[ch] = __builtin_fmaf(-dtf, Q[ch], dtf);
+        }
+    }
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++) st[ch] = __builtin_fma(-st[ch], (double)Q[ch], st[ch]);
+}
+
 // VolumeRenderSampler.sample (image_samplers.pyx:453-487).
 // MODE bit 0: ColorTransferFunction in transmittance form; bit 1: n_samples == 10 unrolled;
 // bit 2: early ray termination.
@@ -584,37 +618,10 @@ struct VRSampler {
         }
     }
 
-    // ColorTransferFunction in transmittance form.  ku: the TF bin
-    // coordinate along the visit's sub-samples, u(s) = ku0 + ku1 s + ku2 s^2
-    // + ku3 s^3.  The visit's opacity Q_c = 1 - prod_s (1 - dt e_s) by the
-    // recurrence Q' = Q + e G, G = dt (1 - Q): Q stays small where the
-    // medium is thin, so FP32 keeps its relative precision; the FP64
-    // transmittance then takes T (1 - Q) once per visit.
     template <int NS>
     __device__ __forceinline__ void ctf_samples(const float (&ku)[4], double dt, int ns)
     {
-        const float dtf = (float)dt;
-        float Q[3] = {0.0f, 0.0f, 0.0f};
-        float G[3] = {dtf, dtf, dtf};
-        const int nsq = (NS > 0) ? NS : ns;
-        float sf = 0.0f;
-#pragma unroll
-        for (int s = 0; s < nsq; s++, sf += 1.0f) {
-            float e[3];
-            // Sub-sample 0 is ku0 (the coefficients are finite here).
-            const float u = (NS > 0 && s == 0)
-                                ? ku[0]
-                                : __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(ku[3], sf, ku[2]), sf, ku[1]),
-                                                 sf, ku[0]);
-            fit_eval_ctf(*tf, lt, u, e);
-#pragma unroll
-            for (int ch = 0; ch < 3; ch++) {
-                Q[ch] = __builtin_fmaf(e[ch], G[ch], Q[ch]);
-                G[ch] = __builtin_fmaf(-dtf, Q[ch], dtf);
-            }
-        }
-#pragma unroll
-        for (int ch = 0; ch < 3; ch++) st[ch] = __builtin_fma(-st[ch], (double)Q[ch], st[ch]);
+        ctf_visit<NS>(*tf, lt, ku, (float)dt, ns, st);
     }
 
     template <int NS>
@@ -738,6 +745,160 @@ struct VRSampler {
             samples<0>(k, dt, n_samples);
     }
 };
+
+// Byte offsets of the vertex blocks (vertex_offset times 4, one field) as
+// a sum over axes: (i & 3) W = i W - (i >> 2) 4 W, so the offset of
+// corner (0, 0, 0) of cell (c0, c1, c2) is
+//   4 (25 c0 + 5 c1 + c2) + (c0 >> 2) K0 + (c1 >> 2) K1 + (c2 >> 2) K2
+// with K0 = 4 (128 nbj nbk - 100), K1 = 4 (128 nbk - 20), K2 = 4 (128 - 4)
+// for VB = 4, VB1 = 5, VBP = 128 (nbj, nbk: vertex blocks along axes 1, 2).
+constexpr unsigned VW0 = 4u * VB1 * VB1, VW1 = 4u * VB1, VW2 = 4u;
+constexpr unsigned VK2 = 4u * (VBP - VB);
+__host__ __device__ constexpr uint64_t vk0_bytes(uint64_t nbj, uint64_t nbk) { return 4u * (VBP * nbj * nbk - VB * VB1 * VB1); }
+__host__ __device__ constexpr uint64_t vk1_bytes(uint64_t nbk) { return 4u * (VBP * nbk - VB * VB1); }
+
+// The smaller of two FP64 values that are not NaN, as one instruction
+// (fmin's lowering first quiets each operand).
+__device__ __forceinline__ double min_f64(double a, double b)
+{
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
+// VRSampler with a ColorTransferFunction in transmittance form (MODE bit
+// 0), one field and a shared ray direction, walked by walk_loop's DDA, for
+// the common case, taken per wave and brick when every walking lane starts
+// with all three cell indices inside the brick (so every visit is inside
+// it: one axis steps at a time and the walk stops when it leaves), the
+// brick's vertex blocks span fewer than 2^32 bytes and K0, K1 fit 24-bit
+// multiplies.  The same visits and sub-samples in fewer operations:
+// * no per-visit bounds test;
+// * corner (0, 0, 0)'s byte offset from the sum over axes above, the 8
+//   corners read through a buffer resource (four 8-byte loads);
+// * the cell-relative position fma(tt, pa, r) with r = pb - c kept per axis
+//   and stepped with its index, tt = en + (ex - en) / (2 n_samples);
+// * the cubic in the bin coordinate from the field's partial derivatives
+//   and per-brick products of P = (float)(v_dir idds): with the
+//   sub-sample length d, u(s) = idbin (k0 + (s d) (fx P0 + fy P1 + fz P2) +
+//   (s d)^2 (fxy P0 P1 + fxz P0 P2 + fyz P1 P2) + (s d)^3 D P0 P1 P2) + qc1;
+// * the axis-0/1 choice and the axis-2 step as nested branches, so that
+//   the execution masks come from the two comparisons.
+template <int NS>
+__device__ __forceinline__ void walk_vr_ctf_fast(Walk &w, double max_t, const BrickReg &b,
+                                                 __amdgpu_buffer_rsrc_t rsrc, unsigned K0, unsigned K1,
+                                                 const uint8_t *mask, const double (&pa)[3],
+                                                 const double (&pb)[3], const TFCache &tf,
+                                                 const float4 *__restrict__ lt, double inv_ns, int ns,
+                                                 double (&st)[4])
+{
+    double t0 = w.tmax[0], t1 = w.tmax[1], t2 = w.tmax[2];
+    int c0 = w.cur[0], c1 = w.cur[1], c2 = w.cur[2];
+    double en = w.enter_t;
+    // (the direction is shared: steps, increments and strides are scalar)
+    const int st0 = uni(w.step[0]), st1 = uni(w.step[1]), st2 = uni(w.step[2]);
+    const double td0 = uni_d(w.tdelta[0]), td1 = uni_d(w.tdelta[1]), td2 = uni_d(w.tdelta[2]);
+    // the steps as doubles from scalar selects (no conversion at each step)
+    const double sd0 = (st0 > 0) ? 1.0 : ((st0 < 0) ? -1.0 : 0.0);
+    const double sd1 = (st1 > 0) ? 1.0 : ((st1 < 0) ? -1.0 : 0.0);
+    const double sd2 = (st2 > 0) ? 1.0 : ((st2 < 0) ? -1.0 : 0.0);
+    K0 = (unsigned)uni((int)K0);
+    K1 = (unsigned)uni((int)K1);
+    double r0 = pb[0] - (double)c0, r1 = pb[1] - (double)c1, r2 = pb[2] - (double)c2;
+    const double hns = 0.5 * inv_ns;
+    const float inv_nsf = unif((float)inv_ns);
+    const float P0 = unif((float)pa[0]), P1 = unif((float)pa[1]), P2 = unif((float)pa[2]);
+    const float P01 = unif(P0 * P1), P02 = unif(P0 * P2), P12 = unif(P1 * P2);
+    const float P012 = unif(P01 * P2);
+    const float idb = tf.idbin, qc1 = tf.qc1;
+    const unsigned D0 = (unsigned)b.d0, D1 = (unsigned)b.d1, D2 = (unsigned)b.d2;
+    while (true) {
+        // walk_loop's axis choice and exit time (grid_traversal.pyx:120-138);
+        // the chosen tmax is the smallest, so its value is a minimum (the
+        // tmax are never NaN)
+        const bool a01 = t0 < t1;
+        const double m = min_f64(t0, t1);
+        const bool bm = m < t2;
+        const double tm = min_f64(m, t2);
+        const bool ge = tm >= max_t;
+        const double ex = min_f64(tm, max_t);
+        {
+            const unsigned o = __umul24((unsigned)c0 >> 2, K0) +
+                               (__umul24((unsigned)c1 >> 2, K1) +
+                                (__umul24((unsigned)c2 >> 2, VK2) +
+                                 (__umul24((unsigned)c0, VW0) + (__umul24((unsigned)c1, VW1) + ((unsigned)c2 << 2)))));
+            int mval = 1;
+            if (mask != nullptr) mval = mask[(int)__umul24(__umul24(c0, D1) + c1, D2) + c2];
+            float cv[8];
+            BufLoad<2>::get(rsrc, o, cv);
+            BufLoad<2>::get(rsrc, o + 4u * vcorner(0, 1, 0), cv + 2);
+            BufLoad<2>::get(rsrc, o + 4u * vcorner(1, 0, 0), cv + 4);
+            BufLoad<2>::get(rsrc, o + 4u * vcorner(1, 1, 0), cv + 6);
+            __builtin_amdgcn_sched_barrier(0);
+            const double d = ex - en;
+            const double tt = __builtin_fma(d, hns, en);
+            float x0 = (float)__builtin_fma(tt, pa[0], r0);
+            float y0 = (float)__builtin_fma(tt, pa[1], r1);
+            float z0 = (float)__builtin_fma(tt, pa[2], r2);
+            const float dtf = (float)d * inv_nsf;
+            asm volatile("" : "+v"(x0), "+v"(y0), "+v"(z0));
+#pragma unroll
+            for (int q = 0; q < 8; q++) asm volatile("" : "+v"(cv[q]));
+            if (mval == 1) {
+                // offset_interpolate's trilinear form (fixed_interpolator.cpp:
+                // 34-52) as A + Bx x + By y + Bz z + Cxy xy + Cxz xz + Cyz yz
+                // + D xyz, and its partial derivatives at (x0, y0, z0)
+                const float A = cv[0];
+                const float Bz = cv[1] - cv[0], e1 = cv[3] - cv[2], e2 = cv[5] - cv[4], e3 = cv[7] - cv[6];
+                const float By = cv[2] - cv[0], g1 = cv[6] - cv[4];
+                const float Cyz = e1 - Bz, g3 = e3 - e2;
+                const float Bx = cv[4] - cv[0], Cxy = g1 - By, Cxz = e2 - Bz, D = g3 - Cyz;
+                const float fxy = __builtin_fmaf(D, z0, Cxy);
+                const float fxz = __builtin_fmaf(D, y0, Cxz);
+                const float fyz = __builtin_fmaf(D, x0, Cyz);
+                const float tyz = __builtin_fmaf(Cyz, z0, By);
+                const float fx = __builtin_fmaf(y0, fxy, __builtin_fmaf(Cxz, z0, Bx));
+                const float fy = __builtin_fmaf(x0, fxy, tyz);
+                const float fz = __builtin_fmaf(x0, fxz, __builtin_fmaf(Cyz, y0, Bz));
+                const float k0 = __builtin_fmaf(x0, fx, __builtin_fmaf(y0, tyz, __builtin_fmaf(z0, Bz, A)));
+                const float q1 = dtf * idb, q2 = q1 * dtf, q3 = q2 * dtf;
+                const float ku[4] = {
+                    __builtin_fmaf(k0, idb, qc1),
+                    __builtin_fmaf(fx, P0, __builtin_fmaf(fy, P1, fz * P2)) * q1,
+                    __builtin_fmaf(fxy, P01, __builtin_fmaf(fxz, P02, fyz * P12)) * q2,
+                    D * P012 * q3};
+                // quad_visit_ok on k0 and the scaled higher coefficients
+                const float mm = __builtin_fabsf(k0) + __builtin_fabsf(ku[1]) + __builtin_fabsf(ku[2]) +
+                                 __builtin_fabsf(ku[3]);
+                if (__builtin_amdgcn_classf(mm, (1 << 7) | (1 << 8))) ctf_visit<NS>(tf, lt, ku, dtf, ns, st);
+            }
+        }
+        bool out;
+        if (bm) {
+            if (a01) {
+                asm volatile("");
+                c0 += st0;
+                t0 += td0;
+                r0 -= sd0;
+                out = (unsigned)c0 >= D0;
+            } else {
+                asm volatile("");
+                c1 += st1;
+                t1 += td1;
+                r1 -= sd1;
+                out = (unsigned)c1 >= D1;
+            }
+        } else {
+            asm volatile("");
+            c2 += st2;
+            t2 += td2;
+            r2 -= sd2;
+            out = (unsigned)c2 >= D2;
+        }
+        en = tm;
+        if (out || ge) break;
+    }
+}
 
 // InterpolatedProjectionSampler.sample (image_samplers.pyx:368-402): the
 // trilinear value of each field at the visit's n_samples midpoint
@@ -1102,7 +1263,25 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
                     s.tb[c] = 1.0;
                 }
                 s.rec = &rec;
-                walk_loop(b, w, max_t, s);
+                bool fastw = false;
+                if constexpr (!REC && NFP == 1 && (MODE & 1) != 0 && !PP) {
+                    // the fast walk's conditions (walk_vr_ctf_fast)
+                    const uint64_t bytes = (uint64_t)vblocks(s.b.d0) * (unsigned)s.b.vnbj *
+                                           (unsigned)s.b.vnbk * (VBP * 4u);
+                    const uint64_t k0 = vk0_bytes((unsigned)s.b.vnbj, (unsigned)s.b.vnbk);
+                    const bool inside = (unsigned)w.cur[0] < (unsigned)s.b.d0 &&
+                                        (unsigned)w.cur[1] < (unsigned)s.b.d1 &&
+                                        (unsigned)w.cur[2] < (unsigned)s.b.d2;
+                    const uint64_t k1 = vk1_bytes((unsigned)s.b.vnbk);
+                    if (bytes < (1ull << 32) && k0 < (1u << 24) && k1 < (1u << 24) && __all(inside)) {
+                        walk_vr_ctf_fast<(MODE & 2) ? 10 : 0>(
+                            w, max_t, s.b, brick_rsrc(s.data, (unsigned)bytes), (unsigned)k0,
+                            (unsigned)k1, s.mask, s.pa, s.pb, tfc, lt,
+                            s.inv_ns, s.n_samples, s.st);
+                        fastw = true;
+                    }
+                }
+                if (!fastw) walk_loop(b, w, max_t, s);
                 if constexpr (ert) {
                     double cb[4], tb[4];
 #pragma unroll
