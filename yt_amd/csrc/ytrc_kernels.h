@@ -221,8 +221,8 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t brick_rsrc(const float *p, uns
 //   (c2>>2) 60, with K0 = 64 (nbj nbk - 1), K1 = 64 nbk - 16 (the same
 //   value as blocked_offset, in fewer operations);
 // * the steps of a round run for every lane of the wave; a lane past its
-//   ray's end gives its load the out-of-range offset (reads 0) and its
-//   length 0;
+//   ray's end gives its load the out-of-range offset, which reads 0 (its
+//   lengths stay finite, so its products are 0);
 // * each round's visits are summed in visit order with fused multiply-adds
 //   and the round's sum is added to the image with Kahan compensation.
 template <int NFP>
@@ -258,7 +258,8 @@ __device__ __forceinline__ void walk_projection_fast(Walk &w, double max_t, Proj
             const unsigned o = __umul24((unsigned)c0 >> 2, K0) +
                                (__umul24((unsigned)c1 >> 2, K1) + (__umul24((unsigned)c2 >> 2, 60u) + lin));
             off[u] = live ? o * (unsigned)(4 * NFP) : 0xffffffffu;
-            dl[u] = live ? (float)(exit_t - enter_t) : 0.0f;
+            // (a lane past its end reads 0; its length stays finite)
+            dl[u] = (float)(exit_t - enter_t);
             // Step the chosen axis under an execution mask (the empty asm
             // keeps each update a masked region rather than selects).
             if (s0) {
