@@ -550,16 +550,21 @@ def main():
     if not args.no_tune:
         t = time.time()
         cfg, table = autotune.tune(ctx, specs, flags, use_cache=False)
-        ranked = sorted(table, key=lambda x: x[1])
+        ranked, shown = [], set()
+        for c, ms in sorted(table, key=lambda x: x[1]):
+            if ms not in shown:  # one line per timed launch configuration
+                shown.add(ms)
+                ranked.append((c, ms))
         by_key = {json.dumps(c, sort_keys=True): ms for c, ms in table}
         default_ms = by_key.get(json.dumps(autotune.DEFAULT, sort_keys=True), float("nan"))
-        tune_info = {"search": "coordinate descent over tile shape, rays per item, waves per "
-                               "workgroup and workgroups per CU",
-                     "measured": len(table), "seconds": round(time.time() - t, 1),
+        timed = len({ms for _, ms in table})  # configurations with the same launches share a timing
+        tune_info = {"search": "coordinate descent over tile shape, rays per item, transfer-function "
+                               "table bytes per launch, waves per workgroup and workgroups per CU",
+                     "configurations": len(table), "timed": timed, "seconds": round(time.time() - t, 1),
                      "best": autotune.describe(cfg), "best_ms": round(ranked[0][1], 3),
                      "default_ms": round(default_ms, 3),
                      "worst_ms": round(ranked[-1][1], 3)}
-        log(f"autotune: {len(table)} launch shapes in {time.time() - t:.1f}s; best "
+        log(f"autotune: {len(table)} configurations ({timed} timed) in {time.time() - t:.1f}s; best "
             + ", ".join(f"{autotune.describe(c)}={ms:.2f}ms" for c, ms in ranked[:4])
             + f"; default {default_ms:.2f}ms; worst {ranked[-1][1]:.2f}ms")
     batch = autotune.make_batch(ctx, specs, cfg, flags)

@@ -82,6 +82,16 @@ extern "C" {
 #define YTRC_BATCH_ITEM_TILES_2 64u
 #define YTRC_BATCH_ITEM_TILES_4 128u
 #define YTRC_BATCH_ITEM_TILES_8 192u
+/* Transfer-function table bytes one persistent launch stages in each
+ * workgroup's LDS: a launch holds the distinct tables of its requests; with
+ * a cap, a kernel variant's requests are split over launches by table (a
+ * table is never split; one larger than the cap gets a launch of its own).
+ * Fewer LDS bytes per workgroup let more workgroups share a CU.  Images do
+ * not depend on it. */
+#define YTRC_BATCH_TF_LDS_ALL 0u   /* one launch per kernel variant          */
+#define YTRC_BATCH_TF_LDS_32K 256u /* at most 32 KiB of tables per launch    */
+#define YTRC_BATCH_TF_LDS_16K 512u /* at most 16 KiB of tables per launch    */
+#define YTRC_BATCH_TF_LDS_ONE 768u /* one distinct table per launch          */
 
 typedef struct ytrc_ctx ytrc_ctx;
 typedef struct ytrc_batch ytrc_batch;
@@ -287,6 +297,11 @@ int ytrc_batch_set_launch(ytrc_batch *b, int32_t waves_per_group, int32_t groups
 int ytrc_batch_request_ms(ytrc_batch *b, int32_t i, float *ms);
 /* Device bytes held by the batch (descriptors, extents, outputs). */
 int ytrc_batch_bytes(const ytrc_batch *b, uint64_t *bytes);
+/* Persistent batches: the kernel launches one launch of the batch makes
+ * (n_launches) and the largest transfer-function table bytes one of them
+ * stages in LDS (max_tf_bytes); set by the batch flags (YTRC_BATCH_TF_LDS_*).
+ * No counterpart in the reference, which has no launches. */
+int ytrc_batch_launch_plan(const ytrc_batch *b, int32_t *n_launches, uint64_t *max_tf_bytes);
 void ytrc_batch_destroy(ytrc_batch *b);
 
 /* Serving: continuous batching against resident datasets.  The reference

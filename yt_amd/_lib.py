@@ -40,6 +40,9 @@ BATCH_GRAPH = 2
 # rays per 64-lane tile, rows x columns
 BATCH_TILE = {(8, 8): 0, (4, 16): 16, (16, 4): 32}
 BATCH_ITEM_TILES = {1: 0, 2: 64, 4: 128, 8: 192}
+# transfer-function table bytes one persistent launch stages in LDS:
+# "all" (one launch per kernel variant), at most 32 or 16 KiB, one table
+BATCH_TF_LDS = {"all": 0, "32k": 256, "16k": 512, "one": 768}
 
 EXPORTED = [
     "ytrc_ctx_create",
@@ -57,6 +60,7 @@ EXPORTED = [
     "ytrc_batch_read_partial",
     "ytrc_batch_set_launch",
     "ytrc_batch_bytes",
+    "ytrc_batch_launch_plan",
     "ytrc_batch_destroy",
     "ytrc_batch_kernel_ms",
     "ytrc_batch_request_ms",
@@ -236,6 +240,7 @@ def _declare(lib):
         P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_uint64), C.POINTER(C.c_int64)]
     lib.ytrc_batch_set_launch.argtypes = [P, C.c_int32, C.c_int32]
     lib.ytrc_batch_bytes.argtypes = [P, C.POINTER(C.c_uint64)]
+    lib.ytrc_batch_launch_plan.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_uint64)]
     lib.ytrc_batch_destroy.argtypes = [P]
     lib.ytrc_batch_destroy.restype = None
     lib.ytrc_batch_kernel_ms.argtypes = [P, C.POINTER(C.c_float), C.POINTER(C.c_int32)]
@@ -647,16 +652,20 @@ class RequestSpec:
 class Batch:
     """A set of requests staged on the device; launch() runs them."""
 
-    def __init__(self, ctx, specs, flags=BATCH_PLAIN, tile=(8, 8), item_tiles=1):
+    def __init__(self, ctx, specs, flags=BATCH_PLAIN, tile=(8, 8), item_tiles=1, tf_lds="all"):
         self.ctx = ctx
         self.specs = list(specs)
         if tuple(tile) not in BATCH_TILE:
             raise ValueError(f"tile must be one of {sorted(BATCH_TILE)}")
         if item_tiles not in BATCH_ITEM_TILES:
             raise ValueError(f"item_tiles must be one of {sorted(BATCH_ITEM_TILES)}")
+        if tf_lds not in BATCH_TF_LDS:
+            raise ValueError(f"tf_lds must be one of {sorted(BATCH_TF_LDS)}")
         self.tile = tuple(tile)
         self.item_tiles = int(item_tiles)
-        flags = int(flags) | BATCH_TILE[self.tile] | BATCH_ITEM_TILES[self.item_tiles]
+        self.tf_lds = tf_lds
+        flags = (int(flags) | BATCH_TILE[self.tile] | BATCH_ITEM_TILES[self.item_tiles] |
+                 BATCH_TF_LDS[tf_lds])
         n = len(self.specs)
         arr = (Request * n)()
         for i, s in enumerate(self.specs):
@@ -721,6 +730,13 @@ class Batch:
         v = C.c_uint64()
         _check(lib().ytrc_batch_bytes(self._h, C.byref(v)))
         return v.value
+
+    def launch_plan(self):
+        """(kernel launches per launch of a persistent batch, largest LDS
+        bytes of transfer-function tables one of them stages)."""
+        n, b = C.c_int32(), C.c_uint64()
+        _check(lib().ytrc_batch_launch_plan(self._h, C.byref(n), C.byref(b)))
+        return n.value, b.value
 
     def read_image(self, i):
         nv = self.specs[i].nv

@@ -10,9 +10,17 @@ chip:
   * rays per work item: 1, 2, 4 or 8 consecutive tiles (64-512 rays) that
     one wave renders in a row per item of the persistent queue (the
     analog of the prange chunk);
+  * transfer-function table bytes per launch in LDS: every workgroup of a
+    persistent launch stages the distinct tables of the launch's requests;
+    "all" keeps one launch per kernel variant, "32k"/"16k" split a variant's
+    requests over launches so that each stages at most that many bytes,
+    "one" gives every distinct table its own launch (fewer LDS bytes per
+    workgroup, more workgroups per CU, more launches).  Values that give
+    the same launches (a batch whose variants each have one table) are
+    measured once;
   * waves per workgroup: 1, 2 or 4 tiles launched together, which share
-    one copy of the batch's transfer-function tables in LDS (LDS per wave:
-    table bytes / waves per workgroup);
+    one copy of the launch's tables in LDS (LDS per wave: table bytes /
+    waves per workgroup);
   * workgroups per CU for the persistent grid (0 = the occupancy limit).
 
 ``tune`` measures candidates on the caller's own batch (HIP events around
@@ -39,8 +47,9 @@ TILES = ((8, 8), (4, 16), (16, 4))
 ITEM_TILES = (1, 2, 4, 8)
 WAVES_PER_GROUP = (1, 2, 4)
 GROUPS_PER_CU = (0, 1, 2, 4, 8)
-DEFAULT = {"tile": [8, 8], "item_tiles": 1, "waves_per_group": 4, "groups_per_cu": 0}
-KNOBS = ("tile", "item_tiles", "waves_per_group", "groups_per_cu")
+TF_LDS = ("all", "32k", "16k", "one")
+DEFAULT = {"tile": [8, 8], "item_tiles": 1, "tf_lds": "all", "waves_per_group": 4, "groups_per_cu": 0}
+KNOBS = ("tile", "item_tiles", "tf_lds", "waves_per_group", "groups_per_cu")
 
 
 def cache_path():
@@ -93,36 +102,57 @@ def _full(cfg):
 def make_batch(ctx, specs, cfg, flags=_lib.BATCH_PERSISTENT | _lib.BATCH_GRAPH):
     """A batch built and launched with the launch shape ``cfg``."""
     cfg = _full(cfg)
-    b = _lib.Batch(ctx, specs, flags, tile=tuple(cfg["tile"]), item_tiles=int(cfg["item_tiles"]))
+    b = _lib.Batch(ctx, specs, flags, tile=tuple(cfg["tile"]), item_tiles=int(cfg["item_tiles"]),
+                   tf_lds=cfg["tf_lds"])
     b.set_launch(cfg["waves_per_group"], cfg["groups_per_cu"])
     return b
 
 
 class _Timer:
     """Times launch shapes on one set of specs, reusing one batch per
-    (tile shape, rays per item) and each measurement per configuration."""
+    (tile shape, rays per item, launch plan) and each measurement per
+    configuration; table-size values that give the same launches share a
+    batch and a measurement."""
 
     def __init__(self, ctx, specs, flags, reps, log):
         self.ctx, self.specs, self.flags, self.reps, self.log = ctx, specs, flags, reps, log
-        self.batches = {}
+        self.batches = {}  # (tile, item_tiles, n_launches) -> batch
+        self.alias = {}    # (tile, item_tiles, tf_lds) -> that key
+        self.ms = {}       # (batch key, waves, groups) -> ms
         self.seen = {}
+
+    def _batch(self, cfg):
+        ak = (tuple(cfg["tile"]), cfg["item_tiles"], cfg["tf_lds"])
+        if ak not in self.alias:
+            b = _lib.Batch(self.ctx, self.specs, self.flags, tile=ak[0], item_tiles=ak[1],
+                           tf_lds=ak[2])
+            bk = (ak[0], ak[1], b.launch_plan()[0])
+            if bk in self.batches:
+                b.close()  # the same launches as a batch already built
+            else:
+                self.batches[bk] = b
+            self.alias[ak] = bk
+        bk = self.alias[ak]
+        return bk, self.batches[bk]
 
     def __call__(self, cfg):
         key = json.dumps(cfg, sort_keys=True)
         if key in self.seen:
             return self.seen[key]
-        bk = (tuple(cfg["tile"]), cfg["item_tiles"])
-        if bk not in self.batches:
-            self.batches[bk] = _lib.Batch(self.ctx, self.specs, self.flags, tile=bk[0],
-                                          item_tiles=bk[1])
-        b = self.batches[bk]
-        b.set_launch(cfg["waves_per_group"], cfg["groups_per_cu"])
-        b.run()
-        ms = min(b.run() for _ in range(max(1, self.reps)))
-        self.seen[key] = ms
-        if self.log:
-            self.log(f"autotune {describe(cfg)}: {ms:.3f} ms")
-        return ms
+        bk, b = self._batch(cfg)
+        mk = (bk, cfg["waves_per_group"], cfg["groups_per_cu"])
+        if mk not in self.ms:
+            b.set_launch(cfg["waves_per_group"], cfg["groups_per_cu"])
+            b.run()
+            self.ms[mk] = min(b.run() for _ in range(max(1, self.reps)))
+            if self.log:
+                self.log(f"autotune {describe(cfg)}: {self.ms[mk]:.3f} ms")
+        self.seen[key] = self.ms[mk]
+        return self.ms[mk]
+
+    def measured(self):
+        """Distinct launch configurations actually timed."""
+        return len(self.ms)
 
     def close(self):
         for b in self.batches.values():
@@ -132,13 +162,13 @@ class _Timer:
 
 def describe(cfg):
     cfg = _full(cfg)
-    return (f"{cfg['tile'][0]}x{cfg['tile'][1]}/i{cfg['item_tiles']}/w{cfg['waves_per_group']}"
-            f"/g{cfg['groups_per_cu']}")
+    return (f"{cfg['tile'][0]}x{cfg['tile'][1]}/i{cfg['item_tiles']}/t{cfg['tf_lds']}"
+            f"/w{cfg['waves_per_group']}/g{cfg['groups_per_cu']}")
 
 
 def tune(ctx, specs, flags=_lib.BATCH_PERSISTENT | _lib.BATCH_GRAPH, tiles=TILES,
          item_tiles=ITEM_TILES, waves_per_group=WAVES_PER_GROUP, groups_per_cu=GROUPS_PER_CU,
-         reps=2, use_cache=True, path=None, log=None, search="descent", rounds=3):
+         reps=2, use_cache=True, path=None, log=None, search="descent", rounds=3, tf_lds=TF_LDS):
     """Returns (best config dict, list of (config, ms) measured).  With
     ``use_cache`` a cached entry for this signature is returned unmeasured."""
     path = path or cache_path()
@@ -147,7 +177,7 @@ def tune(ctx, specs, flags=_lib.BATCH_PERSISTENT | _lib.BATCH_GRAPH, tiles=TILES
         hit = _load(path).get(key)
         if hit is not None:
             return _full(hit["cfg"]), []
-    values = {"tile": [list(t) for t in tiles], "item_tiles": list(item_tiles),
+    values = {"tile": [list(t) for t in tiles], "item_tiles": list(item_tiles), "tf_lds": list(tf_lds),
               "waves_per_group": list(waves_per_group), "groups_per_cu": list(groups_per_cu)}
     t0 = time.time()
     timer = _Timer(ctx, specs, flags, reps, log)
@@ -155,10 +185,11 @@ def tune(ctx, specs, flags=_lib.BATCH_PERSISTENT | _lib.BATCH_GRAPH, tiles=TILES
         if search == "grid":
             for tile in values["tile"]:
                 for it in values["item_tiles"]:
-                    for wpg in values["waves_per_group"]:
-                        for gpc in values["groups_per_cu"]:
-                            timer({"tile": tile, "item_tiles": it, "waves_per_group": wpg,
-                                   "groups_per_cu": gpc})
+                    for tl in values["tf_lds"]:
+                        for wpg in values["waves_per_group"]:
+                            for gpc in values["groups_per_cu"]:
+                                timer({"tile": tile, "item_tiles": it, "tf_lds": tl,
+                                       "waves_per_group": wpg, "groups_per_cu": gpc})
             best = min(timer.seen.items(), key=lambda kv: kv[1])
             best = json.loads(best[0])
         elif search == "descent":
@@ -188,6 +219,7 @@ def tune(ctx, specs, flags=_lib.BATCH_PERSISTENT | _lib.BATCH_GRAPH, tiles=TILES
     if use_cache:
         d = _load(path)
         d[key] = {"cfg": best, "ms": min(t[1] for t in table), "searched": len(table),
+                  "timed": timer.measured(),
                   "seconds": round(time.time() - t0, 2)}
         _store(path, d)
     return _full(best), table

@@ -324,6 +324,8 @@ struct PGroup {
     int counter = 0;
     size_t lds = 0;
     int grid = 0;
+    int tf_base = 0;  // the launch's tables: float4 offset in the batch table
+    int tf_bins = 0;  // and length (what each workgroup stages in LDS)
 };
 
 struct ytrc_batch {
@@ -1372,24 +1374,82 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
         delete B;
         return fail(code, m);
     };
-    // Unique transfer-function tables share one slot of the batch table.
-    std::vector<float4> tf_all;
-    std::map<std::string, int> tf_slot_of;
-    std::vector<int> slot(n_req, 0);
+    // Persistent launches: one per kernel variant, items = (request, tile).
+    // A launch's workgroups stage its requests' distinct transfer-function
+    // tables in LDS, so with several tables per variant the batch flag
+    // YTRC_BATCH_TF_LDS_* caps the table bytes of one launch: the variant's
+    // requests are then split over launches by table (a table is never
+    // split; one larger than the cap gets a launch of its own).  Fewer LDS
+    // bytes per workgroup let more workgroups share a CU.
+    const int lcode = (int)((batch_flags >> 8) & 3);
+    const size_t tf_cap = lcode == 1 ? (32u << 10) : (lcode == 2 ? (16u << 10) : (lcode == 3 ? 1u : 0u));
+    std::map<std::tuple<int, int, int, int>, std::vector<int>> gmap;
     for (int i = 0; i < n_req; i++) {
-        if (!has_tf(reqs[i].kind)) continue;
-        std::string key((const char *)P[i].tft.data(), P[i].tft.size() * sizeof(float4));
-        auto it = tf_slot_of.find(key);
-        if (it == tf_slot_of.end()) {
-            const int b0 = (int)tf_all.size();
-            tf_all.insert(tf_all.end(), P[i].tft.begin(), P[i].tft.end());
-            it = tf_slot_of.emplace(key, b0).first;
+        const bool rec = (reqs[i].flags & YTRC_FLAG_RECORD) != 0;
+        gmap[std::make_tuple(reqs[i].kind, P[i].q.nfp, rec ? 1 : 0, P[i].q.mode)].push_back(i);
+    }
+    struct LaunchPlan {
+        std::tuple<int, int, int, int> variant;
+        std::vector<int> ids;
+        std::vector<std::string> keys;  // its distinct tables, in first-use order
+        size_t table_bytes = 0;
+    };
+    std::vector<LaunchPlan> plans;
+    for (auto &kv : gmap) {
+        const bool tfk = has_tf(std::get<0>(kv.first));
+        std::vector<LaunchPlan> mine;
+        std::map<std::string, int> plan_of;  // table -> index in mine
+        for (int i : kv.second) {
+            std::string key;
+            size_t tb = 0;
+            if (tfk) {
+                key.assign((const char *)P[i].tft.data(), P[i].tft.size() * sizeof(float4));
+                tb = key.size();
+            }
+            auto it = plan_of.find(key);
+            if (it == plan_of.end()) {
+                int pi = -1;
+                if (!mine.empty() && (tf_cap == 0 || mine.back().keys.empty() ||
+                                      mine.back().table_bytes + tb <= tf_cap))
+                    pi = (int)mine.size() - 1;
+                if (pi < 0) {
+                    mine.emplace_back();
+                    mine.back().variant = kv.first;
+                    pi = (int)mine.size() - 1;
+                }
+                if (tfk) {
+                    mine[pi].keys.push_back(key);
+                    mine[pi].table_bytes += tb;
+                }
+                it = plan_of.emplace(key, pi).first;
+            }
+            mine[it->second].ids.push_back(i);
         }
-        slot[i] = it->second;
+        for (auto &m : mine) plans.push_back(std::move(m));
+    }
+    // The batch table: each launch's distinct tables, launch after launch.
+    std::vector<float4> tf_all;
+    std::vector<int> slot(n_req, 0), rel_slot(n_req, 0);
+    std::vector<int> plan_base(plans.size(), 0), plan_bins(plans.size(), 0);
+    for (size_t L = 0; L < plans.size(); L++) {
+        const int b0 = (int)tf_all.size();
+        plan_base[L] = b0;
+        std::map<std::string, int> at;
+        for (const std::string &k : plans[L].keys) {
+            at[k] = (int)tf_all.size();
+            const float4 *q = (const float4 *)k.data();
+            tf_all.insert(tf_all.end(), q, q + k.size() / sizeof(float4));
+        }
+        plan_bins[L] = (int)tf_all.size() - b0;
+        if (!plans[L].keys.empty())
+            for (int i : plans[L].ids) {
+                const std::string key((const char *)P[i].tft.data(), P[i].tft.size() * sizeof(float4));
+                slot[i] = at[key];
+                rel_slot[i] = slot[i] - b0;
+            }
     }
     const size_t tfall_off = off;
     off = align_up(off + sizeof(float4) * std::max<size_t>(1, tf_all.size()), 256);
-    // Persistent groups: one per kernel variant, items = (request, tile).
     // Each image's tiles are cut into QSPLIT contiguous chunks and chunk r
     // goes to queue range r (the range one XCD drains first), so every XCD
     // renders a compact part of every image; within a range the images
@@ -1398,15 +1458,10 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
     // sparser than the cells and wait on memory, then run beside waves of
     // large images, which keep the VALUs busy: 5-6 % faster than running
     // the images one after another, larger first.
-    std::map<std::tuple<int, int, int, int>, std::vector<int>> gmap;
-    for (int i = 0; i < n_req; i++) {
-        const bool rec = (reqs[i].flags & YTRC_FLAG_RECORD) != 0;
-        gmap[std::make_tuple(reqs[i].kind, P[i].q.nfp, rec ? 1 : 0, P[i].q.mode)].push_back(i);
-    }
     std::vector<std::vector<int2>> gitems;
     std::vector<size_t> gitems_off;
-    for (auto &kv : gmap) {
-        std::vector<int> ids = kv.second;
+    for (size_t L = 0; L < plans.size(); L++) {
+        std::vector<int> ids = plans[L].ids;
         std::stable_sort(ids.begin(), ids.end(),
                          [&](int a, int b) { return P[a].q.ntiles > P[b].q.ntiles; });
         std::vector<int2> items;
@@ -1434,16 +1489,18 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
             for (const Key &k : ks) items.push_back(k.item);
         }
         PGroup g;
-        g.kind = std::get<0>(kv.first);
-        g.nfp = std::get<1>(kv.first);
-        g.rec = std::get<2>(kv.first) != 0;
-        g.mode = std::get<3>(kv.first);
+        g.kind = std::get<0>(plans[L].variant);
+        g.nfp = std::get<1>(plans[L].variant);
+        g.rec = std::get<2>(plans[L].variant) != 0;
+        g.mode = std::get<3>(plans[L].variant);
         g.fn = pick_persist(g.kind, g.nfp, g.rec, g.mode);
         g.n_items = (int)items.size();
         g.counter = (int)B->groups.size();
-        g.lds = has_tf(g.kind) ? sizeof(float4) * tf_all.size() : 0;
+        g.tf_base = plan_base[L];
+        g.tf_bins = plan_bins[L];
+        g.lds = has_tf(g.kind) ? sizeof(float4) * (size_t)g.tf_bins : 0;
         if (g.lds > ctx->max_lds)
-            return bad(YTRC_EINVAL, "ytrc_batch_create: the batch's distinct transfer functions need " +
+            return bad(YTRC_EINVAL, "ytrc_batch_create: the distinct transfer functions of one launch need " +
                                         std::to_string(g.lds) + " B of LDS, more than the " +
                                         std::to_string(ctx->max_lds) + " B a workgroup may allocate on this device");
         if (g.lds > 64 * 1024) {
@@ -1473,6 +1530,8 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
         const int rc = stage_request(ctx, P[i], A + base[i], (const float4 *)(A + tfall_off), slot[i],
                                      B->h_imgs[i], stage);
         if (rc != YTRC_OK) return bad(rc, g_err);
+        // the persistent kernels read the table from their launch's block
+        B->h_imgs[i].tf_slot = rel_slot[i];
     }
     B->tf_all = (const float4 *)(A + tfall_off);
     B->tf_bins = (int)tf_all.size();
@@ -1519,6 +1578,23 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
     return YTRC_OK;
 }
 
+int ytrc_batch_launch_plan(const ytrc_batch *b, int32_t *n_launches, uint64_t *max_tf_bytes)
+{
+    if (b == nullptr || n_launches == nullptr || max_tf_bytes == nullptr)
+        return fail(YTRC_EINVAL, "ytrc_batch_launch_plan: NULL argument");
+    int n = 0;
+    uint64_t mx = 0;
+    if (b->flags & YTRC_BATCH_PERSISTENT)
+        for (const PGroup &g : b->groups)
+            if (g.n_items > 0) {
+                n++;
+                mx = std::max<uint64_t>(mx, g.lds);
+            }
+    *n_launches = n;
+    *max_tf_bytes = mx;
+    return YTRC_OK;
+}
+
 int ytrc_batch_set_launch(ytrc_batch *b, int32_t waves_per_group, int32_t groups_per_cu)
 {
     if (b == nullptr) return fail(YTRC_EINVAL, "ytrc_batch_set_launch: NULL batch");
@@ -1557,8 +1633,8 @@ int enqueue_persistent(ytrc_batch *b, hipStream_t s)
         g.grid = std::max(1, std::min(per_cu * b->ctx->n_cu,
                                       (g.n_items + b->waves_per_group - 1) / b->waves_per_group));
         hipLaunchKernelGGL(g.fn, dim3(g.grid), dim3(threads), g.lds, s, (const DImg *)b->d_imgs,
-                           g.items, g.n_items, b->counters + (size_t)g.counter * QSPLIT, b->tf_all,
-                           b->tf_bins);
+                           g.items, g.n_items, b->counters + (size_t)g.counter * QSPLIT, b->tf_all + g.tf_base,
+                           g.tf_bins);
         HIPCHK(hipGetLastError());
     }
     return YTRC_OK;
