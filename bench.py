@@ -494,6 +494,9 @@ def main():
                     help="skip the 64-brick (128^3 bricks) variant of the batched step")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the launch-shape autotuner (8x8 tiles, 4 waves/group, occupancy grid)")
+    ap.add_argument("--launch", default=None,
+                    help="JSON object of launch knobs (autotune.KNOBS) applied after tuning, "
+                         'e.g. \'{"vertex_lds": true}\'')
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_k_persist.json"),
                     help="measured HBM bytes per dispatch (rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
@@ -559,7 +562,8 @@ def main():
         default_ms = by_key.get(json.dumps(autotune.DEFAULT, sort_keys=True), float("nan"))
         timed = len({ms for _, ms in table})  # configurations with the same launches share a timing
         tune_info = {"search": "coordinate descent over tile shape, rays per item, transfer-function "
-                               "table bytes per launch, waves per workgroup and workgroups per CU",
+                               "table bytes per launch, vertex blocks in LDS, waves per workgroup and "
+                               "workgroups per CU",
                      "configurations": len(table), "timed": timed, "seconds": round(time.time() - t, 1),
                      "best": autotune.describe(cfg), "best_ms": round(ranked[0][1], 3),
                      "default_ms": round(default_ms, 3),
@@ -567,6 +571,13 @@ def main():
         log(f"autotune: {len(table)} configurations ({timed} timed) in {time.time() - t:.1f}s; best "
             + ", ".join(f"{autotune.describe(c)}={ms:.2f}ms" for c, ms in ranked[:4])
             + f"; default {default_ms:.2f}ms; worst {ranked[-1][1]:.2f}ms")
+    if args.launch:
+        extra = json.loads(args.launch)
+        unknown = set(extra) - set(autotune.KNOBS)
+        if unknown:
+            raise SystemExit(f"--launch: unknown knobs {sorted(unknown)}")
+        cfg = dict(autotune._full(cfg), **extra)
+        log(f"launch shape fixed to {autotune.describe(cfg)}")
     batch = autotune.make_batch(ctx, specs, cfg, flags)
     for _ in range(args.warmup):
         batch.run()

@@ -92,6 +92,12 @@ extern "C" {
 #define YTRC_BATCH_TF_LDS_32K 256u /* at most 32 KiB of tables per launch    */
 #define YTRC_BATCH_TF_LDS_16K 512u /* at most 16 KiB of tables per launch    */
 #define YTRC_BATCH_TF_LDS_ONE 768u /* one distinct table per launch          */
+/* Persistent batches: volume renders with a ColorTransferFunction, one
+ * field and 10 sub-samples stage vertex blocks in LDS, 512 bytes per wave:
+ * at a visit where all lanes of a wave are in one block, the wave reads it
+ * once and takes the corners from LDS (the L2 path otherwise).  Images do
+ * not depend on it. */
+#define YTRC_BATCH_VERTEX_LDS 1024u
 
 typedef struct ytrc_ctx ytrc_ctx;
 typedef struct ytrc_batch ytrc_batch;

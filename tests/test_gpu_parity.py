@@ -337,8 +337,8 @@ def test_autotune_picks_a_working_shape(ctx, tmp_path):
     knobs, and the default coordinate descent), caches the fastest, returns
     the cached entry unmeasured the second time, and every launch shape it
     can choose -- tile shape, rays per work item (1-8 tiles), waves per
-    workgroup, workgroups per CU, table bytes per launch -- renders the same
-    images as the default."""
+    workgroup, workgroups per CU, table bytes per launch, vertex blocks in
+    LDS -- renders the same images as the default."""
     from yt_amd import autotune
 
     ds = uniform_lognormal(32, seed=71)
@@ -351,7 +351,7 @@ def test_autotune_picks_a_working_shape(ctx, tmp_path):
         best, table = autotune.tune(ctx, specs, item_tiles=(1, 4), waves_per_group=(1, 4),
                                     groups_per_cu=(0, 2), tf_lds=("all", "one"), reps=1, path=path,
                                     search="grid")
-        assert len(table) == 3 * 2 * 2 * 2 * 2 and all(ms > 0 for _, ms in table)
+        assert len(table) == 3 * 2 * 2 * 2 * 2 * 2 and all(ms > 0 for _, ms in table)
         # one table per variant: both table-size values give the same
         # launches and share one measurement
         by = {json.dumps(c, sort_keys=True): ms for c, ms in table}
@@ -364,10 +364,11 @@ def test_autotune_picks_a_working_shape(ctx, tmp_path):
         assert "torch" not in sys.modules
         assert "gfx" in ctx.device_name
         d_best, d_table = autotune.tune(ctx, specs, reps=1, use_cache=False)
-        assert 0 < len(d_table) < 3 * 4 * 4 * 3 * 5
+        assert 0 < len(d_table) < 3 * 4 * 4 * 2 * 3 * 5
         base = _run(ctx, specs)
         for cfg in [best, d_best, dict(autotune.DEFAULT, item_tiles=2),
-                    dict(autotune.DEFAULT, item_tiles=8, tile=[4, 16], waves_per_group=1)]:
+                    dict(autotune.DEFAULT, item_tiles=8, tile=[4, 16], waves_per_group=1),
+                    dict(autotune.DEFAULT, vertex_lds=True, waves_per_group=2)]:
             b = autotune.make_batch(ctx, specs, cfg)
             try:
                 b.run()

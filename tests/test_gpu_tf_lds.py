@@ -59,3 +59,37 @@ def test_tf_lds_cap_splits_launches_and_keeps_images(ctx):
 def test_tf_lds_flag_rejects_unknown():
     with pytest.raises(ValueError):
         _lib.Batch(None, [], _lib.BATCH_PERSISTENT, tf_lds="8k")
+
+
+def test_vertex_blocks_in_lds_keep_images(ctx):
+    """YTRC_BATCH_VERTEX_LDS: volume renders with a ColorTransferFunction,
+    one field and 10 sub-samples read their corners from a per-wave LDS copy
+    of the vertex block where all lanes of a wave are in one block.  The
+    values are the same, so the images equal those of the L2 path bit for
+    bit: a uniform field in 8 bricks at image sides from 48 (rays several
+    cells apart) to 320 (rays closer than cells), and an AMR set whose child
+    masks skip cells; every launch shape the flag combines with."""
+    from .helpers import small_amr
+
+    for ds, sides in ((uniform_lognormal(64, seed=93, nprocs=8), (48, 160, 320)),
+                      (small_amr(root_n=32, levels=2, seed=94), (96, 256))):
+        vol = render.ResidentVolume(ctx, ds, vr_field="density")
+        try:
+            tf = vr_tf_for(ds, nbins=256)
+            specs = [vol.vr_request(c0_params(n, normal=NORMALS[k]), tf) for k, n in enumerate(sides)]
+            assert all(s.n_samples == 10 for s in specs)
+            flags = _lib.BATCH_PERSISTENT | _lib.BATCH_GRAPH
+            base, _ = _images(ctx, specs, flags)
+            for kw in ({}, {"tile": (4, 16)}, {"item_tiles": 4}):
+                for wpg in (1, 4):
+                    b = _lib.Batch(ctx, specs, flags, vertex_lds=True, **kw)
+                    try:
+                        b.set_launch(wpg, 0)
+                        b.run()
+                        b.run()
+                        for i in range(len(specs)):
+                            assert np.array_equal(b.read_image(i), base[i]), (kw, wpg, i)
+                    finally:
+                        b.close()
+        finally:
+            vol.free()

@@ -342,12 +342,13 @@ def test_serving_summary_and_p99_interval():
 
 
 def test_autotune_descent_finds_a_separable_optimum(monkeypatch, tmp_path):
-    """Coordinate descent over the five launch knobs reaches the minimum of a
+    """Coordinate descent over the six launch knobs reaches the minimum of a
     separable cost (what the knobs' effects roughly are) with far fewer
     measurements than the grid, and the grid finds the same point."""
     from yt_amd import autotune
 
-    target = {"tile": [4, 16], "item_tiles": 4, "tf_lds": "16k", "waves_per_group": 2, "groups_per_cu": 2}
+    target = {"tile": [4, 16], "item_tiles": 4, "tf_lds": "16k", "vertex_lds": True, "waves_per_group": 2,
+              "groups_per_cu": 2}
 
     class FakeTimer:
         def __init__(self, ctx, specs, flags, reps, log):
@@ -370,8 +371,8 @@ def test_autotune_descent_finds_a_separable_optimum(monkeypatch, tmp_path):
     monkeypatch.setattr(autotune, "_Timer", FakeTimer)
     best, table = autotune.tune(Ctx(), [], use_cache=False)
     assert best == target
-    assert len(table) < 3 * 4 * 4 * 3 * 5 // 3
+    assert len(table) < 3 * 4 * 4 * 2 * 3 * 5 // 3
     gbest, gtable = autotune.tune(Ctx(), [], use_cache=False, search="grid")
-    assert gbest == target and len(gtable) == 3 * 4 * 4 * 3 * 5
+    assert gbest == target and len(gtable) == 3 * 4 * 4 * 2 * 3 * 5
     with pytest.raises(ValueError):
         autotune.tune(Ctx(), [], use_cache=False, search="random")

@@ -43,6 +43,9 @@ BATCH_ITEM_TILES = {1: 0, 2: 64, 4: 128, 8: 192}
 # transfer-function table bytes one persistent launch stages in LDS:
 # "all" (one launch per kernel variant), at most 32 or 16 KiB, one table
 BATCH_TF_LDS = {"all": 0, "32k": 256, "16k": 512, "one": 768}
+# persistent volume renders (ColorTransferFunction, one field, 10 sub-samples)
+# stage vertex blocks in LDS where a wave's lanes share one
+BATCH_VERTEX_LDS = 1024
 
 EXPORTED = [
     "ytrc_ctx_create",
@@ -652,7 +655,8 @@ class RequestSpec:
 class Batch:
     """A set of requests staged on the device; launch() runs them."""
 
-    def __init__(self, ctx, specs, flags=BATCH_PLAIN, tile=(8, 8), item_tiles=1, tf_lds="all"):
+    def __init__(self, ctx, specs, flags=BATCH_PLAIN, tile=(8, 8), item_tiles=1, tf_lds="all",
+                 vertex_lds=False):
         self.ctx = ctx
         self.specs = list(specs)
         if tuple(tile) not in BATCH_TILE:
@@ -664,8 +668,9 @@ class Batch:
         self.tile = tuple(tile)
         self.item_tiles = int(item_tiles)
         self.tf_lds = tf_lds
+        self.vertex_lds = bool(vertex_lds)
         flags = (int(flags) | BATCH_TILE[self.tile] | BATCH_ITEM_TILES[self.item_tiles] |
-                 BATCH_TF_LDS[tf_lds])
+                 BATCH_TF_LDS[tf_lds] | (BATCH_VERTEX_LDS if self.vertex_lds else 0))
         n = len(self.specs)
         arr = (Request * n)()
         for i, s in enumerate(self.specs):

@@ -18,6 +18,10 @@ chip:
     workgroup, more workgroups per CU, more launches).  Values that give
     the same launches (a batch whose variants each have one table) are
     measured once;
+  * vertex blocks staged in LDS (volume renders with a ColorTransferFunction,
+    one field, 10 sub-samples): off, or each wave keeps one 512-byte vertex
+    block in LDS and reads a visit's corners from it when all its lanes are
+    in that block;
   * waves per workgroup: 1, 2 or 4 tiles launched together, which share
     one copy of the launch's tables in LDS (LDS per wave: table bytes /
     waves per workgroup);
@@ -48,8 +52,10 @@ ITEM_TILES = (1, 2, 4, 8)
 WAVES_PER_GROUP = (1, 2, 4)
 GROUPS_PER_CU = (0, 1, 2, 4, 8)
 TF_LDS = ("all", "32k", "16k", "one")
-DEFAULT = {"tile": [8, 8], "item_tiles": 1, "tf_lds": "all", "waves_per_group": 4, "groups_per_cu": 0}
-KNOBS = ("tile", "item_tiles", "tf_lds", "waves_per_group", "groups_per_cu")
+VERTEX_LDS = (False, True)
+DEFAULT = {"tile": [8, 8], "item_tiles": 1, "tf_lds": "all", "vertex_lds": False, "waves_per_group": 4,
+           "groups_per_cu": 0}
+KNOBS = ("tile", "item_tiles", "tf_lds", "vertex_lds", "waves_per_group", "groups_per_cu")
 
 
 def cache_path():
@@ -103,7 +109,7 @@ def make_batch(ctx, specs, cfg, flags=_lib.BATCH_PERSISTENT | _lib.BATCH_GRAPH):
     """A batch built and launched with the launch shape ``cfg``."""
     cfg = _full(cfg)
     b = _lib.Batch(ctx, specs, flags, tile=tuple(cfg["tile"]), item_tiles=int(cfg["item_tiles"]),
-                   tf_lds=cfg["tf_lds"])
+                   tf_lds=cfg["tf_lds"], vertex_lds=bool(cfg["vertex_lds"]))
     b.set_launch(cfg["waves_per_group"], cfg["groups_per_cu"])
     return b
 
@@ -116,17 +122,17 @@ class _Timer:
 
     def __init__(self, ctx, specs, flags, reps, log):
         self.ctx, self.specs, self.flags, self.reps, self.log = ctx, specs, flags, reps, log
-        self.batches = {}  # (tile, item_tiles, n_launches) -> batch
-        self.alias = {}    # (tile, item_tiles, tf_lds) -> that key
+        self.batches = {}  # (tile, item_tiles, vertex_lds, n_launches) -> batch
+        self.alias = {}    # (tile, item_tiles, vertex_lds, tf_lds) -> that key
         self.ms = {}       # (batch key, waves, groups) -> ms
         self.seen = {}
 
     def _batch(self, cfg):
-        ak = (tuple(cfg["tile"]), cfg["item_tiles"], cfg["tf_lds"])
+        ak = (tuple(cfg["tile"]), cfg["item_tiles"], bool(cfg["vertex_lds"]), cfg["tf_lds"])
         if ak not in self.alias:
             b = _lib.Batch(self.ctx, self.specs, self.flags, tile=ak[0], item_tiles=ak[1],
-                           tf_lds=ak[2])
-            bk = (ak[0], ak[1], b.launch_plan()[0])
+                           vertex_lds=ak[2], tf_lds=ak[3])
+            bk = (ak[0], ak[1], ak[2], b.launch_plan()[0])
             if bk in self.batches:
                 b.close()  # the same launches as a batch already built
             else:
@@ -163,12 +169,13 @@ class _Timer:
 def describe(cfg):
     cfg = _full(cfg)
     return (f"{cfg['tile'][0]}x{cfg['tile'][1]}/i{cfg['item_tiles']}/t{cfg['tf_lds']}"
-            f"/w{cfg['waves_per_group']}/g{cfg['groups_per_cu']}")
+            f"/v{int(bool(cfg['vertex_lds']))}/w{cfg['waves_per_group']}/g{cfg['groups_per_cu']}")
 
 
 def tune(ctx, specs, flags=_lib.BATCH_PERSISTENT | _lib.BATCH_GRAPH, tiles=TILES,
          item_tiles=ITEM_TILES, waves_per_group=WAVES_PER_GROUP, groups_per_cu=GROUPS_PER_CU,
-         reps=2, use_cache=True, path=None, log=None, search="descent", rounds=3, tf_lds=TF_LDS):
+         reps=2, use_cache=True, path=None, log=None, search="descent", rounds=3, tf_lds=TF_LDS,
+         vertex_lds=VERTEX_LDS):
     """Returns (best config dict, list of (config, ms) measured).  With
     ``use_cache`` a cached entry for this signature is returned unmeasured."""
     path = path or cache_path()
@@ -178,7 +185,8 @@ def tune(ctx, specs, flags=_lib.BATCH_PERSISTENT | _lib.BATCH_GRAPH, tiles=TILES
         if hit is not None:
             return _full(hit["cfg"]), []
     values = {"tile": [list(t) for t in tiles], "item_tiles": list(item_tiles), "tf_lds": list(tf_lds),
-              "waves_per_group": list(waves_per_group), "groups_per_cu": list(groups_per_cu)}
+              "vertex_lds": list(vertex_lds), "waves_per_group": list(waves_per_group),
+              "groups_per_cu": list(groups_per_cu)}
     t0 = time.time()
     timer = _Timer(ctx, specs, flags, reps, log)
     try:
@@ -186,10 +194,11 @@ def tune(ctx, specs, flags=_lib.BATCH_PERSISTENT | _lib.BATCH_GRAPH, tiles=TILES
             for tile in values["tile"]:
                 for it in values["item_tiles"]:
                     for tl in values["tf_lds"]:
-                        for wpg in values["waves_per_group"]:
-                            for gpc in values["groups_per_cu"]:
-                                timer({"tile": tile, "item_tiles": it, "tf_lds": tl,
-                                       "waves_per_group": wpg, "groups_per_cu": gpc})
+                        for vl in values["vertex_lds"]:
+                            for wpg in values["waves_per_group"]:
+                                for gpc in values["groups_per_cu"]:
+                                    timer({"tile": tile, "item_tiles": it, "tf_lds": tl, "vertex_lds": vl,
+                                           "waves_per_group": wpg, "groups_per_cu": gpc})
             best = min(timer.seen.items(), key=lambda kv: kv[1])
             best = json.loads(best[0])
         elif search == "descent":

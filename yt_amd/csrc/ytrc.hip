@@ -208,7 +208,7 @@ pfn_t pick_persist(int kind, int nfp, bool rec, int mode)
     if (kind == 2) return pick_persist_iproj(nfp, rec, m);
     if (kind == 3) return pick_persist_light(nfp, rec, m);
     if (m < 2) return pick_persist_vr_a(nfp, rec, m);
-    if (m < 4) return pick_persist_vr_b(nfp, rec, m);
+    if (m < 4 || m == 19) return pick_persist_vr_b(nfp, rec, m);
     if (m < 8) return pick_persist_vr_c(nfp, rec, m);
     return pick_persist_vr_d(nfp, rec, m);
 }
@@ -1383,10 +1383,16 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
     // bytes per workgroup let more workgroups share a CU.
     const int lcode = (int)((batch_flags >> 8) & 3);
     const size_t tf_cap = lcode == 1 ? (32u << 10) : (lcode == 2 ? (16u << 10) : (lcode == 3 ? 1u : 0u));
+    // YTRC_BATCH_VERTEX_LDS: the volume renders that take the ColorTransferFunction
+    // fast walk with 10 sub-samples (mode 3, one field, no record) run the
+    // variant that stages vertex blocks in LDS (mode bit 4).
+    const bool vlds = (batch_flags & YTRC_BATCH_VERTEX_LDS) != 0;
     std::map<std::tuple<int, int, int, int>, std::vector<int>> gmap;
     for (int i = 0; i < n_req; i++) {
         const bool rec = (reqs[i].flags & YTRC_FLAG_RECORD) != 0;
-        gmap[std::make_tuple(reqs[i].kind, P[i].q.nfp, rec ? 1 : 0, P[i].q.mode)].push_back(i);
+        int mode = P[i].q.mode;
+        if (vlds && reqs[i].kind == YTRC_VOLUME_RENDER && P[i].q.nfp == 1 && !rec && mode == 3) mode |= 16;
+        gmap[std::make_tuple(reqs[i].kind, P[i].q.nfp, rec ? 1 : 0, mode)].push_back(i);
     }
     struct LaunchPlan {
         std::tuple<int, int, int, int> variant;
@@ -1499,13 +1505,15 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
         g.tf_base = plan_base[L];
         g.tf_bins = plan_bins[L];
         g.lds = has_tf(g.kind) ? sizeof(float4) * (size_t)g.tf_bins : 0;
-        if (g.lds > ctx->max_lds)
+        // with mode bit 4, up to 4 waves' vertex-block buffers follow the tables
+        const size_t lds_most = g.lds + ((g.mode & 16) ? 4 * VBP * sizeof(float) : 0);
+        if (lds_most > ctx->max_lds)
             return bad(YTRC_EINVAL, "ytrc_batch_create: the distinct transfer functions of one launch need " +
-                                        std::to_string(g.lds) + " B of LDS, more than the " +
+                                        std::to_string(lds_most) + " B of LDS, more than the " +
                                         std::to_string(ctx->max_lds) + " B a workgroup may allocate on this device");
-        if (g.lds > 64 * 1024) {
+        if (lds_most > 64 * 1024) {
             const hipError_t e = hipFuncSetAttribute((const void *)g.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                     (int)g.lds);
+                                                     (int)lds_most);
             if (e != hipSuccess)
                 return bad(YTRC_EDEVICE, std::string("ytrc_batch_create: hipFuncSetAttribute: ") + hipGetErrorString(e));
         }
@@ -1624,15 +1632,17 @@ int enqueue_persistent(ytrc_batch *b, hipStream_t s)
     const int threads = 64 * b->waves_per_group;
     for (auto &g : b->groups) {
         if (g.n_items == 0) continue;
+        // mode bit 4: a 512-byte vertex-block buffer per wave after the tables
+        const size_t lds = g.lds + ((g.mode & 16) ? (size_t)b->waves_per_group * VBP * sizeof(float) : 0);
         int per_cu = b->groups_per_cu;
         if (per_cu == 0) {
             int nb = 0;
-            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, g.fn, threads, g.lds));
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, g.fn, threads, lds));
             per_cu = std::max(1, nb);
         }
         g.grid = std::max(1, std::min(per_cu * b->ctx->n_cu,
                                       (g.n_items + b->waves_per_group - 1) / b->waves_per_group));
-        hipLaunchKernelGGL(g.fn, dim3(g.grid), dim3(threads), g.lds, s, (const DImg *)b->d_imgs,
+        hipLaunchKernelGGL(g.fn, dim3(g.grid), dim3(threads), lds, s, (const DImg *)b->d_imgs,
                            g.items, g.n_items, b->counters + (size_t)g.counter * QSPLIT, b->tf_all + g.tf_base,
                            g.tf_bins);
         HIPCHK(hipGetLastError());

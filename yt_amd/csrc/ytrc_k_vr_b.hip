@@ -48,6 +48,8 @@ pfn_t pick_persist_vr_b(int nfp, bool rec, int mode)
     switch (mode) {
     case 2: return variant_p<1, 2>(nfp, rec);
     case 3: return variant_p<1, 3>(nfp, rec);
+    // mode 3 with vertex blocks staged in LDS (MODE bit 4): one field, no record
+    case 19: return (nfp == 1 && !rec) ? k_persist<1, 1, false, 19> : nullptr;
     default: return nullptr;
     }
 }

@@ -785,14 +785,22 @@ __device__ __forceinline__ double min_f64(double a, double b)
 //   (s d)^2 (fxy P0 P1 + fxz P0 P2 + fyz P1 P2) + (s d)^3 D P0 P1 P2) + qc1;
 // * the axis-0/1 choice and the axis-2 step as nested branches, so that
 //   the execution masks come from the two comparisons.
-template <int NS>
+// VLDS (MODE bit 4, YTRC_BATCH_VERTEX_LDS) stages vertex blocks in LDS:
+// each wave owns a 512-byte buffer (vbuf) holding one block of the brick.
+// At a visit where every active lane is in the same block, the wave reads
+// that block into vbuf once (the active lanes share its 64 8-byte pieces)
+// and takes the 8 corners from LDS while it stays there; at other visits
+// the lanes read their corners through the buffer resource.  The same
+// values either way.
+template <int NS, bool VLDS = false>
 __device__ __forceinline__ void walk_vr_ctf_fast(Walk &w, double max_t, const BrickReg &b,
                                                  __amdgpu_buffer_rsrc_t rsrc, unsigned K0, unsigned K1,
                                                  const uint8_t *mask, const double (&pa)[3],
                                                  const double (&pb)[3], const TFCache &tf,
                                                  const float4 *__restrict__ lt, double inv_ns, int ns,
-                                                 double (&st)[4])
+                                                 double (&st)[4], float *vbuf = nullptr)
 {
+    unsigned staged_at = 0xffffffffu;  // byte offset of the block in vbuf (wave-uniform)
     double t0 = w.tmax[0], t1 = w.tmax[1], t2 = w.tmax[2];
     int c0 = w.cur[0], c1 = w.cur[1], c2 = w.cur[2];
     double en = w.enter_t;
@@ -831,10 +839,46 @@ __device__ __forceinline__ void walk_vr_ctf_fast(Walk &w, double max_t, const Br
             int mval = 1;
             if (mask != nullptr) mval = mask[(int)__umul24(__umul24(c0, D1) + c1, D2) + c2];
             float cv[8];
-            BufLoad<2>::get(rsrc, o, cv);
-            BufLoad<2>::get(rsrc, o + 4u * vcorner(0, 1, 0), cv + 2);
-            BufLoad<2>::get(rsrc, o + 4u * vcorner(1, 0, 0), cv + 4);
-            BufLoad<2>::get(rsrc, o + 4u * vcorner(1, 1, 0), cv + 6);
+            bool from_lds = false;
+            if constexpr (VLDS) {
+                // the block's byte offset and the corner's offset inside it
+                const unsigned ib = __umul24((unsigned)c0 & 3u, VW0) +
+                                    (__umul24((unsigned)c1 & 3u, VW1) + (((unsigned)c2 & 3u) << 2));
+                const unsigned bs = o - ib;
+                const unsigned bs0 = (unsigned)__builtin_amdgcn_readfirstlane((int)bs);
+                if (__all(bs == bs0)) {
+                    if (bs0 != staged_at) {
+                        const uint64_t act = __ballot(1);
+                        const int nact = __popcll(act);
+                        const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(act >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)act, 0u));
+                        for (int k = rank; k < 64; k += nact) {
+                            float v2[2];
+                            BufLoad<2>::get(rsrc, bs0 + 8u * (unsigned)k, v2);
+                            vbuf[2 * k] = v2[0];
+                            vbuf[2 * k + 1] = v2[1];
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                        staged_at = bs0;
+                    }
+                    const float *q = vbuf + (ib >> 2);
+                    cv[0] = q[vcorner(0, 0, 0)];
+                    cv[1] = q[vcorner(0, 0, 1)];
+                    cv[2] = q[vcorner(0, 1, 0)];
+                    cv[3] = q[vcorner(0, 1, 1)];
+                    cv[4] = q[vcorner(1, 0, 0)];
+                    cv[5] = q[vcorner(1, 0, 1)];
+                    cv[6] = q[vcorner(1, 1, 0)];
+                    cv[7] = q[vcorner(1, 1, 1)];
+                    from_lds = true;
+                }
+            }
+            if (!from_lds) {
+                BufLoad<2>::get(rsrc, o, cv);
+                BufLoad<2>::get(rsrc, o + 4u * vcorner(0, 1, 0), cv + 2);
+                BufLoad<2>::get(rsrc, o + 4u * vcorner(1, 0, 0), cv + 4);
+                BufLoad<2>::get(rsrc, o + 4u * vcorner(1, 1, 0), cv + 6);
+            }
             __builtin_amdgcn_sched_barrier(0);
             const double d = ex - en;
             const double tt = __builtin_fma(d, hns, en);
@@ -1057,7 +1101,7 @@ struct LightSampler {
 // and `tile` are wave-uniform.
 template <int KIND, int NFP, bool REC, int MODE>
 __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
-                                            const float4 *lt)
+                                            const float4 *lt, float *vbuf = nullptr)
 {
     const int tsh = im.tsh;
     const int tw = 1 << tsh, th = 64 >> tsh;
@@ -1275,10 +1319,10 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
                                         (unsigned)w.cur[2] < (unsigned)s.b.d2;
                     const uint64_t k1 = vk1_bytes((unsigned)s.b.vnbk);
                     if (bytes < (1ull << 32) && k0 < (1u << 24) && k1 < (1u << 24) && __all(inside)) {
-                        walk_vr_ctf_fast<(MODE & 2) ? 10 : 0>(
+                        walk_vr_ctf_fast<(MODE & 2) ? 10 : 0, (MODE & 16) != 0>(
                             w, max_t, s.b, brick_rsrc(s.data, (unsigned)bytes), (unsigned)k0,
                             (unsigned)k1, s.mask, s.pa, s.pb, tfc, lt,
-                            s.inv_ns, s.n_samples, s.st);
+                            s.inv_ns, s.n_samples, s.st, vbuf);
                         fastw = true;
                     }
                 }
@@ -1379,7 +1423,7 @@ constexpr int QSPLIT = 8;
 //   90 VGPRs give.
 template <int KIND, int NFP, bool REC, int MODE>
 struct MinWaves {
-    static constexpr int value = (KIND == 1 && NFP == 1 && !REC && (MODE == 1 || MODE == 3)) ? 6
+    static constexpr int value = (KIND == 1 && NFP == 1 && !REC && (MODE == 1 || MODE == 3 || MODE == 19)) ? 6
                                  : (KIND == 0 && NFP == 1 && !REC)                          ? 6
                                                                                              : 1;
 };
@@ -1397,6 +1441,8 @@ void k_persist(const DImg *__restrict__ imgs,
         __syncthreads();
     }
     const int lane = threadIdx.x & 63;
+    // MODE bit 4: this wave's 512-byte vertex-block buffer, after the tables
+    float *vbuf = (MODE & 16) ? (float *)(s_tf + tf_bins) + (threadIdx.x >> 6) * VBP : nullptr;
     // The item list is split into QSPLIT contiguous ranges with a counter
     // each.  Workgroups are dispatched round-robin over the XCDs, so
     // blockIdx.x % QSPLIT names this workgroup's XCD: it drains the range of
@@ -1432,7 +1478,7 @@ void k_persist(const DImg *__restrict__ imgs,
         // reads queue on the PCIe link (the projection kernel took 2x and
         // 1.1x its time).
         if ((taken++ & 15) == 15 && uni((int)stop_requested(im.stop))) break;
-        for (int t = 0; t < cnt; t++) render_tile<KIND, NFP, REC, MODE>(im, tile + t, lane, s_tf + im.tf_slot);
+        for (int t = 0; t < cnt; t++) render_tile<KIND, NFP, REC, MODE>(im, tile + t, lane, s_tf + im.tf_slot, vbuf);
     }
 }
 
