@@ -1410,14 +1410,18 @@ __global__ __launch_bounds__(256) void k_render(const DImg *__restrict__ imgs, i
 constexpr int QSPLIT = 8;
 
 // Persistent launch: every wave repeatedly takes the next (image, tile)
-// item of a global queue until the queue is drained.  All transfer-function
-// tables of the batch sit in LDS; each image reads its own slot.  The item
+// item of a global queue until the queue is drained.  The launch's
+// transfer-function tables (tf_all, tf_bins: those of its requests) sit in
+// LDS, each image at its own slot, followed with MODE bit 4 by one
+// 512-byte vertex-block buffer per wave.  The item
 // index is made wave-uniform so that image, brick and table descriptors are
 // read with scalar loads.
 // Minimum waves per SIMD requested from the register allocator, 6 (80
 // VGPRs) for the production variants:
-// - volume render, one field, ColorTransferFunction path, no record/ERT: a
-//   32-byte spill; 0.8 % faster than 5 waves (95 VGPRs, no spill), 2.5 %
+// - volume render, one field, ColorTransferFunction path, no record/ERT
+//   (with or without vertex blocks in LDS): spills outside the visit loop;
+//   with the fast walk 2.3 % faster than 5 waves and within 0.5 % of 7;
+//   before it, 0.8 % faster than 5 waves (95 VGPRs, no spill), 2.5 %
 //   faster than 4, and 7 waves (72 VGPRs, 68-byte spill) 0.5 % slower;
 // - projection, one field, no record: 3.6 % faster than the 5 waves its
 //   90 VGPRs give.
