@@ -376,3 +376,51 @@ def test_autotune_descent_finds_a_separable_optimum(monkeypatch, tmp_path):
     assert gbest == target and len(gtable) == 3 * 4 * 4 * 2 * 3 * 5
     with pytest.raises(ValueError):
         autotune.tune(Ctx(), [], use_cache=False, search="random")
+
+
+def test_autotune_shares_timings_of_identical_launch_plans(monkeypatch):
+    """Table-size values that give the same launches (the batch's launch plan
+    has as many launches) share one batch and one timing; values that split
+    a launch, and the vertex-block knob, are timed on batches of their own."""
+    from yt_amd import _lib, autotune
+
+    made, timed = [], []
+
+    class FakeBatch:
+        def __init__(self, ctx, specs, flags, tile=(8, 8), item_tiles=1, tf_lds="all", vertex_lds=False):
+            self.key = (tuple(tile), item_tiles, tf_lds, vertex_lds)
+            self.wpg = 4
+            made.append(self.key)
+
+        def launch_plan(self):
+            # two distinct tables of 12 KB: split by "16k" and "one", not by "32k"
+            return (3 if self.key[2] in ("16k", "one") else 2), 24 << 10
+
+        def set_launch(self, wpg, gpc):
+            self.wpg = wpg
+
+        def run(self):
+            timed.append((self.key, self.wpg))
+            return 10.0 + (0.5 if self.key[2] in ("16k", "one") else 0.0) + 0.1 * self.wpg
+
+        def close(self):
+            pass
+
+    class Ctx:
+        device_name = "fake"
+
+    monkeypatch.setattr(_lib, "Batch", FakeBatch)
+    best, table = autotune.tune(Ctx(), [], tiles=((8, 8),), item_tiles=(1,), waves_per_group=(1, 4),
+                                groups_per_cu=(0,), use_cache=False, search="grid", reps=1)
+    assert len(table) == 4 * 2 * 2
+    by = {(c["tf_lds"], c["vertex_lds"], c["waves_per_group"]): ms for c, ms in table}
+    for v in (False, True):
+        for w in (1, 4):
+            assert by[("32k", v, w)] == by[("all", v, w)]
+            assert by[("one", v, w)] == by[("16k", v, w)]
+            assert by[("16k", v, w)] != by[("all", v, w)]
+    # batches: one per (tf_lds, vertex_lds) built, the aliases closed; timings:
+    # one per distinct plan x vertex knob x waves, each a warm run plus one
+    assert len(made) == 8
+    assert len(set(timed)) == 2 * 2 * 2 and len(timed) == 2 * len(set(timed))
+    assert best["tf_lds"] == "all" and best["waves_per_group"] == 1
