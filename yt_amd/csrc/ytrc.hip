@@ -273,6 +273,9 @@ struct Dataset {
 struct ytrc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // a second stream (default priority) for the concurrent launches of a
+    // persistent batch, joined back to `stream` by the batch's events
+    hipStream_t stream2 = nullptr;
     uint64_t budget = 0;
     uint64_t used = 0;
     int n_cu = 0;
@@ -337,6 +340,7 @@ struct ytrc_batch {
     char *arena = nullptr;
     uint64_t arena_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;  // persistent: the second stream's fork and join
     std::vector<hipEvent_t> ev_req;  // plain mode: before/after each request
     std::vector<PGroup> groups;      // persistent mode
     unsigned int *counters = nullptr;
@@ -628,6 +632,11 @@ int ytrc_ctx_create(int32_t device, uint64_t budget_bytes, ytrc_ctx **out)
         delete c;
         return fail(YTRC_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
+    if ((e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking)) != hipSuccess) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return fail(YTRC_EDEVICE, std::string("ytrc_ctx_create: second stream: ") + hipGetErrorString(e));
+    }
     if (scratch_setup(c) != YTRC_OK) {
         hipStreamDestroy(c->stream);
         delete c;
@@ -647,6 +656,7 @@ void ytrc_ctx_destroy(ytrc_ctx *ctx)
         hipFree(kv.second.d_pool);
         if (kv.second.d_masks) hipFree(kv.second.d_masks);
     }
+    hipStreamDestroy(ctx->stream2);
     hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1369,6 +1379,8 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
         if (B->h_stop) hipHostFree(B->h_stop);
         if (B->ev0) hipEventDestroy(B->ev0);
         if (B->ev1) hipEventDestroy(B->ev1);
+        if (B->ev_fork) hipEventDestroy(B->ev_fork);
+        if (B->ev_join) hipEventDestroy(B->ev_join);
         for (auto &ev : B->ev_req)
             if (ev) hipEventDestroy(ev);
         delete B;
@@ -1570,7 +1582,11 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
     if (e != hipSuccess) return bad(YTRC_EDEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
     if ((e = hipEventCreate(&B->ev0)) != hipSuccess || (e = hipEventCreate(&B->ev1)) != hipSuccess)
         return bad(YTRC_EDEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e));
-    if (!(batch_flags & YTRC_BATCH_PERSISTENT)) {
+    if (batch_flags & YTRC_BATCH_PERSISTENT) {
+        if ((e = hipEventCreateWithFlags(&B->ev_fork, hipEventDisableTiming)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&B->ev_join, hipEventDisableTiming)) != hipSuccess)
+            return bad(YTRC_EDEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e));
+    } else {
         B->ev_req.assign(2 * n_req, nullptr);
         for (auto &ev : B->ev_req)
             if ((e = hipEventCreate(&ev)) != hipSuccess)
@@ -1630,8 +1646,28 @@ int enqueue_persistent(ytrc_batch *b, hipStream_t s)
                        ng * QSPLIT);
     HIPCHK(hipGetLastError());
     const int threads = 64 * b->waves_per_group;
-    for (auto &g : b->groups) {
-        if (g.n_items == 0) continue;
+    // The launches run concurrently: the one with the most estimated work
+    // (tiles x a per-tile cost by kind) first on `s`, the others after it
+    // on the second stream, joined back to `s`.  The first launch's
+    // workgroups take the GPU; the others' fill it as the first one's waves
+    // finish, so its tail (the longest) is not left idle.
+    std::vector<int> order;
+    for (int i = 0; i < (int)b->groups.size(); i++)
+        if (b->groups[i].n_items > 0) order.push_back(i);
+    auto work = [&](int i) {
+        const PGroup &g = b->groups[i];
+        const double per_tile = g.kind == YTRC_PROJECTION ? 1.0 : (g.kind == YTRC_VOLUME_RENDER ? 5.0 : 4.0);
+        return per_tile * (double)g.n_items;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+    const bool fork = order.size() >= 2;
+    if (fork) {
+        HIPCHK(hipEventRecord(b->ev_fork, s));
+        HIPCHK(hipStreamWaitEvent(b->ctx->stream2, b->ev_fork, 0));
+    }
+    for (size_t k = 0; k < order.size(); k++) {
+        PGroup &g = b->groups[order[k]];
+        hipStream_t gs = (k == 0) ? s : b->ctx->stream2;
         // mode bit 4: a 512-byte vertex-block buffer per wave after the tables
         const size_t lds = g.lds + ((g.mode & 16) ? (size_t)b->waves_per_group * VBP * sizeof(float) : 0);
         int per_cu = b->groups_per_cu;
@@ -1642,10 +1678,14 @@ int enqueue_persistent(ytrc_batch *b, hipStream_t s)
         }
         g.grid = std::max(1, std::min(per_cu * b->ctx->n_cu,
                                       (g.n_items + b->waves_per_group - 1) / b->waves_per_group));
-        hipLaunchKernelGGL(g.fn, dim3(g.grid), dim3(threads), lds, s, (const DImg *)b->d_imgs,
+        hipLaunchKernelGGL(g.fn, dim3(g.grid), dim3(threads), lds, gs, (const DImg *)b->d_imgs,
                            g.items, g.n_items, b->counters + (size_t)g.counter * QSPLIT, b->tf_all + g.tf_base,
                            g.tf_bins);
         HIPCHK(hipGetLastError());
+    }
+    if (fork) {
+        HIPCHK(hipEventRecord(b->ev_join, b->ctx->stream2));
+        HIPCHK(hipStreamWaitEvent(s, b->ev_join, 0));
     }
     return YTRC_OK;
 }
@@ -1950,6 +1990,8 @@ void ytrc_batch_destroy(ytrc_batch *b)
     }
     if (b->ev0) hipEventDestroy(b->ev0);
     if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->ev_fork) hipEventDestroy(b->ev_fork);
+    if (b->ev_join) hipEventDestroy(b->ev_join);
     for (auto &e : b->ev_req) hipEventDestroy(e);
     if (b->gexec) hipGraphExecDestroy(b->gexec);
     if (b->graph) hipGraphDestroy(b->graph);
