@@ -30,7 +30,7 @@ NORMALS = [(0.3, 0.5, 0.81), (-0.6, 0.2, 0.77), (0.1, -0.9, 0.42), (0.7, 0.7, -0
 @pytest.fixture(scope="module")
 def big(ctx):
     ds = uniform_lognormal(128, seed=401, nprocs=8)
-    vol = render.ResidentVolume(ctx, ds, vr_field="density")
+    vol = render.ResidentVolume(ctx, ds, proj_fields="density", vr_field="density")
     tf = vr_tf_for(ds, nbins=256)
     specs = [vol.vr_request(c0_params(1024, normal=n), tf) for n in NORMALS]
     yield vol, specs, ds, tf
@@ -55,6 +55,31 @@ def test_cancel_stops_a_persistent_launch(ctx, big):
         again_ms = b.run()
         print(f"full launch {full_ms:.2f} ms, again {again_ms:.2f} ms", flush=True)
         for i in range(len(specs)):
+            assert np.array_equal(b.read_image(i), want[i]), i
+    finally:
+        b.close()
+
+
+def test_cancel_stops_concurrent_launches(ctx, big):
+    """A batch of volume renders and projections runs its two launches
+    concurrently on two streams; a stop request ends both, sync() reports
+    it, and a later launch renders the same images."""
+    vol, specs, _, _ = big
+    mixed = []
+    for i, s in enumerate(specs):
+        mixed.append(s)
+        mixed.append(vol.projection_request(c0_params(1024, normal=NORMALS[i])))
+    b = _lib.Batch(ctx, mixed, _lib.BATCH_PERSISTENT | _lib.BATCH_GRAPH)
+    try:
+        assert b.launch_plan()[0] == 2
+        b.run()
+        want = [b.read_image(i) for i in range(len(mixed))]
+        b.launch()
+        b.cancel()
+        with pytest.raises(_lib.YtrcCancelled):
+            b.sync()
+        b.run()
+        for i in range(len(mixed)):
             assert np.array_equal(b.read_image(i), want[i]), i
     finally:
         b.close()
