@@ -1649,8 +1649,9 @@ int enqueue_persistent(ytrc_batch *b, hipStream_t s)
     // The launches run concurrently: the one with the most estimated work
     // (tiles x a per-tile cost by kind) first on `s`, the others after it
     // on the second stream, joined back to `s`.  The first launch's
-    // workgroups take the GPU; the others' fill it as the first one's waves
-    // finish, so its tail (the longest) is not left idle.
+    // workgroups take the GPU but one per CU; the others' start in that room
+    // and fill the GPU as the first one's waves finish, so its tail (the
+    // longest) is not left idle.
     std::vector<int> order;
     for (int i = 0; i < (int)b->groups.size(); i++)
         if (b->groups[i].n_items > 0) order.push_back(i);
@@ -1674,7 +1675,7 @@ int enqueue_persistent(ytrc_batch *b, hipStream_t s)
         if (per_cu == 0) {
             int nb = 0;
             HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, g.fn, threads, lds));
-            per_cu = std::max(1, nb);
+            per_cu = std::max(1, (fork && k == 0) ? nb - 1 : nb);
         }
         g.grid = std::max(1, std::min(per_cu * b->ctx->n_cu,
                                       (g.n_items + b->waves_per_group - 1) / b->waves_per_group));
