@@ -651,6 +651,7 @@ void ytrc_ctx_destroy(ytrc_ctx *ctx)
     if (ctx == nullptr) return;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    hipStreamSynchronize(ctx->stream2);
     for (auto &kv : ctx->datasets) {
         hipFree(kv.second.d_bricks);
         hipFree(kv.second.d_pool);
@@ -890,6 +891,7 @@ int ytrc_dataset_free(ytrc_ctx *ctx, int32_t ds_id)
                                     " live batch(es) or server request(s); destroy or read them first");
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream2));
     hipFree(it->second.d_bricks);
     hipFree(it->second.d_pool);
     if (it->second.d_masks) hipFree(it->second.d_masks);
@@ -2278,7 +2280,7 @@ std::vector<int64_t> srv_fill_order(ytrc_server *S)
 // Launches chunks while fewer than max_inflight are on the GPU.
 int srv_pump(ytrc_server *S)
 {
-    hipStream_t s = S->ctx->stream;
+    hipStream_t s0 = S->ctx->stream;
     while ((int)S->inflight.size() < S->max_inflight && !S->fifo.empty()) {
         int ci = -1;
         for (int i = 0; i < SRV_CHUNKS; i++)
@@ -2288,6 +2290,11 @@ int srv_pump(ytrc_server *S)
             }
         if (ci < 0) break;
         SrvChunk &C = S->chunks[ci];
+        // Consecutive chunks alternate between the context's two streams, so
+        // that a chunk's launches start in the tail of the one before it.
+        // Chunks still retire in launch order (srv_poll), so a request is
+        // finished only once every chunk that holds its tiles has retired.
+        hipStream_t s = (S->chunks_launched & 1) ? S->ctx->stream2 : s0;
         C.finishing.clear();
         const int nv = (int)S->vkeys.size();
         std::vector<std::vector<int2>> per(nv);
@@ -2670,6 +2677,7 @@ void ytrc_server_destroy(ytrc_server *S)
     if (S == nullptr) return;
     hipSetDevice(S->ctx->device);
     hipStreamSynchronize(S->ctx->stream);
+    hipStreamSynchronize(S->ctx->stream2);
     hipFree(S->arena);
     hipFree(S->d_slots);
     hipFree(S->d_tf);
