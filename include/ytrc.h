@@ -243,7 +243,10 @@ typedef struct {
  * stages every descriptor and output buffer in device memory. */
 int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
                       uint32_t batch_flags, ytrc_batch **out);
-/* Enqueues the batch's kernels on the context stream; returns immediately. */
+/* Enqueues the batch's kernels on the context stream; returns immediately.
+ * A persistent batch of several launches runs all but the heaviest on the
+ * context's second stream, joined back into the first before the launch's
+ * end event. */
 int ytrc_batch_launch(ytrc_batch *b);
 /* Waits for the last launch; ms_out (may be NULL) gets the GPU time between
  * the HIP events recorded around it on the same stream. */
@@ -316,7 +319,8 @@ void ytrc_batch_destroy(ytrc_batch *b);
  * are on the GPU and runs the 8x8 ray tiles of every admitted request, in
  * arrival order (see ytrc_server_set_policy), in chunks of `chunk_tiles`
  * persistent-kernel work items
- * (up to two chunks in flight).  `budget_bytes` of device memory is
+ * (up to two chunks in flight, consecutive chunks on the context's two
+ * streams; they retire in launch order).  `budget_bytes` of device memory is
  * allocated once; per-request descriptors and output images come from it.
  * submit returns YTRC_EBUDGET when the request does not fit now (no free
  * slot or budget): the caller retries after reading finished requests; and
