@@ -503,8 +503,8 @@ def main():
                     help="only run the HBM copy probe on this many GiB and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(every GPU box here has one GPU); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the renderer has no multi-GPU path "
+              "(how it would shard is described, not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from yt_amd import _lib, autotune, render
