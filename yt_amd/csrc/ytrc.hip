@@ -1537,7 +1537,7 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
         gitems.push_back(std::move(items));
     }
     const size_t counters_off = off;
-    off = align_up(off + sizeof(unsigned int) * QSPLIT * std::max<size_t>(1, B->groups.size()), 256);
+    off = align_up(off + sizeof(unsigned int) * QCOUNTERS * std::max<size_t>(1, B->groups.size()), 256);
     const size_t imgs_off = off;
     off = align_up(off + sizeof(DImg) * n_req, 256);
     if (ctx->budget != 0 && ctx->used + off > ctx->budget)
@@ -1644,8 +1644,8 @@ namespace {
 int enqueue_persistent(ytrc_batch *b, hipStream_t s)
 {
     const int ng = (int)b->groups.size();
-    hipLaunchKernelGGL(k_zero_counters, dim3((ng * QSPLIT + 255) / 256), dim3(256), 0, s, b->counters,
-                       ng * QSPLIT);
+    hipLaunchKernelGGL(k_zero_counters, dim3((ng * QCOUNTERS + 255) / 256), dim3(256), 0, s, b->counters,
+                       ng * QCOUNTERS);
     HIPCHK(hipGetLastError());
     const int threads = 64 * b->waves_per_group;
     // The launches run concurrently: the one with the most estimated work
@@ -1682,7 +1682,7 @@ int enqueue_persistent(ytrc_batch *b, hipStream_t s)
         g.grid = std::max(1, std::min(per_cu * b->ctx->n_cu,
                                       (g.n_items + b->waves_per_group - 1) / b->waves_per_group));
         hipLaunchKernelGGL(g.fn, dim3(g.grid), dim3(threads), lds, gs, (const DImg *)b->d_imgs,
-                           g.items, g.n_items, b->counters + (size_t)g.counter * QSPLIT, b->tf_all + g.tf_base,
+                           g.items, g.n_items, b->counters + (size_t)g.counter * QCOUNTERS, b->tf_all + g.tf_base,
                            g.tf_bins);
         HIPCHK(hipGetLastError());
     }
@@ -1770,7 +1770,7 @@ int ytrc_render_noalloc(ytrc_ctx *ctx, const ytrc_request *req, double *image_ou
     const size_t items_off = off;
     off = align_up(off + sizeof(int2) * (size_t)std::max(1, P.q.ntiles), 256);
     const size_t cnt_off = off;
-    off = align_up(off + sizeof(unsigned int) * QSPLIT, 256);
+    off = align_up(off + sizeof(unsigned int) * QCOUNTERS, 256);
     const size_t img_off = off;
     off = align_up(off + sizeof(DImg), 256);
     if (off > (size_t)YTRC_NOALLOC_SCRATCH_BYTES)
@@ -1800,7 +1800,7 @@ int ytrc_render_noalloc(ytrc_ctx *ctx, const ytrc_request *req, double *image_ou
     hipStream_t s = ctx->stream;
     unsigned int *counters = (unsigned int *)(S + cnt_off);
     // the array keeps the previous call's counters: clear them
-    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(256), 0, s, counters, QSPLIT);
+    hipLaunchKernelGGL(k_zero_counters, dim3((QCOUNTERS + 255) / 256), dim3(256), 0, s, counters, QCOUNTERS);
     HIPCHK(hipGetLastError());
     if (P.q.ntiles > 0) {
         const int wpg = 4, threads = 64 * wpg;
@@ -2326,8 +2326,8 @@ int srv_pump(ytrc_server *S)
             for (auto &it : per[v]) C.h_items[n++] = it;
         }
         HIPCHK(hipMemcpyAsync(C.d_items, C.h_items, sizeof(int2) * n, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_zero_counters, dim3((SRV_MAX_VARIANTS * QSPLIT + 255) / 256), dim3(256), 0, s,
-                           C.d_counters, SRV_MAX_VARIANTS * QSPLIT);
+        hipLaunchKernelGGL(k_zero_counters, dim3((SRV_MAX_VARIANTS * QCOUNTERS + 255) / 256), dim3(256), 0, s,
+                           C.d_counters, SRV_MAX_VARIANTS * QCOUNTERS);
         HIPCHK(hipGetLastError());
         for (int v = 0; v < nv; v++) {
             if (vcnt[v] == 0) continue;
@@ -2337,7 +2337,7 @@ int srv_pump(ytrc_server *S)
             HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, S->vfns[v], 256, lds));
             const int grid = std::max(1, std::min(std::max(1, nb) * S->ctx->n_cu, (vcnt[v] + 3) / 4));
             hipLaunchKernelGGL(S->vfns[v], dim3(grid), dim3(256), lds, s, (const DImg *)S->d_slots,
-                               (const int2 *)(C.d_items + voff[v]), vcnt[v], C.d_counters + (size_t)v * QSPLIT,
+                               (const int2 *)(C.d_items + voff[v]), vcnt[v], C.d_counters + (size_t)v * QCOUNTERS,
                                (const float4 *)S->d_tf, vr ? S->tf_hi : 0);
             HIPCHK(hipGetLastError());
         }
@@ -2425,7 +2425,7 @@ int ytrc_server_create(ytrc_ctx *ctx, int32_t max_requests, uint64_t budget_byte
         if (e != hipSuccess) return bad(e, "hipHostMalloc(items)");
         e = hipMalloc(&c.d_items, sizeof(int2) * chunk_tiles);
         if (e != hipSuccess) return bad(e, "hipMalloc(items)");
-        e = hipMalloc(&c.d_counters, sizeof(unsigned int) * SRV_MAX_VARIANTS * QSPLIT);
+        e = hipMalloc(&c.d_counters, sizeof(unsigned int) * SRV_MAX_VARIANTS * QCOUNTERS);
         if (e != hipSuccess) return bad(e, "hipMalloc(counters)");
         e = hipEventCreateWithFlags(&c.done, hipEventDisableTiming);
         if (e != hipSuccess) return bad(e, "hipEventCreate");

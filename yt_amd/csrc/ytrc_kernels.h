@@ -1408,6 +1408,10 @@ __global__ __launch_bounds__(256) void k_render(const DImg *__restrict__ imgs, i
 // Tile-queue partitions of a persistent launch: one per XCD of the MI355X
 // (256 CUs in 8 XCDs of 32, each XCD with its own L2).
 constexpr int QSPLIT = 8;
+// A launch's QSPLIT queue counters sit 256 bytes apart, each on a line of
+// its own, so that the atomics of the eight ranges do not queue on one line.
+constexpr int QSTRIDE = 64;
+constexpr int QCOUNTERS = QSPLIT * QSTRIDE;  // words of counters per launch
 
 // Persistent launch: every wave repeatedly takes the next (image, tile)
 // item of a global queue until the queue is drained.  The launch's
@@ -1460,7 +1464,7 @@ void k_persist(const DImg *__restrict__ imgs,
     int taken = (int)((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & 15);
     while (true) {
         unsigned int it = 0;
-        if (lane == 0) it = atomicAdd(counter + part, 1u);
+        if (lane == 0) it = atomicAdd(counter + part * QSTRIDE, 1u);
         it = (unsigned int)__builtin_amdgcn_readfirstlane((int)__shfl(it, 0));
         const unsigned int lo = (unsigned int)(((int64_t)n_items * part) / QSPLIT);
         const unsigned int hi = (unsigned int)(((int64_t)n_items * (part + 1)) / QSPLIT);
