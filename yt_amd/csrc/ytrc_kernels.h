@@ -196,6 +196,15 @@ struct BufLoad<4> {
     }
 };
 
+// The smaller of two FP64 values that are not NaN, as one instruction
+// (fmin's lowering first quiets each operand).
+__device__ __forceinline__ double min_f64(double a, double b)
+{
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 // A raw buffer resource (gfx9 layout: base, stride 0, byte count, and the
 // third dword's 32-bit data format) over `bytes` bytes at p.  Both are the
 // same in every lane (one brick per wave) and are made scalar, so that the
@@ -244,22 +253,27 @@ __device__ __forceinline__ void walk_projection_fast(Walk &w, double max_t, Proj
             // The reference's axis choice (tmax[0] < tmax[1] ? (tmax[0] <
             // tmax[2] ? 0 : 2) : (tmax[1] < tmax[2] ? 1 : 2)) from two
             // comparisons: m = the first pair's choice, then m against t2.
+            // The chosen tmax is the smallest (the tmax are never NaN), so
+            // m and tm are minima, one instruction each.
             const bool a01 = t0 < t1;
-            const double m = a01 ? t0 : t1;
+            const double m = min_f64(t0, t1);
             const bool bm = m < t2;
             const bool s0 = a01 & bm;
             const bool s1 = !a01 & bm;
-            const double tm = bm ? m : t2;
+            const double tm = min_f64(m, t2);
             // exit_t = fmin(tm, max_t), and the loop's enter_t >= max_t
             // test on the next enter_t (= tm), from one comparison
             const bool ge = tm >= max_t;
-            const double exit_t = ge ? max_t : tm;
+            const double exit_t = min_f64(tm, max_t);
             const unsigned lin = ((unsigned)c0 << 4) + ((unsigned)c1 << 2) + (unsigned)c2;
             const unsigned o = __umul24((unsigned)c0 >> 2, K0) +
                                (__umul24((unsigned)c1 >> 2, K1) + (__umul24((unsigned)c2 >> 2, 60u) + lin));
             off[u] = live ? o * (unsigned)(4 * NFP) : 0xffffffffu;
             // (a lane past its end reads 0; its length stays finite)
             dl[u] = (float)(exit_t - enter_t);
+            // the length is formed here, so that no step's FP64 times stay
+            // live until the round's loads
+            asm volatile("" : "+v"(dl[u]));
             // Step the chosen axis under an execution mask (the empty asm
             // keeps each update a masked region rather than selects).
             if (s0) {
@@ -757,15 +771,6 @@ constexpr unsigned VW0 = 4u * VB1 * VB1, VW1 = 4u * VB1, VW2 = 4u;
 constexpr unsigned VK2 = 4u * (VBP - VB);
 __host__ __device__ constexpr uint64_t vk0_bytes(uint64_t nbj, uint64_t nbk) { return 4u * (VBP * nbj * nbk - VB * VB1 * VB1); }
 __host__ __device__ constexpr uint64_t vk1_bytes(uint64_t nbk) { return 4u * (VBP * nbk - VB * VB1); }
-
-// The smaller of two FP64 values that are not NaN, as one instruction
-// (fmin's lowering first quiets each operand).
-__device__ __forceinline__ double min_f64(double a, double b)
-{
-    double r;
-    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
 
 // VRSampler with a ColorTransferFunction in transmittance form (MODE bit
 // 0), one field and a shared ray direction, walked by walk_loop's DDA, for
