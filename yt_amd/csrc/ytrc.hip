@@ -200,6 +200,16 @@ int variant_mode(int kind, int nfp, int mode)
     return mode;
 }
 
+// LDS bytes a persistent launch gives each wave after its tables (the
+// kernel's per-wave buffer): the vertex-block buffer with mode bit 4, or a
+// projection's table of offset terms (walk_projection_fast).
+size_t persist_wave_lds(int kind, int nfp, bool rec, int mode)
+{
+    if (mode & 16) return VBP * sizeof(float);
+    if (kind == YTRC_PROJECTION && !rec && nfp <= 4) return PTAB_WAVE * sizeof(unsigned);
+    return 0;
+}
+
 pfn_t pick_persist(int kind, int nfp, bool rec, int mode)
 {
     const int m = variant_mode(kind, nfp, mode);
@@ -1519,8 +1529,8 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
         g.tf_base = plan_base[L];
         g.tf_bins = plan_bins[L];
         g.lds = has_tf(g.kind) ? sizeof(float4) * (size_t)g.tf_bins : 0;
-        // with mode bit 4, up to 4 waves' vertex-block buffers follow the tables
-        const size_t lds_most = g.lds + ((g.mode & 16) ? 4 * VBP * sizeof(float) : 0);
+        // up to 4 waves' buffers follow the tables
+        const size_t lds_most = g.lds + 4 * persist_wave_lds(g.kind, g.nfp, g.rec, g.mode);
         if (lds_most > ctx->max_lds)
             return bad(YTRC_EINVAL, "ytrc_batch_create: the distinct transfer functions of one launch need " +
                                         std::to_string(lds_most) + " B of LDS, more than the " +
@@ -1671,8 +1681,8 @@ int enqueue_persistent(ytrc_batch *b, hipStream_t s)
     for (size_t k = 0; k < order.size(); k++) {
         PGroup &g = b->groups[order[k]];
         hipStream_t gs = (k == 0) ? s : b->ctx->stream2;
-        // mode bit 4: a 512-byte vertex-block buffer per wave after the tables
-        const size_t lds = g.lds + ((g.mode & 16) ? (size_t)b->waves_per_group * VBP * sizeof(float) : 0);
+        // each wave's buffer after the tables
+        const size_t lds = g.lds + (size_t)b->waves_per_group * persist_wave_lds(g.kind, g.nfp, g.rec, g.mode);
         int per_cu = b->groups_per_cu;
         if (per_cu == 0) {
             int nb = 0;
@@ -1794,7 +1804,8 @@ int ytrc_render_noalloc(ytrc_ctx *ctx, const ytrc_request *req, double *image_ou
     HIPCHK(hipMemcpy(S + img_off, &d, sizeof(DImg), hipMemcpyHostToDevice));
     const pfn_t fn = pick_persist(req->kind, P.q.nfp, false, P.q.mode);
     if (fn == nullptr) return fail(YTRC_ENOTIMPL, "ytrc_render_noalloc: no kernel for this request");
-    const size_t lds = has_tf(req->kind) ? sizeof(float4) * P.tft.size() : 0;
+    const size_t lds = (has_tf(req->kind) ? sizeof(float4) * P.tft.size() : 0) +
+                       4 * persist_wave_lds(req->kind, P.q.nfp, false, P.q.mode);
     if (lds > 64 * 1024)
         HIPCHK(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipStream_t s = ctx->stream;
@@ -2332,7 +2343,9 @@ int srv_pump(ytrc_server *S)
         for (int v = 0; v < nv; v++) {
             if (vcnt[v] == 0) continue;
             const bool vr = has_tf(std::get<0>(S->vkeys[v]));
-            const size_t lds = vr ? sizeof(float4) * S->tf_hi : 0;
+            const size_t lds = (vr ? sizeof(float4) * S->tf_hi : 0) +
+                               4 * persist_wave_lds(std::get<0>(S->vkeys[v]), std::get<1>(S->vkeys[v]),
+                                                    std::get<2>(S->vkeys[v]) != 0, std::get<3>(S->vkeys[v]));
             int nb = 0;
             HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, S->vfns[v], 256, lds));
             const int grid = std::max(1, std::min(std::max(1, nb) * S->ctx->n_cu, (vcnt[v] + 3) / 4));

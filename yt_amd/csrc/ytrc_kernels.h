@@ -218,6 +218,16 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t brick_rsrc(const float *p, uns
                                              __builtin_amdgcn_readfirstlane((int)bytes), 0x00020000);
 }
 
+// Per-wave LDS table of walk_projection_fast<NFP, true>: the offset terms of
+// up to PTAB cells per axis, each axis's run with PTPAD padding words before
+// and after.  A lane past its ray's end keeps stepping until its round ends
+// (at most PTPAD - 1 more steps), so its indices stay within PTPAD of the
+// brick and its reads within the wave's table.
+constexpr int PTAB = 512;
+constexpr int PTPAD = 12;
+constexpr int PTAB0 = PTPAD, PTAB1 = PTAB + 3 * PTPAD, PTAB2 = 2 * PTAB + 5 * PTPAD;
+constexpr int PTAB_WAVE = 3 * PTAB + 6 * PTPAD;  // words per wave
+
 // walk_projection for the common case, taken per wave and brick when
 // every walking lane starts with all three cell indices inside the brick
 // (so that, stepping one axis at a time and stopping when that axis leaves
@@ -234,17 +244,43 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t brick_rsrc(const float *p, uns
 //   lengths stay finite, so its products are 0);
 // * each round's visits are summed in visit order with fused multiply-adds
 //   and the round's sum is added to the image with Kahan compensation.
-template <int NFP>
+// TAB (persistent launches, brick sides up to PTAB): the offset is a sum
+// of three per-axis terms, 4 NFP ((c0>>2) K0 + 16 c0), 4 NFP ((c1>>2) K1 +
+// 4 c1) and 4 NFP ((c2>>2) 60 + c2), which the wave tabulates in LDS (tab,
+// PTAB_WAVE words) for the brick's sides before it walks; a step then
+// reads the new term of the axis it steps instead of forming the offset.
+template <int NFP, bool TAB = false>
 __device__ __forceinline__ void walk_projection_fast(Walk &w, double max_t, ProjSampler<NFP, false> &S,
-                                                     __amdgpu_buffer_rsrc_t rsrc, unsigned K0, unsigned K1)
+                                                     __amdgpu_buffer_rsrc_t rsrc, unsigned K0, unsigned K1,
+                                                     unsigned *tab = nullptr)
 {
     constexpr int PF = 12;
+    static_assert(PF <= PTPAD, "a round's steps past a ray's end must stay in the table's padding");
     double t0 = w.tmax[0], t1 = w.tmax[1], t2 = w.tmax[2];
     int c0 = w.cur[0], c1 = w.cur[1], c2 = w.cur[2];
     double enter_t = w.enter_t;
     const int st0 = w.step[0], st1 = w.step[1], st2 = w.step[2];
     const double td0 = w.tdelta[0], td1 = w.tdelta[1], td2 = w.tdelta[2];
     bool live = true;
+    unsigned f0 = 0, f1 = 0, f2 = 0;  // (TAB) the three terms at (c0, c1, c2)
+    if constexpr (TAB) {
+        // the active lanes fill the terms of the brick's cells in turn
+        const uint64_t act = __ballot(1);
+        const int nact = __popcll(act);
+        const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(act >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((unsigned)act, 0u));
+        constexpr unsigned sc = 4u * NFP;
+        for (int i = rank; i < S.dims0; i += nact)
+            tab[PTAB0 + i] = (__umul24((unsigned)i >> 2, K0) + ((unsigned)i << 4)) * sc;
+        for (int i = rank; i < S.d1; i += nact)
+            tab[PTAB1 + i] = (__umul24((unsigned)i >> 2, K1) + ((unsigned)i << 2)) * sc;
+        for (int i = rank; i < S.d2; i += nact)
+            tab[PTAB2 + i] = (__umul24((unsigned)i >> 2, 60u) + (unsigned)i) * sc;
+        __builtin_amdgcn_wave_barrier();
+        f0 = tab[PTAB0 + c0];
+        f1 = tab[PTAB1 + c1];
+        f2 = tab[PTAB2 + c2];
+    }
     while (live) {
         unsigned off[PF];
         float dl[PF];
@@ -265,10 +301,17 @@ __device__ __forceinline__ void walk_projection_fast(Walk &w, double max_t, Proj
             // test on the next enter_t (= tm), from one comparison
             const bool ge = tm >= max_t;
             const double exit_t = min_f64(tm, max_t);
-            const unsigned lin = ((unsigned)c0 << 4) + ((unsigned)c1 << 2) + (unsigned)c2;
-            const unsigned o = __umul24((unsigned)c0 >> 2, K0) +
-                               (__umul24((unsigned)c1 >> 2, K1) + (__umul24((unsigned)c2 >> 2, 60u) + lin));
-            off[u] = live ? o * (unsigned)(4 * NFP) : 0xffffffffu;
+            if constexpr (TAB) {
+                off[u] = live ? f0 + f1 + f2 : 0xffffffffu;
+                // formed at its step, like the length: no step's terms stay
+                // live until the round's loads
+                asm volatile("" : "+v"(off[u]));
+            } else {
+                const unsigned lin = ((unsigned)c0 << 4) + ((unsigned)c1 << 2) + (unsigned)c2;
+                const unsigned o = __umul24((unsigned)c0 >> 2, K0) +
+                                   (__umul24((unsigned)c1 >> 2, K1) + (__umul24((unsigned)c2 >> 2, 60u) + lin));
+                off[u] = live ? o * (unsigned)(4 * NFP) : 0xffffffffu;
+            }
             // (a lane past its end reads 0; its length stays finite)
             dl[u] = (float)(exit_t - enter_t);
             // the length is formed here, so that no step's FP64 times stay
@@ -276,21 +319,26 @@ __device__ __forceinline__ void walk_projection_fast(Walk &w, double max_t, Proj
             asm volatile("" : "+v"(dl[u]));
             // Step the chosen axis under an execution mask (the empty asm
             // keeps each update a masked region rather than selects).
+            // (a term read at a step out of the brick lands in the table's
+            // padding and is never used: the lane is past its end)
             if (s0) {
                 asm volatile("");
                 c0 += st0;
                 t0 += td0;
+                if constexpr (TAB) f0 = tab[PTAB0 + c0];
             }
             if (s1) {
                 asm volatile("");
                 c1 += st1;
                 t1 += td1;
+                if constexpr (TAB) f1 = tab[PTAB1 + c1];
             }
             const bool s2 = !(s0 | s1);
             if (s2) {
                 asm volatile("");
                 c2 += st2;
                 t2 += td2;
+                if constexpr (TAB) f2 = tab[PTAB2 + c2];
             }
             const bool out = (s0 & ((unsigned)c0 >= (unsigned)S.dims0)) |
                              (s1 & ((unsigned)c1 >= (unsigned)S.d1)) |
@@ -1106,7 +1154,7 @@ struct LightSampler {
 // and `tile` are wave-uniform.
 template <int KIND, int NFP, bool REC, int MODE>
 __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
-                                            const float4 *lt, float *vbuf = nullptr)
+                                            const float4 *lt, float *wbuf = nullptr)
 {
     const int tsh = im.tsh;
     const int tw = 1 << tsh, th = 64 >> tsh;
@@ -1236,8 +1284,12 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
                                         (unsigned)w.cur[1] < (unsigned)s.d1 &&
                                         (unsigned)w.cur[2] < (unsigned)s.d2;
                     if (bytes < (1ull << 32) && k0 < (1u << 24) && __all(inside)) {
-                        walk_projection_fast<NFP>(w, max_t, s, brick_rsrc(s.data, (unsigned)bytes),
-                                                  (unsigned)k0 - 64u, (unsigned)s.nbk * 64u - 16u);
+                        const __amdgpu_buffer_rsrc_t rs = brick_rsrc(s.data, (unsigned)bytes);
+                        const unsigned K0 = (unsigned)k0 - 64u, K1 = (unsigned)s.nbk * 64u - 16u;
+                        if (wbuf != nullptr && s.dims0 <= PTAB && s.d1 <= PTAB && s.d2 <= PTAB)
+                            walk_projection_fast<NFP, true>(w, max_t, s, rs, K0, K1, (unsigned *)wbuf);
+                        else
+                            walk_projection_fast<NFP>(w, max_t, s, rs, K0, K1);
                         fastw = true;
                     }
                 }
@@ -1327,7 +1379,7 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
                         walk_vr_ctf_fast<(MODE & 2) ? 10 : 0, (MODE & 16) != 0>(
                             w, max_t, s.b, brick_rsrc(s.data, (unsigned)bytes), (unsigned)k0,
                             (unsigned)k1, s.mask, s.pa, s.pb, tfc, lt,
-                            s.inv_ns, s.n_samples, s.st, vbuf);
+                            s.inv_ns, s.n_samples, s.st, wbuf);
                         fastw = true;
                     }
                 }
@@ -1454,8 +1506,11 @@ void k_persist(const DImg *__restrict__ imgs,
         __syncthreads();
     }
     const int lane = threadIdx.x & 63;
-    // MODE bit 4: this wave's 512-byte vertex-block buffer, after the tables
-    float *vbuf = (MODE & 16) ? (float *)(s_tf + tf_bins) + (threadIdx.x >> 6) * VBP : nullptr;
+    // This wave's LDS after the tables: with MODE bit 4 its 512-byte
+    // vertex-block buffer; for projections, its table of offset terms.
+    float *wbuf = (MODE & 16)                             ? (float *)(s_tf + tf_bins) + (threadIdx.x >> 6) * VBP
+                  : (KIND == 0 && !REC && NFP <= 4) ? (float *)(s_tf + tf_bins) + (threadIdx.x >> 6) * PTAB_WAVE
+                                                          : nullptr;
     // The item list is split into QSPLIT contiguous ranges with a counter
     // each.  Workgroups are dispatched round-robin over the XCDs, so
     // blockIdx.x % QSPLIT names this workgroup's XCD: it drains the range of
@@ -1491,7 +1546,7 @@ void k_persist(const DImg *__restrict__ imgs,
         // reads queue on the PCIe link (the projection kernel took 2x and
         // 1.1x its time).
         if ((taken++ & 15) == 15 && uni((int)stop_requested(im.stop))) break;
-        for (int t = 0; t < cnt; t++) render_tile<KIND, NFP, REC, MODE>(im, tile + t, lane, s_tf + im.tf_slot, vbuf);
+        for (int t = 0; t < cnt; t++) render_tile<KIND, NFP, REC, MODE>(im, tile + t, lane, s_tf + im.tf_slot, wbuf);
     }
 }
 
