@@ -200,13 +200,13 @@ int variant_mode(int kind, int nfp, int mode)
     return mode;
 }
 
-// LDS bytes a persistent launch gives each wave after its tables (the
-// kernel's per-wave buffer): the vertex-block buffer with mode bit 4, or a
+// Words of LDS per wave that a persistent launch gives after its tables
+// (k_persist's wave_words): the vertex-block buffer with mode bit 4, a
 // projection's table of offset terms (walk_projection_fast).
-size_t persist_wave_lds(int kind, int nfp, bool rec, int mode)
+int required_wave_words(int kind, int nfp, bool rec, int mode)
 {
-    if (mode & 16) return VBP * sizeof(float);
-    if (kind == YTRC_PROJECTION && !rec && nfp <= 4) return PTAB_WAVE * sizeof(unsigned);
+    if (mode & 16) return VBP;
+    if (kind == YTRC_PROJECTION && !rec && nfp <= 4) return PTAB_WAVE;
     return 0;
 }
 
@@ -1530,7 +1530,7 @@ int ytrc_batch_create(ytrc_ctx *ctx, int32_t n_req, const ytrc_request *reqs,
         g.tf_bins = plan_bins[L];
         g.lds = has_tf(g.kind) ? sizeof(float4) * (size_t)g.tf_bins : 0;
         // up to 4 waves' buffers follow the tables
-        const size_t lds_most = g.lds + 4 * persist_wave_lds(g.kind, g.nfp, g.rec, g.mode);
+        const size_t lds_most = g.lds + 4 * sizeof(unsigned) * required_wave_words(g.kind, g.nfp, g.rec, g.mode);
         if (lds_most > ctx->max_lds)
             return bad(YTRC_EINVAL, "ytrc_batch_create: the distinct transfer functions of one launch need " +
                                         std::to_string(lds_most) + " B of LDS, more than the " +
@@ -1682,7 +1682,8 @@ int enqueue_persistent(ytrc_batch *b, hipStream_t s)
         PGroup &g = b->groups[order[k]];
         hipStream_t gs = (k == 0) ? s : b->ctx->stream2;
         // each wave's buffer after the tables
-        const size_t lds = g.lds + (size_t)b->waves_per_group * persist_wave_lds(g.kind, g.nfp, g.rec, g.mode);
+        const int ww = required_wave_words(g.kind, g.nfp, g.rec, g.mode);
+        const size_t lds = g.lds + (size_t)b->waves_per_group * ww * sizeof(unsigned);
         int per_cu = b->groups_per_cu;
         if (per_cu == 0) {
             int nb = 0;
@@ -1693,7 +1694,7 @@ int enqueue_persistent(ytrc_batch *b, hipStream_t s)
                                       (g.n_items + b->waves_per_group - 1) / b->waves_per_group));
         hipLaunchKernelGGL(g.fn, dim3(g.grid), dim3(threads), lds, gs, (const DImg *)b->d_imgs,
                            g.items, g.n_items, b->counters + (size_t)g.counter * QCOUNTERS, b->tf_all + g.tf_base,
-                           g.tf_bins);
+                           g.tf_bins, ww);
         HIPCHK(hipGetLastError());
     }
     if (fork) {
@@ -1804,8 +1805,9 @@ int ytrc_render_noalloc(ytrc_ctx *ctx, const ytrc_request *req, double *image_ou
     HIPCHK(hipMemcpy(S + img_off, &d, sizeof(DImg), hipMemcpyHostToDevice));
     const pfn_t fn = pick_persist(req->kind, P.q.nfp, false, P.q.mode);
     if (fn == nullptr) return fail(YTRC_ENOTIMPL, "ytrc_render_noalloc: no kernel for this request");
-    const size_t lds = (has_tf(req->kind) ? sizeof(float4) * P.tft.size() : 0) +
-                       4 * persist_wave_lds(req->kind, P.q.nfp, false, P.q.mode);
+    const int wpg = 4, threads = 64 * wpg;
+    const int ww = required_wave_words(req->kind, P.q.nfp, false, P.q.mode);
+    const size_t lds = (has_tf(req->kind) ? sizeof(float4) * P.tft.size() : 0) + (size_t)wpg * ww * sizeof(unsigned);
     if (lds > 64 * 1024)
         HIPCHK(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipStream_t s = ctx->stream;
@@ -1814,13 +1816,12 @@ int ytrc_render_noalloc(ytrc_ctx *ctx, const ytrc_request *req, double *image_ou
     hipLaunchKernelGGL(k_zero_counters, dim3((QCOUNTERS + 255) / 256), dim3(256), 0, s, counters, QCOUNTERS);
     HIPCHK(hipGetLastError());
     if (P.q.ntiles > 0) {
-        const int wpg = 4, threads = 64 * wpg;
         int nb = 0;
         HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, threads, lds));
         const int grid = std::max(1, std::min(std::max(1, nb) * ctx->n_cu, (P.q.ntiles + wpg - 1) / wpg));
         hipLaunchKernelGGL(fn, dim3(grid), dim3(threads), lds, s, (const DImg *)(S + img_off),
                            (const int2 *)(S + items_off), P.q.ntiles, counters,
-                           (const float4 *)(S + tf_off), (int)P.tft.size());
+                           (const float4 *)(S + tf_off), (int)P.tft.size(), ww);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(s));
@@ -2343,15 +2344,15 @@ int srv_pump(ytrc_server *S)
         for (int v = 0; v < nv; v++) {
             if (vcnt[v] == 0) continue;
             const bool vr = has_tf(std::get<0>(S->vkeys[v]));
-            const size_t lds = (vr ? sizeof(float4) * S->tf_hi : 0) +
-                               4 * persist_wave_lds(std::get<0>(S->vkeys[v]), std::get<1>(S->vkeys[v]),
-                                                    std::get<2>(S->vkeys[v]) != 0, std::get<3>(S->vkeys[v]));
+            const int ww = required_wave_words(std::get<0>(S->vkeys[v]), std::get<1>(S->vkeys[v]),
+                                               std::get<2>(S->vkeys[v]) != 0, std::get<3>(S->vkeys[v]));
+            const size_t lds = (vr ? sizeof(float4) * S->tf_hi : 0) + 4 * (size_t)ww * sizeof(unsigned);
             int nb = 0;
             HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, S->vfns[v], 256, lds));
             const int grid = std::max(1, std::min(std::max(1, nb) * S->ctx->n_cu, (vcnt[v] + 3) / 4));
             hipLaunchKernelGGL(S->vfns[v], dim3(grid), dim3(256), lds, s, (const DImg *)S->d_slots,
                                (const int2 *)(C.d_items + voff[v]), vcnt[v], C.d_counters + (size_t)v * QCOUNTERS,
-                               (const float4 *)S->d_tf, vr ? S->tf_hi : 0);
+                               (const float4 *)S->d_tf, vr ? S->tf_hi : 0, ww);
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipEventRecord(C.done, s));

@@ -1154,7 +1154,7 @@ struct LightSampler {
 // and `tile` are wave-uniform.
 template <int KIND, int NFP, bool REC, int MODE>
 __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
-                                            const float4 *lt, float *wbuf = nullptr)
+                                            const float4 *lt, float *wbuf = nullptr, int wave_words = 0)
 {
     const int tsh = im.tsh;
     const int tw = 1 << tsh, th = 64 >> tsh;
@@ -1286,7 +1286,8 @@ __device__ __forceinline__ void render_tile(const DImg &im, int tile, int lane,
                     if (bytes < (1ull << 32) && k0 < (1u << 24) && __all(inside)) {
                         const __amdgpu_buffer_rsrc_t rs = brick_rsrc(s.data, (unsigned)bytes);
                         const unsigned K0 = (unsigned)k0 - 64u, K1 = (unsigned)s.nbk * 64u - 16u;
-                        if (wbuf != nullptr && s.dims0 <= PTAB && s.d1 <= PTAB && s.d2 <= PTAB)
+                        if (wbuf != nullptr && wave_words >= PTAB_WAVE && s.dims0 <= PTAB && s.d1 <= PTAB &&
+                            s.d2 <= PTAB)
                             walk_projection_fast<NFP, true>(w, max_t, s, rs, K0, K1, (unsigned *)wbuf);
                         else
                             walk_projection_fast<NFP>(w, max_t, s, rs, K0, K1);
@@ -1473,8 +1474,8 @@ constexpr int QCOUNTERS = QSPLIT * QSTRIDE;  // words of counters per launch
 // Persistent launch: every wave repeatedly takes the next (image, tile)
 // item of a global queue until the queue is drained.  The launch's
 // transfer-function tables (tf_all, tf_bins: those of its requests) sit in
-// LDS, each image at its own slot, followed with MODE bit 4 by one
-// 512-byte vertex-block buffer per wave.  The item
+// LDS, each image at its own slot, followed by wave_words words per wave
+// (the vertex-block buffer with MODE bit 4, or a walk's offset table).  The item
 // index is made wave-uniform so that image, brick and table descriptors are
 // read with scalar loads.
 // Minimum waves per SIMD requested from the register allocator, 6 (80
@@ -1498,7 +1499,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves<KI
 void k_persist(const DImg *__restrict__ imgs,
                                                  const int2 *__restrict__ items, int n_items,
                                                  unsigned int *__restrict__ counter,
-                                                 const float4 *__restrict__ tf_all, int tf_bins)
+                                                 const float4 *__restrict__ tf_all, int tf_bins, int wave_words)
 {
     extern __shared__ float4 s_tf[];
     if (KIND == 1 || KIND == 3) {
@@ -1506,11 +1507,10 @@ void k_persist(const DImg *__restrict__ imgs,
         __syncthreads();
     }
     const int lane = threadIdx.x & 63;
-    // This wave's LDS after the tables: with MODE bit 4 its 512-byte
-    // vertex-block buffer; for projections, its table of offset terms.
-    float *wbuf = (MODE & 16)                             ? (float *)(s_tf + tf_bins) + (threadIdx.x >> 6) * VBP
-                  : (KIND == 0 && !REC && NFP <= 4) ? (float *)(s_tf + tf_bins) + (threadIdx.x >> 6) * PTAB_WAVE
-                                                          : nullptr;
+    // This wave's wave_words words of LDS after the tables (none if 0): with
+    // MODE bit 4 its 512-byte vertex-block buffer, for a projection its table
+    // of offset terms (walk_projection_fast uses it if it is large enough).
+    float *wbuf = (wave_words > 0) ? (float *)(s_tf + tf_bins) + (threadIdx.x >> 6) * wave_words : nullptr;
     // The item list is split into QSPLIT contiguous ranges with a counter
     // each.  Workgroups are dispatched round-robin over the XCDs, so
     // blockIdx.x % QSPLIT names this workgroup's XCD: it drains the range of
@@ -1546,7 +1546,7 @@ void k_persist(const DImg *__restrict__ imgs,
         // reads queue on the PCIe link (the projection kernel took 2x and
         // 1.1x its time).
         if ((taken++ & 15) == 15 && uni((int)stop_requested(im.stop))) break;
-        for (int t = 0; t < cnt; t++) render_tile<KIND, NFP, REC, MODE>(im, tile + t, lane, s_tf + im.tf_slot, wbuf);
+        for (int t = 0; t < cnt; t++) render_tile<KIND, NFP, REC, MODE>(im, tile + t, lane, s_tf + im.tf_slot, wbuf, wave_words);
     }
 }
 

@@ -7,7 +7,7 @@
 namespace ytrc {
 
 // k_persist<KIND, NFP, REC, MODE> and k_render<KIND, NFP, REC, MODE>.
-typedef void (*pfn_t)(const DImg *, const int2 *, int, unsigned int *, const float4 *, int);
+typedef void (*pfn_t)(const DImg *, const int2 *, int, unsigned int *, const float4 *, int, int);
 typedef void (*kfn_t)(const DImg *, int);
 
 // Each returns the variant for (nfp, rec, mode) within its unit's family,
