@@ -3,8 +3,11 @@ general-walk fallback) on edge geometry, against the FP64 oracle.
 
 Bricks whose sides are not multiples of the 4-cell blocks (12/13 x 18/19 x
 7/8 cells), bricks one cell thick (1 x 9 x 9; projection only: vertex
-centring needs two cells per grid axis, interpolators.pyx:177-226) and the
-kd bricks of an AMR set, whose child masks skip cells; rays along an axis (two
+centring needs two cells per grid axis, interpolators.pyx:177-226), the
+kd bricks of an AMR set, whose child masks skip cells, and one brick of
+512 x 6 x 5 and of 600 x 6 x 5 cells (the persistent projection walk
+tabulates its offset terms in LDS for sides up to 512 cells and forms them
+arithmetically beyond); rays along an axis (two
 direction components 0: tMax 1e60, no steps on those axes), along a face
 diagonal and the body diagonal (ties between two and three axes at every
 corner the rays pass), a camera wider than the volume (lanes that miss it)
@@ -18,7 +21,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle
-from yt_amd import _lib, render
+from yt_amd import _lib, data, render
 
 from .helpers import (c0_params, oracle_proj_bricks, oracle_vr_bricks, small_amr, tf_dict, uniform_lognormal,
                       vr_tf_for)
@@ -48,10 +51,14 @@ def _dataset(name):
         return uniform_lognormal(37, seed=311, psize=(3, 2, 5)), True
     if name == "slabs9":
         return uniform_lognormal(9, seed=312, psize=(9, 1, 1)), False
+    if name.startswith("long"):
+        n0 = int(name[4:])
+        rho = np.random.default_rng(314).lognormal(0.0, 1.0, size=(n0, 6, 5))
+        return data.load_uniform_grid({"density": rho}), True
     return small_amr(root_n=24, levels=2, seed=313), True
 
 
-@pytest.mark.parametrize("dname", ["uniform37", "slabs9", "amr"])
+@pytest.mark.parametrize("dname", ["uniform37", "slabs9", "amr", "long512", "long600"])
 def test_timed_kernels_on_edge_geometry(ctx, dname):
     ds, with_vr = _dataset(dname)
     vol = render.ResidentVolume(ctx, ds, proj_fields="density", vr_field="density" if with_vr else None)
