@@ -424,3 +424,16 @@ def test_autotune_shares_timings_of_identical_launch_plans(monkeypatch):
     assert len(made) == 8
     assert len(set(timed)) == 2 * 2 * 2 and len(timed) == 2 * len(set(timed))
     assert best["tf_lds"] == "all" and best["waves_per_group"] == 1
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and says
+    so, before anything touches a device."""
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    res = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                         text=True, timeout=120, cwd=root)
+    assert res.returncode == 2
+    assert "one GPU" in res.stderr and res.stdout == ""
